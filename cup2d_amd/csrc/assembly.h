@@ -1,0 +1,64 @@
+// assembly.h -- host-side description of the AMR Poisson operator in the
+// layouts the HIP kernels read.
+//
+// The operator is the h^2-scaled finite-volume Laplacian of the reference
+// (source/Poisson/ExpAMRSolver.cpp:180-313).  It is stored two ways:
+//
+//  * stencil form (what the fused solver kernels read): every row whose cell
+//    has no face on a coarse-fine block boundary is the plain 5-point stencil
+//    sum_f (x_nb(f) - x_self) over the non-wall faces, described by one
+//    same-level neighbour id per block face (nbr[4*b+f], -1 = wall, -2 =
+//    coarse-fine).  Rows with a coarse-fine face ("explicit rows") are stored
+//    as a small CSR (exc_*), grouped by block.
+//  * plain CSR (int32 columns) of the whole matrix, for the CSR SpMV kernel
+//    and for the host-side checks.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cup2d {
+
+constexpr int BS = 8;
+constexpr int BLEN = BS * BS;
+
+struct MeshDesc {
+  int bpdx, bpdy, levels;
+  int periodic_x, periodic_y;
+  double extent;
+  long long n_blocks;
+  const int *level, *bx, *by;
+};
+
+struct Operator {
+  long long nb = 0;  // blocks
+  long long n = 0;   // unknowns = 64 nb
+  // stencil form
+  std::vector<int> nbr;          // [4 nb] faces Xm, Xp, Ym, Yp
+  std::vector<uint64_t> exc_mask;  // [nb] bit c set when cell c is an explicit row
+  std::vector<int> exc_bptr;     // [nb+1] explicit rows of block b: exc_bptr[b]..exc_bptr[b+1]
+  std::vector<int> exc_cell;     // [n_exc] cell index (0..63) of each explicit row
+  std::vector<int> exc_rptr;     // [n_exc+1] entry range of each explicit row
+  std::vector<int> exc_col;      // [nnz_exc] global column
+  std::vector<double> exc_val;   // [nnz_exc]
+  // full CSR
+  std::vector<int> rp;           // [n+1]
+  std::vector<int> col;          // [nnz]
+  std::vector<double> val;       // [nnz]
+  // h^2 per block and the mean-constraint row (-1 when unused)
+  std::vector<double> h2;
+  long long mean_row = -1;
+  long long corner_block = -1;   // block with index (0,0), whose cell (0,0) is the mean row
+
+  long long nnz() const { return (long long)col.size(); }
+  long long n_exc() const { return (long long)exc_cell.size(); }
+};
+
+// Builds the operator for the mesh.  Returns false with a message on a mesh
+// the operator cannot be built on (out-of-range or duplicate blocks, a face
+// whose neighbour is neither same-level, one level coarser nor one level
+// finer).
+bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err);
+
+}  // namespace cup2d

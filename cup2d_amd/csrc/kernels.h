@@ -1,0 +1,167 @@
+// kernels.h -- device-side building blocks shared by the solver kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cup2d {
+
+constexpr int WAVES = 4;            // wavefronts per workgroup
+constexpr int WG = 64 * WAVES;      // threads per workgroup
+constexpr int GB = 4;               // blocks per wave in preconditioner passes
+
+// Device view of the operator in stencil form (see assembly.h).
+struct OpView {
+  int nb;
+  int mean_row;                          // -1 when the mean row is not replaced
+  const int4 *nbr;                       // [nb] same-level neighbour per face, -1 wall, -2 coarse-fine
+  const unsigned long long *mask;        // [nb] explicit-row bitmask
+  const int *bptr;                       // [nb+1] explicit rows per block
+  const int *cell;                       // [n_exc] cell of each explicit row
+  const int *rptr;                       // [n_exc+1]
+  const int *col;                        // [nnz_exc]
+  const double *val;                     // [nnz_exc]
+  const double *h2;                      // [nb]
+};
+
+// Solver scalars, resident on the device for the whole solve.  Written only
+// by the last workgroup of a reduction kernel (or the init kernel), read by
+// every kernel at launch.
+struct Scal {
+  double alpha, beta, omega, rho, rho_prev;
+  double err, err0, err_opt;
+  double mean;                           // h^2-weighted mean of the solution
+  double tol_abs, tol_rel, eps, bd_tol;
+  double red[8];                         // last reduction results (diagnostics)
+  int iter, max_iter, restarts, max_restarts;
+  int done, converged, improved, restart;
+  int norm;
+  int pad;
+  unsigned int ticket;                   // arrival counter of the last-workgroup reductions
+  unsigned int pad2;
+};
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Orders the wave's own LDS writes before its later LDS reads (lanes of one
+// wavefront exchange values through LDS without a workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// 5-point stencil row of cell c of block b: sum over non-wall faces of
+// (x_nb - x_c).  xs is the wave's LDS copy of x on block b.
+__device__ __forceinline__ double stencil_row(const double *__restrict__ x, int4 nb, int c,
+                                              const double *xs, double xc) {
+  const int ix = c & 7, iy = c >> 3;
+  double y = 0.0;
+  int deg = 0;
+  if (iy > 0) { y += xs[c - 8]; deg++; } else if (nb.z >= 0) { y += x[(long)nb.z * 64 + 56 + ix]; deg++; }
+  if (ix > 0) { y += xs[c - 1]; deg++; } else if (nb.x >= 0) { y += x[(long)nb.x * 64 + iy * 8 + 7]; deg++; }
+  if (ix < 7) { y += xs[c + 1]; deg++; } else if (nb.y >= 0) { y += x[(long)nb.y * 64 + iy * 8]; deg++; }
+  if (iy < 7) { y += xs[c + 8]; deg++; } else if (nb.w >= 0) { y += x[(long)nb.w * 64 + ix]; deg++; }
+  return y - double(deg) * xc;
+}
+
+// y = A x on block b, one cell per lane.  xs: wave LDS scratch of 64 doubles
+// holding x on the block, ys: a second 64-double scratch.  The mean row, when
+// replaced, is left to the caller.
+__device__ __forceinline__ double apply_block(const OpView &A, const double *__restrict__ x, int b,
+                                              int c, double *xs, double *ys, double xc) {
+  const int4 nb = A.nbr[b];
+  double y = stencil_row(x, nb, c, xs, xc);
+  const unsigned long long m = A.mask[b];
+  if (m) {
+    const int e0 = A.bptr[b], e1 = A.bptr[b + 1];
+    for (int e = e0 + c; e < e1; e += 64) {
+      double acc = 0.0;
+      const int k1 = A.rptr[e + 1];
+      for (int k = A.rptr[e]; k < k1; k++) acc += A.val[k] * x[A.col[k]];
+      ys[A.cell[e]] = acc;
+    }
+    wave_sync();
+    if ((m >> c) & 1ull) y = ys[c];
+    wave_sync();
+  }
+  return y;
+}
+
+// z = P v for GB blocks at once: lane c owns output row c; P (64x64,
+// symmetric) sits in LDS, v holds the GB input vectors (64 each).
+__device__ __forceinline__ void prec_group(const double *Ps, const double *v, int c, double z[GB]) {
+#pragma unroll
+  for (int j = 0; j < GB; j++) z[j] = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < 64; k++) {
+    const double pk = Ps[k * 64 + c];
+#pragma unroll
+    for (int j = 0; j < GB; j++) z[j] = fma(pk, v[j * 64 + k], z[j]);
+  }
+}
+
+// Workgroup reduction of NQ quantities (bit q of MAXQ set: max, else sum),
+// written to part[q*gridDim.x + blockIdx.x]; the last workgroup to arrive
+// reduces all partials in a fixed order and returns true with the totals in
+// out[] (every thread of that workgroup).  Deterministic for a fixed grid.
+template <int NQ, unsigned MAXQ>
+__device__ bool grid_reduce(double (&v)[NQ], double *part, unsigned int *ticket, double (&out)[NQ]) {
+  __shared__ double red[NQ][WAVES];
+  __shared__ double fin[NQ][WAVES];
+  __shared__ int is_last;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) v[q] = ((MAXQ >> q) & 1) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[q][w] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      double a = red[q][0];
+      for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, red[q][i]) : a + red[q][i];
+      part[q * gridDim.x + blockIdx.x] = a;
+    }
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  const int G = gridDim.x;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < G; i += WG) {
+      const double p = __builtin_nontemporal_load(&part[q * G + i]);
+      a = ((MAXQ >> q) & 1) ? fmax(a, p) : a + p;
+    }
+    a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
+    if (lane == 0) fin[q][w] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double a = fin[q][0];
+    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, fin[q][i]) : a + fin[q][i];
+    out[q] = a;
+  }
+  if (threadIdx.x == 0) *ticket = 0u;
+  return true;
+}
+
+}  // namespace cup2d

@@ -1,0 +1,307 @@
+"""Host-side interface of the solver, mirroring the reference's plugin
+boundary for the pressure-Poisson path:
+
+* ``PoissonSolver`` with ``solve(input, output)``      -- source/Poisson/Base.h:8-15
+* ``makePoissonSolver(sim)`` string factory             -- source/Poisson/Base.cpp:8-30
+* the Python binding ``poisson.Solver.solve(input=, output=)`` -- source/Bindings/Poisson.cpp:7-15
+
+``HipPoissonSolver`` plays the role of ``ExpAMRSolver`` (the reference's
+"cuda_iterative" solver, ExpAMRSolver.cpp:354-416): it reads the solver
+parameters from the simulation data, re-assembles the operator when the mesh
+changed (the reference keys this on ``pres->UpdateFluxCorrection``,
+ExpAMRSolver.cpp:370-381), solves on the GPU through the C ABI and writes the
+mean-free solution into ``output``.
+
+``Solver`` is the thin object wrapper of the C ABI used by the tests and the
+benchmark.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .mesh import BLEN, Mesh
+
+FP64, MIXED = 0, 1
+NORM_INF, NORM_L2 = 0, 1
+
+
+def _check(h, rc):
+    if rc != 0:
+        msg = _lib.load().cup2d_ps_last_error(h).decode()
+        raise _lib.SolverError(f"cup2d_ps error {rc}: {msg}")
+
+
+def default_config(**kw) -> _lib.Config:
+    cfg = _lib.Config()
+    _lib.load().cup2d_ps_config_default(ctypes.byref(cfg))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise TypeError(f"unknown config field {k}")
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _mesh_desc(mesh: Mesh):
+    keep = [np.ascontiguousarray(mesh.level, dtype=np.int32),
+            np.ascontiguousarray(mesh.bx, dtype=np.int32),
+            np.ascontiguousarray(mesh.by, dtype=np.int32)]
+    d = _lib.MeshDesc(mesh.bpdx, mesh.bpdy, mesh.levels, int(mesh.periodic_x), int(mesh.periodic_y),
+                      float(mesh.extent), mesh.n_blocks, keep[0].ctypes.data, keep[1].ctypes.data,
+                      keep[2].ctypes.data)
+    return d, keep
+
+
+def assemble_csr(mesh: Mesh, mean_constraint=0):
+    """Host-only assembly of the operator's FP64 CSR (cup2d_op_csr).  Returns
+    (row_ptr int64, col int32, val float64, mean_row)."""
+    lib = _lib.load()
+    d, keep = _mesh_desc(mesh)
+    mr = ctypes.c_longlong()
+    nnz = lib.cup2d_op_csr(ctypes.byref(d), int(mean_constraint), None, None, None, 0, ctypes.byref(mr))
+    if nnz == -1:
+        raise _lib.SolverError(lib.cup2d_ps_last_error(None).decode())
+    nnz = -nnz
+    rp = np.empty(mesh.n + 1, np.int64)
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, np.float64)
+    got = lib.cup2d_op_csr(ctypes.byref(d), int(mean_constraint), rp.ctypes.data, col.ctypes.data,
+                           val.ctypes.data, nnz, ctypes.byref(mr))
+    assert got == nnz
+    return rp, col, val, mr.value
+
+
+class DeviceArray:
+    """FP64 device buffer owned through a solver handle."""
+
+    def __init__(self, solver: "Solver", n: int):
+        self.solver = solver
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        _check(solver.h, _lib.load().cup2d_ps_dmalloc(solver.h, self.n * 8, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, solver, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(solver, a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        _check(self.solver.h, _lib.load().cup2d_ps_h2d(self.solver.h, self.ptr, a.ctypes.data, self.n * 8))
+
+    def numpy(self):
+        out = np.empty(self.n, np.float64)
+        _check(self.solver.h, _lib.load().cup2d_ps_d2h(self.solver.h, out.ctypes.data, self.ptr, self.n * 8))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _lib.load().cup2d_ps_dfree(self.solver.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.solver.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Solver:
+    """One solver handle (one HIP stream, its device buffers, one mesh)."""
+
+    def __init__(self, config: Optional[_lib.Config] = None, **kw):
+        lib = _lib.load()
+        self.config = config if config is not None else default_config(**kw)
+        h = ctypes.c_void_p()
+        rc = lib.cup2d_ps_create(ctypes.byref(self.config), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.SolverError(f"cup2d_ps_create error {rc}: {lib.cup2d_ps_last_error(None).decode()}")
+        self.h = h.value
+        self.mesh: Optional[Mesh] = None
+
+    def close(self):
+        if self.h:
+            _lib.load().cup2d_ps_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- mesh ------------------------------------------------------------
+    def set_mesh(self, mesh: Mesh):
+        d, self._keep = _mesh_desc(mesh)
+        _check(self.h, _lib.load().cup2d_ps_set_mesh(self.h, ctypes.byref(d)))
+        self.mesh = mesh
+        return self
+
+    def sizes(self):
+        n, nnz, ne, mr = (ctypes.c_longlong() for _ in range(4))
+        _check(self.h, _lib.load().cup2d_ps_sizes(self.h, ctypes.byref(n), ctypes.byref(nnz),
+                                                   ctypes.byref(ne), ctypes.byref(mr)))
+        return {"n": n.value, "nnz": nnz.value, "n_explicit": ne.value, "mean_row": mr.value}
+
+    def csr(self):
+        s = self.sizes()
+        rp = np.empty(s["n"] + 1, np.int64)
+        col = np.empty(s["nnz"], np.int32)
+        val = np.empty(s["nnz"], np.float64)
+        _check(self.h, _lib.load().cup2d_ps_get_csr(self.h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return rp, col, val
+
+    # -- solve -----------------------------------------------------------
+    def solve(self, b, x0=None, tol=0.0, rel_tol=1e-10, max_restarts=100):
+        """Host arrays in, (x, stats) out; x is mean-free (ExpAMRSolver.cpp:383-415)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        st = _lib.Stats()
+        _check(self.h, _lib.load().cup2d_ps_solve(self.h, b.ctypes.data, x.ctypes.data, float(tol),
+                                                  float(rel_tol), int(max_restarts), ctypes.byref(st)))
+        return x, st.as_dict()
+
+    def solve_device(self, d_b: DeviceArray, d_x: DeviceArray, tol=0.0, rel_tol=1e-10, max_restarts=100):
+        st = _lib.Stats()
+        _check(self.h, _lib.load().cup2d_ps_solve_device(self.h, d_b.ptr, d_x.ptr, float(tol),
+                                                         float(rel_tol), int(max_restarts),
+                                                         ctypes.byref(st)))
+        return st.as_dict()
+
+    # -- operator pieces ---------------------------------------------------
+    def spmv(self, x, kind=0):
+        dx = DeviceArray.from_host(self, x)
+        dy = DeviceArray(self, dx.n)
+        _check(self.h, _lib.load().cup2d_ps_spmv_device(self.h, dx.ptr, dy.ptr, int(kind)))
+        return dy.numpy()
+
+    def spmv_time(self, d_x: DeviceArray, d_y: DeviceArray, kind=0, reps=50):
+        ms = ctypes.c_double()
+        _check(self.h, _lib.load().cup2d_ps_spmv_time(self.h, d_x.ptr, d_y.ptr, int(kind), int(reps),
+                                                      ctypes.byref(ms)))
+        return ms.value
+
+    def precond(self, r):
+        dr = DeviceArray.from_host(self, r)
+        dz = DeviceArray(self, dr.n)
+        _check(self.h, _lib.load().cup2d_ps_precond_device(self.h, dr.ptr, dz.ptr))
+        return dz.numpy()
+
+    def residual_device(self, d_b: DeviceArray, d_x: DeviceArray):
+        out = (ctypes.c_double * 4)()
+        _check(self.h, _lib.load().cup2d_ps_residual_device(self.h, d_b.ptr, d_x.ptr, out))
+        r2, rinf, b2, binf = list(out)
+        return {"r2": r2, "rinf": rinf, "b2": b2, "binf": binf,
+                "rel2": r2 / b2 if b2 > 0 else 0.0, "relinf": rinf / binf if binf > 0 else 0.0}
+
+    def residual(self, b, x):
+        return self.residual_device(DeviceArray.from_host(self, b), DeviceArray.from_host(self, x))
+
+
+# ---------------------------------------------------------------------------
+# Mirror of the reference's solver plugin interface
+# ---------------------------------------------------------------------------
+
+@dataclass
+class ScalarGrid:
+    """Block grid of scalars: data[b, iy, ix] on the blocks of ``mesh``
+    (the reference's ScalarGrid, Definitions.h:225-235).  ``UpdateFluxCorrection``
+    is the mesh-changed flag the reference solver reads and clears
+    (ExpAMRSolver.cpp:370-372)."""
+
+    mesh: Mesh
+    data: np.ndarray = None
+    UpdateFluxCorrection: bool = True
+
+    def __post_init__(self):
+        if self.data is None:
+            self.data = np.zeros((self.mesh.n_blocks, 8, 8), np.float64)
+
+
+@dataclass
+class SimulationData:
+    """The solver parameters the reference reads from SimulationData
+    (SimulationData.h:92-98, defaults Simulation.cpp:239-245)."""
+
+    poissonSolver: str = "hip_iterative"
+    PoissonTol: float = 1e-6
+    PoissonTolRel: float = 0.0
+    maxPoissonRestarts: int = 30
+    maxPoissonIterations: int = 1000
+    bMeanConstraint: int = 0
+    step: int = 0
+    verbose: bool = False
+    device: int = 0
+    tmp: Optional[ScalarGrid] = None
+    pres: Optional[ScalarGrid] = None
+    extra: dict = field(default_factory=dict)
+
+
+class PoissonSolver:
+    """Base.h:8-15."""
+
+    def solve(self, input: ScalarGrid, output: ScalarGrid) -> None:  # noqa: A002
+        raise NotImplementedError
+
+
+class HipPoissonSolver(PoissonSolver):
+    """GPU BiCGSTAB + 64x64 block-Jacobi behind PoissonSolver (the role of
+    ExpAMRSolver, ExpAMRSolver.cpp:25-416)."""
+
+    def __init__(self, sim: SimulationData, precision=FP64, norm=NORM_INF, **cfg):
+        self.sim = sim
+        self.solver = Solver(default_config(device=sim.device, precision=precision,
+                                            mean_constraint=int(sim.bMeanConstraint != 0), norm=norm,
+                                            max_iter=int(sim.maxPoissonIterations), **cfg))
+        self.last_stats = None
+        self._mesh_id = None
+
+    def solve(self, input: ScalarGrid, output: ScalarGrid) -> None:  # noqa: A002
+        sim = self.sim
+        # tolerances of the first 10 steps (ExpAMRSolver.cpp:366-368)
+        tol = 0.0 if sim.step < 10 else sim.PoissonTol
+        rel = 0.0 if sim.step < 10 else sim.PoissonTolRel
+        restarts = 100 if sim.step < 10 else sim.maxPoissonRestarts
+        if output.UpdateFluxCorrection or self._mesh_id is not id(output.mesh):
+            output.UpdateFluxCorrection = False
+            self.solver.set_mesh(output.mesh)
+            self._mesh_id = id(output.mesh)
+        if input.data.shape != output.data.shape:
+            raise ValueError("input and output grids differ")
+        b = input.data.reshape(-1)
+        x = output.data.reshape(-1)
+        xs, st = self.solver.solve(b, x, tol, rel, restarts)
+        output.data[...] = xs.reshape(output.data.shape)
+        self.last_stats = st
+        if sim.verbose:
+            print(f"  [BiCGSTAB/HIP]: {st['iterations']} iterations, {st['restarts']} restarts, "
+                  f"error {st['err_final']:.3e} / {st['err_init']:.3e}")
+
+
+def makePoissonSolver(sim: SimulationData) -> PoissonSolver:
+    """String factory of Base.cpp:8-30.  "hip_iterative" (and the reference's
+    GPU name "cuda_iterative", for which this solver is the drop-in) give the
+    HIP solver in FP64; "hip_mixed" the mixed-precision variant.  The CPU
+    solver "iterative" is not part of this library."""
+    name = sim.poissonSolver
+    if name in ("hip_iterative", "cuda_iterative"):
+        return HipPoissonSolver(sim, FP64)
+    if name == "hip_mixed":
+        return HipPoissonSolver(sim, MIXED)
+    if name == "iterative":
+        raise RuntimeError('Poisson solver: "iterative" (the CPU AMRSolver) is not provided by '
+                           "cup2d_amd; use \"hip_iterative\"")
+    raise ValueError(f'Poisson solver: "{name}" unrecognized!')
+
+
+Solver_ = PoissonSolver  # name of the Python binding class (Bindings/Poisson.cpp:10)

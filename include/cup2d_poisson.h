@@ -1,0 +1,162 @@
+/* cup2d_poisson.h -- C ABI of the MI355X pressure-Poisson solver
+ * (libcup2d_poisson.so, HIP for gfx950).
+ *
+ * Drop-in for the reference's solver plugin boundary:
+ *   class PoissonSolver { virtual void solve(const ScalarGrid *input, ScalarGrid *output) = 0; };
+ *   std::shared_ptr<PoissonSolver> makePoissonSolver(SimulationData&);
+ *     (source/Poisson/Base.h:8-15, source/Poisson/Base.cpp:8-30)
+ * whose GPU implementation is ExpAMRSolver (source/Poisson/ExpAMRSolver.h:14-307)
+ * -> LocalSpMatDnVec (source/Poisson/LocalSpMatDnVec.h:44-126)
+ * -> BiCGSTABSolver (source/Poisson/BiCGSTAB.cuh:24-148).
+ *
+ * Conventions: unknowns are ordered block-major, 64 per 8x8-cell block, cell
+ * iy*8+ix (ExpAMRSolver.h:74-75); all vectors are FP64; every entry point
+ * returns a cup2d_ps_status; no C++ exception crosses the ABI.  A handle owns
+ * one HIP stream and its device buffers; it is not thread-safe (serialise
+ * calls per handle).
+ */
+#ifndef CUP2D_POISSON_H
+#define CUP2D_POISSON_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cup2d_ps cup2d_ps;
+
+typedef enum {
+  CUP2D_PS_OK = 0,
+  CUP2D_PS_EINVAL = 1,   /* invalid argument / mesh */
+  CUP2D_PS_ENOMESH = 2,  /* solve before cup2d_ps_set_mesh */
+  CUP2D_PS_EHIP = 3,     /* HIP runtime error (message in cup2d_ps_last_error) */
+  CUP2D_PS_ENOMEM = 4,   /* device allocation failed */
+  CUP2D_PS_ENODEV = 5    /* no usable gfx950 device */
+} cup2d_ps_status;
+
+typedef enum { CUP2D_PS_FP64 = 0, CUP2D_PS_MIXED = 1 } cup2d_ps_precision;
+typedef enum { CUP2D_PS_NORM_INF = 0, CUP2D_PS_NORM_L2 = 1 } cup2d_ps_norm;
+
+/* Block description of the mesh: what the reference reads from
+ * cubism::BlockInfo (level, index) at ExpAMRSolver.cpp:186-249.
+ * Block b owns unknowns 64b..64b+63; the tree must be 2:1 balanced across
+ * faces (ExpAMRSolver.cpp:177 throws otherwise). */
+typedef struct {
+  int bpdx, bpdy;        /* level-0 blocks per direction (-bpdx, -bpdy) */
+  int levels;            /* -levelMax */
+  int periodic_x, periodic_y;
+  double extent;         /* -extent: length of the longer side */
+  long long n_blocks;
+  const int *level;      /* [n_blocks] */
+  const int *bx;         /* [n_blocks] block index at its level */
+  const int *by;         /* [n_blocks] */
+} cup2d_mesh_desc;
+
+typedef struct {
+  int device;             /* HIP device ordinal */
+  int precision;          /* cup2d_ps_precision */
+  int mean_constraint;    /* bMeanConstraint (SimulationData.h:98): 0 = post-hoc mean removal only,
+                             nonzero = the (0,0) row of the corner block is replaced by
+                             sum h^2 x = 0 (BiCGSTAB.cu:393-409) */
+  int norm;               /* stopping norm: CUP2D_PS_NORM_INF as BiCGSTAB.cu:572-587, or
+                             CUP2D_PS_NORM_L2 as AMRSolver.cpp:466-471 */
+  int max_iter;           /* iteration cap (reference GPU: 1000, BiCGSTAB.cu:468) */
+  int chunk;              /* iterations per captured graph launch; 0 = automatic */
+  int use_graph;          /* 1 = replay iteration chunks as a hipGraph */
+  int reserved_i[5];
+  double eps;             /* denominator guard; reference GPU 1e-21 (BiCGSTAB.cu:427),
+                             reference CPU 1e-100 (AMRSolver.cpp:167) */
+  double breakdown_tol;   /* restart when rho^2 < breakdown_tol |r|^2 |rhat|^2 (BiCGSTAB.cu:483) */
+  double inner_tol;       /* mixed precision: relative reduction of the FP32 recurrence residual
+                             that triggers an FP64 residual update */
+  double reserved_d[5];
+} cup2d_ps_config;
+
+typedef struct {
+  int iterations;         /* BiCGSTAB iterations (inner iterations for the mixed solver) */
+  int restarts;           /* breakdown restarts (BiCGSTAB.cu:488-509) */
+  int converged;          /* 1 when the stopping test was met; 0 returns the best iterate x_opt */
+  int refinements;        /* FP64 residual updates (mixed precision) */
+  double err_init;        /* stopping norm of r0 = b - A x0 */
+  double err_final;       /* stopping norm of the returned iterate (recurrence residual) */
+  double ms;              /* wall time of the solve on the handle's stream, ms */
+  double reserved[5];
+} cup2d_ps_stats;
+
+/* Fills the defaults: FP64, mean_constraint 0, NORM_INF, max_iter 1000,
+ * eps 1e-100, breakdown_tol 1e-16, graph replay on. */
+void cup2d_ps_config_default(cup2d_ps_config *cfg);
+
+/* Creates a solver (reference: ExpAMRSolver ctor, ExpAMRSolver.cpp:25-99, which
+ * builds P_inv = -K^-1 and the BiCGSTABSolver, BiCGSTAB.cu:8-47). */
+int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out);
+int cup2d_ps_destroy(cup2d_ps *h);
+
+/* Mesh changed: assemble the operator and upload it
+ * (reference: ExpAMRSolver::getMat + LocalSpMatDnVec::make + BiCGSTABSolver::updateAll,
+ * ExpAMRSolver.cpp:180-313, LocalSpMatDnVec.cpp:81-189, BiCGSTAB.cu:142-244). */
+int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *mesh);
+
+/* Solve A x = b (reference: ExpAMRSolver::solve, ExpAMRSolver.cpp:354-416).
+ * b: right-hand side (not modified; with mean_constraint its mean row is
+ * taken as 0, ExpAMRSolver.cpp:340-345); x: initial guess on entry, solution
+ * with its h^2-weighted mean removed on exit.  tol_abs / tol_rel are
+ * PoissonTol / PoissonTolRel; the caller applies the reference's step<10 rule
+ * (tol 0, 100 restarts, ExpAMRSolver.cpp:366-368).  Host pointers. */
+int cup2d_ps_solve(cup2d_ps *h, const double *b, double *x, double tol_abs, double tol_rel,
+                   int max_restarts, cup2d_ps_stats *stats);
+
+/* Same with device pointers (n doubles each, resident in HBM). */
+int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs,
+                          double tol_rel, int max_restarts, cup2d_ps_stats *stats);
+
+/* y = A x on the device (the operator of the last cup2d_ps_set_mesh, with the
+ * mean-constraint row if enabled).  kind 0: stencil kernel used by the solver,
+ * kind 1: plain CSR kernel.  Reference: BiCGSTABSolver::hd_cusparseSpMV
+ * (BiCGSTAB.cu:315-410). */
+int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind);
+
+/* Times `reps` back-to-back launches of one SpMV kind on the handle's stream
+ * with HIP events; writes the mean ms per launch. */
+int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, int reps,
+                       double *ms_per_launch);
+
+/* Block preconditioner z = P_inv r on every block (reference: cublasDgemm at
+ * BiCGSTAB.cu:520, :545). */
+int cup2d_ps_precond_device(cup2d_ps *h, const double *d_r, double *d_z);
+
+/* True residual of x: out[0] = |b - A x|_2, out[1] = |b - A x|_inf,
+ * out[2] = |b|_2, out[3] = |b|_inf, all in FP64 on the device (b's mean row
+ * taken as 0 with mean_constraint). */
+int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]);
+
+/* Operator sizes of the current mesh: n, nnz, explicit (coarse-fine) rows. */
+int cup2d_ps_sizes(const cup2d_ps *h, long long *n, long long *nnz, long long *n_explicit,
+                   long long *mean_row);
+/* Copies the assembled FP64 CSR (row_ptr[n+1], col[nnz], val[nnz]). */
+int cup2d_ps_get_csr(const cup2d_ps *h, long long *row_ptr, int *col, double *val);
+
+/* Host-only assembly (no device needed): the FP64 CSR of the operator of
+ * `mesh` (ExpAMRSolver::getMat, ExpAMRSolver.cpp:180-313).  Returns nnz, or
+ * -(needed nnz) when cap is too small (arrays untouched), or -1 on an invalid
+ * mesh (message via cup2d_ps_last_error(NULL)).  *mean_row receives the
+ * replaced row or -1. */
+long long cup2d_op_csr(const cup2d_mesh_desc *mesh, int mean_constraint, long long *row_ptr,
+                       int *col, double *val, long long cap, long long *mean_row);
+
+/* Device memory helpers for callers without an allocator of their own. */
+int cup2d_ps_dmalloc(cup2d_ps *h, size_t bytes, void **ptr);
+int cup2d_ps_dfree(cup2d_ps *h, void *ptr);
+int cup2d_ps_h2d(cup2d_ps *h, void *dst, const void *src, size_t bytes);
+int cup2d_ps_d2h(cup2d_ps *h, void *dst, const void *src, size_t bytes);
+
+/* Message of the last failure on this handle (or of cup2d_ps_create when
+ * h is NULL). */
+const char *cup2d_ps_last_error(const cup2d_ps *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUP2D_POISSON_H */
