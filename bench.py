@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: replay iteration chunks from a hipGraph; 0: plain stream launches")
+    ap.add_argument("--chunk", type=int, default=0, help="iterations per chunk (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
@@ -77,7 +80,7 @@ def main():
 
     mesh = W.config2_mesh()
     b = W.config_rhs(mesh, 2, with_sources=True)
-    s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=4000)
+    s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=4000, use_graph=args.graph, chunk=args.chunk)
     s.set_mesh(mesh)
     sz = s.sizes()
     d_b = PS.DeviceArray.from_host(s, b)
@@ -129,6 +132,7 @@ def main():
             "iterations": stats[-1]["iterations"], "restarts": stats[-1]["restarts"],
             "converged": conv, "true_rel2_residual": res["rel2"],
             "solve_ms_device": float(np.median([st["ms"] for st in stats])),
+            "graph": args.graph,
         },
         "roofline": {
             "bound": "hbm",

@@ -7,7 +7,6 @@ namespace cup2d {
 
 constexpr int WAVES = 4;            // wavefronts per workgroup
 constexpr int WG = 64 * WAVES;      // threads per workgroup
-constexpr int GB = 4;               // blocks per wave in preconditioner passes
 
 // Device view of the operator in stencil form (see assembly.h).
 struct OpView {
@@ -23,9 +22,12 @@ struct OpView {
   const double *h2;                      // [nb]
 };
 
-// Solver scalars, resident on the device for the whole solve.  Written only
-// by the last workgroup of a reduction kernel (or the init kernel), read by
-// every kernel at launch.
+// Solver scalars, resident on the device for the whole solve.  The solver
+// keeps two copies, indexed by iteration parity: iteration i reads and
+// updates slot i%2 only through fields no other workgroup of the same launch
+// reads, and derives slot i%2 from slot (i-1)%2 plus the partial sums of the
+// previous kernel.  Kernel boundaries publish the writes; no kernel of the
+// iteration needs a device-wide fence.
 struct Scal {
   double alpha, beta, omega, rho, rho_prev;
   double err, err0, err_opt;
@@ -36,8 +38,6 @@ struct Scal {
   int done, converged, improved, restart;
   int norm;
   int pad;
-  unsigned int ticket;                   // arrival counter of the last-workgroup reductions
-  unsigned int pad2;
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -60,6 +60,55 @@ __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+// Workgroup reduction of NQ per-thread quantities (bit q of MAXQ: max, else
+// sum); thread 0 stores them to part[q*gridDim.x + blockIdx.x].  No fence:
+// the consumer is a later kernel.
+template <int NQ, unsigned MAXQ>
+__device__ void store_partials(double (&v)[NQ], double *part) {
+  __shared__ double red[NQ][WAVES];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) v[q] = ((MAXQ >> q) & 1) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[q][w] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      double a = red[q][0];
+      for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, red[q][i]) : a + red[q][i];
+      part[q * gridDim.x + blockIdx.x] = a;
+    }
+  }
+}
+
+// Every workgroup reduces the G partials of each quantity written by the
+// previous kernel, in a fixed order (identical totals in all workgroups);
+// every thread receives the totals.
+template <int NQ, unsigned MAXQ>
+__device__ void load_totals(const double *__restrict__ part, int G, double (&out)[NQ]) {
+  __shared__ double fin[NQ][WAVES];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < G; i += WG) {
+      const double p = part[q * G + i];
+      a = ((MAXQ >> q) & 1) ? fmax(a, p) : a + p;
+    }
+    a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
+    if (lane == 0) fin[q][w] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double a = fin[q][0];
+    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, fin[q][i]) : a + fin[q][i];
+    out[q] = a;
+  }
 }
 
 // 5-point stencil row of cell c of block b: sum over non-wall faces of
@@ -99,69 +148,18 @@ __device__ __forceinline__ double apply_block(const OpView &A, const double *__r
   return y;
 }
 
-// z = P v for GB blocks at once: lane c owns output row c; P (64x64,
-// symmetric) sits in LDS, v holds the GB input vectors (64 each).
-__device__ __forceinline__ void prec_group(const double *Ps, const double *v, int c, double z[GB]) {
+// z = P v for NB blocks at once: lane c owns output row c; P (64x64,
+// symmetric) sits in LDS, v holds the NB input vectors (64 each).
+template <int NB>
+__device__ __forceinline__ void prec_group(const double *Ps, const double *v, int c, double (&z)[NB]) {
 #pragma unroll
-  for (int j = 0; j < GB; j++) z[j] = 0.0;
+  for (int j = 0; j < NB; j++) z[j] = 0.0;
 #pragma unroll 8
   for (int k = 0; k < 64; k++) {
     const double pk = Ps[k * 64 + c];
 #pragma unroll
-    for (int j = 0; j < GB; j++) z[j] = fma(pk, v[j * 64 + k], z[j]);
+    for (int j = 0; j < NB; j++) z[j] = fma(pk, v[j * 64 + k], z[j]);
   }
-}
-
-// Workgroup reduction of NQ quantities (bit q of MAXQ set: max, else sum),
-// written to part[q*gridDim.x + blockIdx.x]; the last workgroup to arrive
-// reduces all partials in a fixed order and returns true with the totals in
-// out[] (every thread of that workgroup).  Deterministic for a fixed grid.
-template <int NQ, unsigned MAXQ>
-__device__ bool grid_reduce(double (&v)[NQ], double *part, unsigned int *ticket, double (&out)[NQ]) {
-  __shared__ double red[NQ][WAVES];
-  __shared__ double fin[NQ][WAVES];
-  __shared__ int is_last;
-  const int lane = lane_id(), w = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NQ; q++) v[q] = ((MAXQ >> q) & 1) ? wave_max(v[q]) : wave_sum(v[q]);
-  if (lane == 0)
-#pragma unroll
-    for (int q = 0; q < NQ; q++) red[q][w] = v[q];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      double a = red[q][0];
-      for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, red[q][i]) : a + red[q][i];
-      part[q * gridDim.x + blockIdx.x] = a;
-    }
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  const int G = gridDim.x;
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    double a = 0.0;
-    for (int i = threadIdx.x; i < G; i += WG) {
-      const double p = __builtin_nontemporal_load(&part[q * G + i]);
-      a = ((MAXQ >> q) & 1) ? fmax(a, p) : a + p;
-    }
-    a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
-    if (lane == 0) fin[q][w] = a;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    double a = fin[q][0];
-    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, fin[q][i]) : a + fin[q][i];
-    out[q] = a;
-  }
-  if (threadIdx.x == 0) *ticket = 0u;
-  return true;
 }
 
 }  // namespace cup2d

@@ -33,8 +33,7 @@ using namespace cup2d;
 
 namespace {
 
-constexpr int MAXG = 1024;   // workgroups of the vector kernels (grid-stride beyond)
-constexpr int NQMAX = 8;     // reduction quantities per kernel
+constexpr int MAXG = 512;    // workgroups of the vector kernels (grid-stride beyond)
 
 // ----------------------------------------------------------------------------
 // kernels
@@ -45,26 +44,96 @@ __device__ __forceinline__ void load_P(double *Ps, const double *__restrict__ P)
   __syncthreads();
 }
 
-// r = b - A x0, rhat = r, p = nu = 0, x = x_opt = x0; initial scalars
-// (BiCGSTAB.cu:427-465).
+// State at the top of iteration i from the state of iteration i-1 and the
+// totals o = (|r|_inf, |r|^2, (rhat,r), |rhat|^2) of iteration i-1's update
+// (BiCGSTAB.cu:471-509 and :572-600).  prev.iter < 0 marks the state after
+// r0 = b - A x0 (BiCGSTAB.cu:427-465).  Deterministic: recomputing it from
+// the same inputs gives the same state.
+__device__ Scal evaluate(const Scal &prev, const double (&o)[4]) {
+  Scal s = prev;
+  const double rr = o[1], rhr = o[2], hh = o[3];
+  const double err = prev.norm ? sqrt(rr) : o[0];
+  if (prev.iter < 0) {
+    s.err = s.err0 = s.err_opt = err;
+    s.iter = 0;
+    s.alpha = s.omega = s.rho_prev = 1.0;
+    s.rho = rhr;
+    s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+    s.restarts = s.converged = s.improved = s.restart = 0;
+    s.done = (err == 0.0) ? 1 : 0;
+    return s;
+  }
+  if (prev.done) {
+    s.restart = 0;
+    return s;
+  }
+  s.err = err;
+  s.iter = prev.iter + 1;
+  s.improved = 0;
+  if (err < prev.err_opt) {
+    s.improved = 1;
+    s.err_opt = err;
+    if (err <= s.tol_abs || err / s.err0 <= s.tol_rel) {
+      s.converged = 1;
+      s.done = 1;
+    }
+  }
+  if (!s.done && s.iter >= s.max_iter) s.done = 1;
+  s.rho_prev = prev.rho;
+  s.rho = rhr;
+  const bool breakdown = rhr * rhr < s.bd_tol * rr * hh;
+  s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+  s.restart = 0;
+  if (!s.done && breakdown && s.max_restarts > 0) {
+    s.restarts += 1;
+    if (s.restarts >= s.max_restarts) {
+      s.done = 1;
+    } else {
+      s.restart = 1;
+      s.rho = rr;  // rhat = r
+      s.rho_prev = s.alpha = s.omega = 1.0;
+      s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+    }
+  }
+  s.red[5] = rr;
+  s.red[6] = rhr;
+  s.red[7] = hh;
+  return s;
+}
+
+// Partial sums of h^2 x0 (mean row of A x0 when the mean constraint is on).
+__global__ __launch_bounds__(WG) void k_init_mean(int nb, const double *__restrict__ x0,
+                                                  const double *__restrict__ h2, double *pm) {
+  const long n = (long)nb * 64;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    v[0] += h2[i >> 6] * x0[i];
+  store_partials<1, 0u>(v, pm);
+}
+
+// r = b - A x0, rhat = r, p = nu = 0, x = x_opt = x0 (BiCGSTAB.cu:431-465);
+// partials of (|r|_inf, |r|^2, (rhat,r), |rhat|^2) for the first state.
 __global__ __launch_bounds__(WG) void k_init(OpView A, const double *__restrict__ b,
                                              const double *__restrict__ x0, double *__restrict__ x,
                                              double *__restrict__ xopt, double *__restrict__ r,
                                              double *__restrict__ rhat, double *__restrict__ p,
-                                             double *__restrict__ nu, Scal *s, double *part) {
+                                             double *__restrict__ nu, const double *pm, int Gm,
+                                             double *p5) {
   __shared__ double xs[WAVES][64], ys[WAVES][64];
+  double m0[1] = {0.0};
+  if (A.mean_row >= 0) load_totals<1, 0u>(pm, Gm, m0);
   const int c = lane_id(), w = wave_id();
   const int nw = gridDim.x * WAVES;
-  double v[3] = {0.0, 0.0, 0.0};  // |r|_inf, |r|^2, sum h^2 x0
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
     const long i = (long)bb * 64 + c;
     const double xc = x0[i];
     xs[w][c] = xc;
     wave_sync();
-    double y = apply_block(A, x0, bb, c, xs[w], ys[w], xc);
+    const double y = apply_block(A, x0, bb, c, xs[w], ys[w], xc);
     wave_sync();
-    double ri = b[i] - y;
-    if (i == A.mean_row) ri = 0.0;  // finished by the last workgroup
+    // mean row: b = 0 (ExpAMRSolver.cpp:340-345) and (A x)_row = sum h^2 x
+    const double ri = (i == A.mean_row) ? 0.0 - m0[0] : b[i] - y;
     r[i] = ri;
     rhat[i] = ri;
     p[i] = 0.0;
@@ -73,57 +142,42 @@ __global__ __launch_bounds__(WG) void k_init(OpView A, const double *__restrict_
     xopt[i] = xc;
     v[0] = fmax(v[0], fabs(ri));
     v[1] += ri * ri;
-    v[2] += A.h2[bb] * xc;
   }
-  double o[3];
-  if (!grid_reduce<3, 1u>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0) {
-    double amax = o[0], rr = o[1];
-    if (A.mean_row >= 0) {
-      // mean row: b = 0 (ExpAMRSolver.cpp:340-345), (A x)_row = sum h^2 x
-      const double rm = 0.0 - o[2];
-      r[A.mean_row] = rm;
-      rhat[A.mean_row] = rm;
-      amax = fmax(amax, fabs(rm));
-      rr += rm * rm;
-    }
-    const double err0 = s->norm ? sqrt(rr) : amax;
-    s->err0 = s->err = s->err_opt = err0;
-    s->alpha = s->omega = s->rho_prev = 1.0;
-    s->rho = rr;  // (rhat, r) with rhat = r
-    s->beta = (s->rho / (s->rho_prev + s->eps)) * (s->alpha / (s->omega + s->eps));
-    s->iter = 0;
-    s->restarts = 0;
-    s->converged = 0;
-    s->improved = 0;
-    s->restart = 0;
-    s->done = (err0 == 0.0) ? 1 : 0;
-    s->red[0] = amax;
-    s->red[1] = rr;
-  }
+  v[2] = v[1];
+  v[3] = v[1];
+  store_partials<4, 1u>(v, p5);
 }
 
-// p = r + beta (p - omega nu) (or p = rhat = r after a restart);
-// x_opt = x when the last iteration improved; z1 = P p   (BiCGSTAB.cu:511-520)
+// Top of iteration: evaluate the state (every workgroup, identically; the
+// first one stores it), then p = r + beta (p - omega nu) (or p = rhat = r
+// after a restart), x_opt = x when the last iteration improved, z1 = P p
+// (BiCGSTAB.cu:471-520).
+template <int NB>
 __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict__ P,
                                                 const double *__restrict__ r,
                                                 double *__restrict__ rhat, double *__restrict__ p,
                                                 const double *__restrict__ nu,
                                                 const double *__restrict__ x,
                                                 double *__restrict__ xopt, double *__restrict__ z1,
-                                                const Scal *s) {
-  if (s->done) return;
+                                                Scal *st, int par, const double *p5, int G5) {
+  __shared__ Scal cur;
   __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][GB * 64];
-  const double beta = s->beta, omega = s->omega;
-  const int restart = s->restart, improved = s->improved;
+  __shared__ double vs[WAVES][NB * 64];
+  double o[4];
+  load_totals<4, 1u>(p5, G5, o);
+  if (threadIdx.x == 0) cur = evaluate(st[1 - par], o);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) st[par] = cur;
+  if (cur.done) return;
+  const double beta = cur.beta, omega = cur.omega;
+  const int restart = cur.restart, improved = cur.improved;
   load_P(Ps, P);
   const int c = lane_id(), w = wave_id();
-  const int ng = (nb + GB - 1) / GB, nw = gridDim.x * WAVES;
+  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
   for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
 #pragma unroll
-    for (int j = 0; j < GB; j++) {
-      const int bb = g * GB + j;
+    for (int j = 0; j < NB; j++) {
+      const int bb = g * NB + j;
       double pi = 0.0;
       if (bb < nb) {
         const long i = (long)bb * 64 + c;
@@ -140,27 +194,35 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
       vs[w][j * 64 + c] = pi;
     }
     wave_sync();
-    double z[GB];
-    prec_group(Ps, vs[w], c, z);
+    double z[NB];
+    prec_group<NB>(Ps, vs[w], c, z);
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < GB; j++) {
-      const int bb = g * GB + j;
+    for (int j = 0; j < NB; j++) {
+      const int bb = g * NB + j;
       if (bb < nb) z1[(long)bb * 64 + c] = z[j];
     }
   }
 }
 
-// nu = A z1; alpha = rho / ((rhat, nu) + eps)   (BiCGSTAB.cu:523-533)
+// State evaluation alone (end of an iteration chunk, one workgroup), so the
+// host sees convergence of the chunk's last iteration.
+__global__ __launch_bounds__(WG) void k_eval(Scal *st, int par, const double *p5, int G5) {
+  double o[4];
+  load_totals<4, 1u>(p5, G5, o);
+  if (threadIdx.x == 0) st[par] = evaluate(st[1 - par], o);
+}
+
+// nu = A z1; partials of (rhat, nu) and sum h^2 z1   (BiCGSTAB.cu:523-527)
 __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, const double *__restrict__ z1,
                                                 double *__restrict__ nu,
-                                                const double *__restrict__ rhat, Scal *s,
-                                                double *part) {
-  if (s->done) return;
+                                                const double *__restrict__ rhat, const Scal *st,
+                                                int par, double *p2) {
+  if (st[par].done) return;
   __shared__ double xs[WAVES][64], ys[WAVES][64];
   const int c = lane_id(), w = wave_id();
   const int nw = gridDim.x * WAVES;
-  double v[2] = {0.0, 0.0};  // (rhat, nu), sum h^2 z1
+  double v[2] = {0.0, 0.0};
   for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
     const long i = (long)bb * 64 + c;
     const double zc = z1[i];
@@ -168,70 +230,76 @@ __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, const double *__restri
     wave_sync();
     double y = apply_block(A, z1, bb, c, xs[w], ys[w], zc);
     wave_sync();
-    if (i == A.mean_row) y = 0.0;
+    if (i == A.mean_row) y = 0.0;  // replaced by the consumer
     nu[i] = y;
     v[0] += rhat[i] * y;
     v[1] += A.h2[bb] * zc;
   }
-  double o[2];
-  if (!grid_reduce<2, 0u>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0) {
-    double rn = o[0];
-    if (A.mean_row >= 0) {
-      nu[A.mean_row] = o[1];
-      rn += rhat[A.mean_row] * o[1];
-    }
-    s->red[2] = rn;
-    s->alpha = s->rho / (rn + s->eps);
-  }
+  store_partials<2, 0u>(v, p2);
 }
 
-// s = r - alpha nu (stored in r); z2 = P s   (BiCGSTAB.cu:538-545)
+// alpha = rho / ((rhat, nu) + eps); s = r - alpha nu (stored in r); z2 = P s
+// (BiCGSTAB.cu:527-545)
+template <int NB>
 __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict__ P,
-                                                double *__restrict__ r,
-                                                const double *__restrict__ nu,
-                                                double *__restrict__ z2, const Scal *s) {
-  if (s->done) return;
+                                                double *__restrict__ r, double *__restrict__ nu,
+                                                const double *__restrict__ rhat,
+                                                double *__restrict__ z2, Scal *st, int par,
+                                                const double *p2, int G2, long mean_row) {
+  if (st[par].done) return;
   __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][GB * 64];
-  const double alpha = s->alpha;
+  __shared__ double vs[WAVES][NB * 64];
+  double o[2];
+  load_totals<2, 0u>(p2, G2, o);
+  double rn = o[0];
+  if (mean_row >= 0) rn += rhat[mean_row] * o[1];
+  const double alpha = st[par].rho / (rn + st[par].eps);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[par].alpha = alpha;
+    st[par].red[2] = rn;
+  }
   load_P(Ps, P);
   const int c = lane_id(), w = wave_id();
-  const int ng = (nb + GB - 1) / GB, nw = gridDim.x * WAVES;
+  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
   for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
 #pragma unroll
-    for (int j = 0; j < GB; j++) {
-      const int bb = g * GB + j;
+    for (int j = 0; j < NB; j++) {
+      const int bb = g * NB + j;
       double si = 0.0;
       if (bb < nb) {
         const long i = (long)bb * 64 + c;
-        si = r[i] - alpha * nu[i];
+        double ni = nu[i];
+        if (i == mean_row) {
+          ni = o[1];
+          nu[i] = ni;
+        }
+        si = r[i] - alpha * ni;
         r[i] = si;
       }
       vs[w][j * 64 + c] = si;
     }
     wave_sync();
-    double z[GB];
-    prec_group(Ps, vs[w], c, z);
+    double z[NB];
+    prec_group<NB>(Ps, vs[w], c, z);
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < GB; j++) {
-      const int bb = g * GB + j;
+    for (int j = 0; j < NB; j++) {
+      const int bb = g * NB + j;
       if (bb < nb) z2[(long)bb * 64 + c] = z[j];
     }
   }
 }
 
-// t = A z2; omega = (t, s) / ((t, t) + eps)   (BiCGSTAB.cu:548-561)
+// t = A z2; partials of (t, s), (t, t) and sum h^2 z2   (BiCGSTAB.cu:548-553)
 __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, const double *__restrict__ z2,
                                                double *__restrict__ t,
-                                               const double *__restrict__ sv, Scal *s,
-                                               double *part) {
-  if (s->done) return;
+                                               const double *__restrict__ sv, const Scal *st,
+                                               int par, double *p4) {
+  if (st[par].done) return;
   __shared__ double xs[WAVES][64], ys[WAVES][64];
   const int c = lane_id(), w = wave_id();
   const int nw = gridDim.x * WAVES;
-  double v[3] = {0.0, 0.0, 0.0};  // (t,s), (t,t), sum h^2 z2
+  double v[3] = {0.0, 0.0, 0.0};
   for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
     const long i = (long)bb * 64 + c;
     const double zc = z2[i];
@@ -245,37 +313,40 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, const double *__restric
     v[1] += y * y;
     v[2] += A.h2[bb] * zc;
   }
-  double o[3];
-  if (!grid_reduce<3, 0u>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0) {
-    double ts = o[0], tt = o[1];
-    if (A.mean_row >= 0) {
-      t[A.mean_row] = o[2];
-      ts += o[2] * sv[A.mean_row];
-      tt += o[2] * o[2];
-    }
-    s->red[3] = ts;
-    s->red[4] = tt;
-    s->omega = ts / (tt + s->eps);
-  }
+  store_partials<3, 0u>(v, p4);
 }
 
-// x += alpha z1 + omega z2; r = s - omega t; convergence, best iterate and
-// the scalars of the next iteration (BiCGSTAB.cu:563-600, :471-509)
+// omega = (t, s) / ((t, t) + eps); x += alpha z1 + omega z2; r = s - omega t;
+// partials of (|r|_inf, |r|^2, (rhat, r), |rhat|^2)   (BiCGSTAB.cu:552-572)
 __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
                                                  const double *__restrict__ z1,
                                                  const double *__restrict__ z2,
                                                  double *__restrict__ r,
                                                  const double *__restrict__ t,
-                                                 const double *__restrict__ rhat, Scal *s,
-                                                 double *part) {
-  if (s->done) return;
-  const double alpha = s->alpha, omega = s->omega;
+                                                 const double *__restrict__ rhat, Scal *st,
+                                                 int par, const double *p4, int G4, double *p5,
+                                                 long mean_row) {
+  if (st[par].done) return;
+  double o[3];
+  load_totals<3, 0u>(p4, G4, o);
+  double ts = o[0], tt = o[1];
+  if (mean_row >= 0) {
+    ts += o[2] * r[mean_row];
+    tt += o[2] * o[2];
+  }
+  const double omega = ts / (tt + st[par].eps);
+  const double alpha = st[par].alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[par].omega = omega;
+    st[par].red[3] = ts;
+    st[par].red[4] = tt;
+  }
   const long n = (long)nb * 64;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};  // |r|_inf, |r|^2, (rhat, r), |rhat|^2
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
     x[i] = x[i] + alpha * z1[i] + omega * z2[i];
-    const double ri = r[i] - omega * t[i];
+    const double ti = (i == mean_row) ? o[2] : t[i];
+    const double ri = r[i] - omega * ti;
     r[i] = ri;
     const double rh = rhat[i];
     v[0] = fmax(v[0], fabs(ri));
@@ -283,52 +354,15 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
     v[2] += rh * ri;
     v[3] += rh * rh;
   }
-  double o[4];
-  if (!grid_reduce<4, 1u>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0) {
-    const double rr = o[1], rhr = o[2], hh = o[3];
-    const double err = s->norm ? sqrt(rr) : o[0];
-    s->err = err;
-    s->iter += 1;
-    s->improved = 0;
-    if (err < s->err_opt) {
-      s->improved = 1;
-      s->err_opt = err;
-      if (err <= s->tol_abs || err / s->err0 <= s->tol_rel) {
-        s->converged = 1;
-        s->done = 1;
-      }
-    }
-    if (!s->done && s->iter >= s->max_iter) s->done = 1;
-    // top of the next iteration
-    s->rho_prev = s->rho;
-    s->rho = rhr;
-    const bool breakdown = rhr * rhr < s->bd_tol * rr * hh;
-    s->beta = (s->rho / (s->rho_prev + s->eps)) * (s->alpha / (s->omega + s->eps));
-    s->restart = 0;
-    if (!s->done && breakdown && s->max_restarts > 0) {
-      s->restarts += 1;
-      if (s->restarts >= s->max_restarts) {
-        s->done = 1;
-      } else {
-        s->restart = 1;
-        s->rho = rr;  // rhat = r
-        s->rho_prev = s->alpha = s->omega = 1.0;
-        s->beta = (s->rho / (s->rho_prev + s->eps)) * (s->alpha / (s->omega + s->eps));
-      }
-    }
-    s->red[5] = rr;
-    s->red[6] = rhr;
-    s->red[7] = hh;
-  }
+  store_partials<4, 1u>(v, p5);
 }
 
 // mean of the returned iterate (x if the last iteration improved, else x_opt)
 __global__ __launch_bounds__(WG) void k_final_mean(int nb, const double *__restrict__ x,
                                                    const double *__restrict__ xopt,
-                                                   const double *__restrict__ h2, Scal *s,
-                                                   double *part) {
-  const double *src = s->improved ? x : xopt;
+                                                   const double *__restrict__ h2, const Scal *st,
+                                                   int par, double *pf) {
+  const double *src = st[par].improved ? x : xopt;
   const long n = (long)nb * 64;
   double v[2] = {0.0, 0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
@@ -336,24 +370,26 @@ __global__ __launch_bounds__(WG) void k_final_mean(int nb, const double *__restr
     v[0] += src[i] * hh;
     v[1] += hh;
   }
-  double o[2];
-  if (!grid_reduce<2, 0u>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0) s->mean = o[0] / o[1];
+  store_partials<2, 0u>(v, pf);
 }
 
 __global__ __launch_bounds__(WG) void k_final_write(int nb, const double *__restrict__ x,
                                                     const double *__restrict__ xopt,
-                                                    double *__restrict__ out, const Scal *s) {
-  const double *src = s->improved ? x : xopt;
-  const double mean = s->mean;
+                                                    double *__restrict__ out, Scal *st, int par,
+                                                    const double *pf, int Gf) {
+  double o[2];
+  load_totals<2, 0u>(pf, Gf, o);
+  const double mean = o[0] / o[1];
+  const double *src = st[par].improved ? x : xopt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st[par].mean = mean;
   const long n = (long)nb * 64;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
     out[i] = src[i] - mean;
 }
 
-// y = A x (stencil form), mean row included
+// y = A x (stencil form); partials of sum h^2 x for the mean row
 __global__ __launch_bounds__(WG) void k_spmv(OpView A, const double *__restrict__ x,
-                                             double *__restrict__ y, Scal *s, double *part) {
+                                             double *__restrict__ y, double *pm) {
   __shared__ double xs[WAVES][64], ys[WAVES][64];
   const int c = lane_id(), w = wave_id();
   const int nw = gridDim.x * WAVES;
@@ -368,10 +404,7 @@ __global__ __launch_bounds__(WG) void k_spmv(OpView A, const double *__restrict_
     y[i] = yi;
     v[0] += A.h2[bb] * xc;
   }
-  if (A.mean_row < 0) return;
-  double o[1];
-  if (!grid_reduce<1, 0u>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0) y[A.mean_row] = o[0];
+  if (A.mean_row >= 0) store_partials<1, 0u>(v, pm);
 }
 
 // y = A x, plain CSR, one row per thread
@@ -388,49 +421,47 @@ __global__ __launch_bounds__(WG) void k_spmv_csr(long n, const int *__restrict__
   }
 }
 
-__global__ __launch_bounds__(WG) void k_meanrow(long n, const double *__restrict__ x,
-                                                const double *__restrict__ h2, long mean_row,
-                                                double *__restrict__ y, Scal *s, double *part) {
-  double v[1] = {0.0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
-    v[0] += h2[i >> 6] * x[i];
+// y[mean_row] = sum h^2 x from the partials (one workgroup)
+__global__ __launch_bounds__(WG) void k_meanrow_fix(const double *pm, int G, long mean_row,
+                                                    double *__restrict__ y) {
   double o[1];
-  if (!grid_reduce<1, 0u>(v, part, &s->ticket, o)) return;
+  load_totals<1, 0u>(pm, G, o);
   if (threadIdx.x == 0) y[mean_row] = o[0];
 }
 
 // z = P r on every block
+template <int NB>
 __global__ __launch_bounds__(WG) void k_prec(int nb, const double *__restrict__ P,
                                              const double *__restrict__ r,
                                              double *__restrict__ z) {
   __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][GB * 64];
+  __shared__ double vs[WAVES][NB * 64];
   load_P(Ps, P);
   const int c = lane_id(), w = wave_id();
-  const int ng = (nb + GB - 1) / GB, nw = gridDim.x * WAVES;
+  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
   for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
 #pragma unroll
-    for (int j = 0; j < GB; j++) {
-      const int bb = g * GB + j;
+    for (int j = 0; j < NB; j++) {
+      const int bb = g * NB + j;
       vs[w][j * 64 + c] = bb < nb ? r[(long)bb * 64 + c] : 0.0;
     }
     wave_sync();
-    double zz[GB];
-    prec_group(Ps, vs[w], c, zz);
+    double zz[NB];
+    prec_group<NB>(Ps, vs[w], c, zz);
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < GB; j++) {
-      const int bb = g * GB + j;
+    for (int j = 0; j < NB; j++) {
+      const int bb = g * NB + j;
       if (bb < nb) z[(long)bb * 64 + c] = zz[j];
     }
   }
 }
 
-// |b - A x| and |b| (2- and inf-norms); y holds A x
+// partials of |b - y|^2, |b - y|_inf, |b|^2, |b|_inf; y holds A x
 __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restrict__ b,
                                                  const double *__restrict__ y, long mean_row,
-                                                 Scal *s, double *part) {
-  double v[4] = {0.0, 0.0, 0.0, 0.0};  // |r|^2, |r|inf, |b|^2, |b|inf
+                                                 double *pr) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
     const double bi = (i == mean_row) ? 0.0 : b[i];
     const double ri = bi - y[i];
@@ -439,10 +470,7 @@ __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restric
     v[2] += bi * bi;
     v[3] = fmax(v[3], fabs(bi));
   }
-  double o[4];
-  if (!grid_reduce<4, 0xAu>(v, part, &s->ticket, o)) return;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 4; q++) s->red[q] = o[q];
+  store_partials<4, 0xAu>(v, pr);
 }
 
 // ----------------------------------------------------------------------------
@@ -479,10 +507,12 @@ struct cup2d_ps {
   double *d_x = nullptr, *d_xopt = nullptr, *d_r = nullptr, *d_rhat = nullptr, *d_p = nullptr,
          *d_nu = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_t = nullptr, *d_b = nullptr,
          *d_xio = nullptr;
-  Scal *d_s = nullptr;
-  double *d_part = nullptr;
-  Scal *h_s = nullptr;  // pinned
-  int G = 1;
+  Scal *d_s = nullptr;                 // two state slots (iteration parity)
+  double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr, *d_pf = nullptr;
+  Scal *h_s = nullptr;                 // pinned copy of the state slots
+  int G = 1;                           // workgroups of the SpMV / vector kernels
+  int Gp = 1;                          // workgroups of the preconditioner kernels
+  int nbp = 2;                         // blocks per wave in the preconditioner kernels
   // graph of one chunk of iterations
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
@@ -585,24 +615,49 @@ std::vector<double> make_pinv() {
   return P;
 }
 
-int grid_for(const cup2d_ps *h) { return h->G; }
-
-// Iteration kernels of one BiCGSTAB iteration on the handle's stream.
-void launch_iteration(cup2d_ps *h) {
+// Launches one BiCGSTAB iteration (parity `par` selects the state slot).
+void launch_iteration(cup2d_ps *h, int par) {
   const OpView A = view(h);
-  const int G = grid_for(h);
+  const int G = h->G, Gp = h->Gp;
   const int nb = (int)h->op.nb;
-  const int Gp = std::min(G, std::max(1, ((nb + GB - 1) / GB + WAVES - 1) / WAVES));
-  k_pupdate<<<Gp, WG, 0, h->stream>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
-                                      h->d_xopt, h->d_z1, h->d_s);
-  k_spmv_nu<<<G, WG, 0, h->stream>>>(A, h->d_z1, h->d_nu, h->d_rhat, h->d_s, h->d_part);
-  k_supdate<<<Gp, WG, 0, h->stream>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_z2, h->d_s);
-  k_spmv_t<<<G, WG, 0, h->stream>>>(A, h->d_z2, h->d_t, h->d_r, h->d_s, h->d_part);
-  k_xrupdate<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat,
-                                      h->d_s, h->d_part);
+  const long mr = h->op.mean_row;
+  hipStream_t st = h->stream;
+  if (h->nbp == 4)
+    k_pupdate<4><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
+                                    h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
+  else if (h->nbp == 2)
+    k_pupdate<2><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
+                                    h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
+  else
+    k_pupdate<1><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
+                                    h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
+  k_spmv_nu<<<G, WG, 0, st>>>(A, h->d_z1, h->d_nu, h->d_rhat, h->d_s, par, h->d_p2);
+  if (h->nbp == 4)
+    k_supdate<4><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
+                                    h->d_p2, G, mr);
+  else if (h->nbp == 2)
+    k_supdate<2><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
+                                    h->d_p2, G, mr);
+  else
+    k_supdate<1><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
+                                    h->d_p2, G, mr);
+  k_spmv_t<<<G, WG, 0, st>>>(A, h->d_z2, h->d_t, h->d_r, h->d_s, par, h->d_p4);
+  k_xrupdate<<<G, WG, 0, st>>>(nb, h->d_x, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat, h->d_s,
+                               par, h->d_p4, G, h->d_p5, mr);
 }
 
-int chunk_of(const cup2d_ps *h) { return h->cfg.chunk > 0 ? h->cfg.chunk : 16; }
+// Iterations per chunk: even, so that every chunk starts on state slot 0.
+int chunk_of(const cup2d_ps *h) {
+  int c = h->cfg.chunk > 0 ? h->cfg.chunk : 16;
+  return c + (c & 1);
+}
+
+// One chunk: C iterations, then the evaluation of the next state into slot 0.
+void launch_chunk(cup2d_ps *h) {
+  const int C = chunk_of(h);
+  for (int k = 0; k < C; k++) launch_iteration(h, k & 1);
+  k_eval<<<1, WG, 0, h->stream>>>(h->d_s, 0, h->d_p5, h->G);
+}
 
 int ensure_graph(cup2d_ps *h) {
   const int C = chunk_of(h);
@@ -613,11 +668,12 @@ int ensure_graph(cup2d_ps *h) {
   }
   hipGraph_t g = nullptr;
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < C; k++) launch_iteration(h);
+  launch_chunk(h);
   HIPCHK(h, hipStreamEndCapture(h->stream, &g));
   hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return fail(h, CUP2D_PS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  if (e != hipSuccess)
+    return fail(h, CUP2D_PS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   h->gchunk = C;
   return CUP2D_PS_OK;
 }
@@ -626,7 +682,9 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
                double tol_rel, int max_restarts, cup2d_ps_stats *st) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "solve before cup2d_ps_set_mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  const int G = grid_for(h);
+  const int G = h->G;
+  const int nb = (int)h->op.nb;
+  // slot 1 holds the marker state from which the first evaluation starts
   Scal init{};
   init.tol_abs = tol_abs;
   init.tol_rel = tol_rel;
@@ -635,12 +693,15 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
   init.max_iter = h->cfg.max_iter;
   init.max_restarts = max_restarts;
   init.norm = h->cfg.norm;
-  *h->h_s = init;
+  init.iter = -1;
+  h->h_s[0] = init;
+  h->h_s[1] = init;
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, sizeof(Scal), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
   const OpView A = view(h);
+  if (h->op.mean_row >= 0) k_init_mean<<<G, WG, 0, h->stream>>>(nb, d_x0, h->d_h2, h->d_pm);
   k_init<<<G, WG, 0, h->stream>>>(A, d_b, d_x0, h->d_x, h->d_xopt, h->d_r, h->d_rhat, h->d_p,
-                                  h->d_nu, h->d_s, h->d_part);
+                                  h->d_nu, h->d_pm, G, h->d_p5);
   HIPCHK(h, hipGetLastError());
   const int C = chunk_of(h);
   if (h->cfg.use_graph) {
@@ -652,30 +713,30 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
     if (h->cfg.use_graph) {
       HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
     } else {
-      for (int k = 0; k < C; k++) launch_iteration(h);
+      launch_chunk(h);
       HIPCHK(h, hipGetLastError());
     }
     launched += C;
-    HIPCHK(h, hipMemcpyAsync(&h->h_s->done, &h->d_s->done, sizeof(int), hipMemcpyDeviceToHost,
-                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(&h->h_s[0].done, &h->d_s[0].done, sizeof(int),
+                             hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->h_s->done || launched >= h->cfg.max_iter + C) break;
+    if (h->h_s[0].done || launched > h->cfg.max_iter + C) break;
   }
-  k_final_mean<<<G, WG, 0, h->stream>>>((int)h->op.nb, h->d_x, h->d_xopt, h->d_h2, h->d_s,
-                                        h->d_part);
-  k_final_write<<<G, WG, 0, h->stream>>>((int)h->op.nb, h->d_x, h->d_xopt, d_out, h->d_s);
+  k_final_mean<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, h->d_h2, h->d_s, 0, h->d_pf);
+  k_final_write<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, d_out, h->d_s, 0, h->d_pf, G);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, 2 * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (st) {
+    const Scal &f = h->h_s[0];
     std::memset(st, 0, sizeof(*st));
-    st->iterations = h->h_s->iter;
-    st->restarts = h->h_s->restarts;
-    st->converged = h->h_s->converged;
+    st->iterations = f.iter;
+    st->restarts = f.restarts;
+    st->converged = f.converged;
     st->refinements = 0;
-    st->err_init = h->h_s->err0;
-    st->err_final = h->h_s->err_opt;
+    st->err_init = f.err0;
+    st->err_final = f.err_opt;
     float ms = 0;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     st->ms = ms;
@@ -728,7 +789,7 @@ int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
   if (hipSetDevice(c.device) != hipSuccess) return fail(nullptr, CUP2D_PS_EHIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-      hipHostMalloc((void **)&h->h_s, sizeof(Scal), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc((void **)&h->h_s, 2 * sizeof(Scal), hipHostMallocDefault) != hipSuccess)
     return fail(nullptr, CUP2D_PS_EHIP, "stream/event/pinned allocation failed");
   *out = h.release();
   return CUP2D_PS_OK;
@@ -774,9 +835,16 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
                      &h->d_z1, &h->d_z2, &h->d_t, &h->d_b, &h->d_xio};
   for (double **v : vecs)
     if ((rc = dalloc(h, v, n))) return rc;
-  if ((rc = dalloc(h, &h->d_s, 1)) || (rc = dalloc(h, &h->d_part, (size_t)NQMAX * MAXG))) return rc;
-  HIPCHK(h, hipMemset(h->d_s, 0, sizeof(Scal)));
+  if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_p2, (size_t)2 * MAXG)) ||
+      (rc = dalloc(h, &h->d_p4, (size_t)3 * MAXG)) || (rc = dalloc(h, &h->d_p5, (size_t)4 * MAXG)) ||
+      (rc = dalloc(h, &h->d_pm, (size_t)MAXG)) || (rc = dalloc(h, &h->d_pf, (size_t)4 * MAXG)))
+    return rc;
+  HIPCHK(h, hipMemset(h->d_s, 0, 2 * sizeof(Scal)));
   h->G = (int)std::max<long long>(1, std::min<long long>(MAXG, (op.nb + WAVES - 1) / WAVES));
+  h->nbp = h->cfg.reserved_i[0] > 0 ? h->cfg.reserved_i[0] : (op.nb >= 8192 ? 4 : 2);
+  if (h->nbp != 1 && h->nbp != 2 && h->nbp != 4) h->nbp = 2;
+  const long long groups = (op.nb + h->nbp - 1) / h->nbp;
+  h->Gp = (int)std::max<long long>(1, std::min<long long>(MAXG, (groups + WAVES - 1) / WAVES));
   h->have_mesh = true;
   return CUP2D_PS_OK;
 }
@@ -809,15 +877,15 @@ int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind) 
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   if (kind != 0 && kind != 1) return fail(h, CUP2D_PS_EINVAL, "kind must be 0 or 1");
   HIPCHK(h, hipSetDevice(h->device));
+  const long n = (long)h->op.n;
   if (kind == 0) {
-    k_spmv<<<h->G, WG, 0, h->stream>>>(view(h), d_x, d_y, h->d_s, h->d_part);
+    k_spmv<<<h->G, WG, 0, h->stream>>>(view(h), d_x, d_y, h->d_pm);
   } else {
-    const long n = (long)h->op.n;
     const int G = (int)std::min<long>(4096, (n + WG - 1) / WG);
     k_spmv_csr<<<G, WG, 0, h->stream>>>(n, h->d_csr_rp, h->d_csr_col, h->d_csr_val, d_x, d_y);
-    if (h->op.mean_row >= 0)
-      k_meanrow<<<h->G, WG, 0, h->stream>>>(n, d_x, h->d_h2, h->op.mean_row, d_y, h->d_s, h->d_part);
+    if (h->op.mean_row >= 0) k_init_mean<<<h->G, WG, 0, h->stream>>>((int)h->op.nb, d_x, h->d_h2, h->d_pm);
   }
+  if (h->op.mean_row >= 0) k_meanrow_fix<<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, d_y);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
@@ -834,7 +902,7 @@ int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, in
   const OpView A = view(h);
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int k = 0; k < reps; k++) {
-    if (kind == 0) k_spmv<<<h->G, WG, 0, h->stream>>>(A, d_x, d_y, h->d_s, h->d_part);
+    if (kind == 0) k_spmv<<<h->G, WG, 0, h->stream>>>(A, d_x, d_y, h->d_pm);
     else k_spmv_csr<<<Gc, WG, 0, h->stream>>>(n, h->d_csr_rp, h->d_csr_col, h->d_csr_val, d_x, d_y);
   }
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -851,8 +919,9 @@ int cup2d_ps_precond_device(cup2d_ps *h, const double *d_r, double *d_z) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
   const int nb = (int)h->op.nb;
-  const int Gp = std::min(h->G, std::max(1, ((nb + GB - 1) / GB + WAVES - 1) / WAVES));
-  k_prec<<<Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
+  if (h->nbp == 4) k_prec<4><<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
+  else if (h->nbp == 2) k_prec<2><<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
+  else k_prec<1><<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
@@ -862,15 +931,25 @@ int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, 
   if (!h || !d_b || !d_x || !out) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  k_spmv<<<h->G, WG, 0, h->stream>>>(view(h), d_x, h->d_t, h->d_s, h->d_part);
-  k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_s, h->d_part);
+  k_spmv<<<h->G, WG, 0, h->stream>>>(view(h), d_x, h->d_t, h->d_pm);
+  if (h->op.mean_row >= 0) k_meanrow_fix<<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, h->d_t);
+  k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pf);
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> part(4 * (size_t)h->G);
+  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_pf, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  out[0] = std::sqrt(h->h_s->red[0]);
-  out[1] = h->h_s->red[1];
-  out[2] = std::sqrt(h->h_s->red[2]);
-  out[3] = h->h_s->red[3];
+  double r2 = 0, rinf = 0, b2 = 0, binf = 0;
+  for (int i = 0; i < h->G; i++) {
+    r2 += part[i];
+    rinf = std::max(rinf, part[h->G + i]);
+    b2 += part[2 * h->G + i];
+    binf = std::max(binf, part[3 * h->G + i]);
+  }
+  out[0] = std::sqrt(r2);
+  out[1] = rinf;
+  out[2] = std::sqrt(b2);
+  out[3] = binf;
   return CUP2D_PS_OK;
 }
 
