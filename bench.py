@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: replay iteration chunks from a hipGraph; 0: plain stream launches")
     ap.add_argument("--chunk", type=int, default=0, help="iterations per chunk (0: automatic)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
@@ -80,28 +81,42 @@ def main():
 
     mesh = W.config2_mesh()
     b = W.config_rhs(mesh, 2, with_sources=True)
-    s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=4000, use_graph=args.graph, chunk=args.chunk)
-    s.set_mesh(mesh)
+
+    def run_variant(precision, steps, warmup):
+        s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=4000, use_graph=args.graph,
+                      chunk=args.chunk, precision=precision)
+        s.set_mesh(mesh)
+        d_b = PS.DeviceArray.from_host(s, b)
+        d_x = PS.DeviceArray(s, mesh.n)
+        zeros = np.zeros(mesh.n)
+        for _ in range(warmup):
+            d_x.upload(zeros)
+            s.solve_device(d_b, d_x, rel_tol=1e-10, max_restarts=100)
+        # timed region: `steps` solves, each from x0 = 0
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            d_x.upload(zeros)
+            stats.append(s.solve_device(d_b, d_x, rel_tol=1e-10, max_restarts=100))
+        t1 = time.perf_counter()
+        res = s.residual_device(d_b, d_x)
+        return s, d_x, {
+            "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "solves_per_s": steps / (t1 - t0),
+            "iterations": stats[-1]["iterations"],
+            "refinements": stats[-1]["refinements"],
+            "restarts": stats[-1]["restarts"],
+            "converged": all(st["converged"] for st in stats),
+            "true_rel2_residual": res["rel2"],
+            "solve_ms_device": float(np.median([st["ms"] for st in stats])),
+        }
+
+    main_prec = PS.MIXED if args.precision == "mixed" else PS.FP64
+    other_prec = PS.FP64 if main_prec == PS.MIXED else PS.MIXED
+    s, d_x, main = run_variant(main_prec, args.steps, args.warmup)
+    _, _, other = run_variant(other_prec, max(3, args.steps // 4), 1)
     sz = s.sizes()
-    d_b = PS.DeviceArray.from_host(s, b)
-    d_x = PS.DeviceArray(s, mesh.n)
-    zeros = np.zeros(mesh.n)
-
-    for _ in range(args.warmup):
-        d_x.upload(zeros)
-        s.solve_device(d_b, d_x, rel_tol=1e-10, max_restarts=100)
-
-    # timed region: K solves, each from x0 = 0 (reset on the device between
-    # solves is part of the caller's work and stays inside)
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        d_x.upload(zeros)
-        stats.append(s.solve_device(d_b, d_x, rel_tol=1e-10, max_restarts=100))
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    res = s.residual_device(d_b, d_x)
-    conv = all(st["converged"] for st in stats)
+    ms_per_step = main["ms_per_step"]
 
     # SpMV roofline, live: stencil kernel (the solver's operator application)
     d_y = PS.DeviceArray(s, mesh.n)
@@ -109,6 +124,8 @@ def main():
     ms_csr = s.spmv_time(d_x, d_y, kind=1, reps=args.spmv_reps)
     bytes_spmv = spmv_bytes_csr(sz["n"], sz["nnz"])
     achieved = bytes_spmv / (ms_spmv * 1e-3) / 1e9
+    fp64 = main if main_prec == PS.FP64 else other
+    mixed = main if main_prec == PS.MIXED else other
 
     out = {
         "metric": METRIC,
@@ -121,7 +138,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if main_prec == PS.FP64 else "f32/bf16 inner, f64 refinement",
         "data": "synthetic",
         "config": {
             "workload": "3-level graded AMR Poisson solve (16x8 base blocks of 8x8 cells, disk "
@@ -129,10 +146,14 @@ def main():
                         "FP64 BiCGSTAB + 64x64 block-Jacobi to 1e-10 rel 2-norm residual",
             "n_blocks": mesh.n_blocks, "n": sz["n"], "nnz": sz["nnz"],
             "explicit_rows": sz["n_explicit"],
-            "iterations": stats[-1]["iterations"], "restarts": stats[-1]["restarts"],
-            "converged": conv, "true_rel2_residual": res["rel2"],
-            "solve_ms_device": float(np.median([st["ms"] for st in stats])),
+            "precision": args.precision,
+            "iterations": main["iterations"], "restarts": main["restarts"],
+            "converged": main["converged"], "true_rel2_residual": main["true_rel2_residual"],
+            "solve_ms_device": main["solve_ms_device"],
             "graph": args.graph,
+            "fp64": fp64,
+            "mixed": mixed,
+            "mixed_speedup_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
         },
         "roofline": {
             "bound": "hbm",

@@ -50,6 +50,7 @@ SIGNATURES = {
     "cup2d_ps_spmv_device": (_I, [_P, _P, _P, _I]),
     "cup2d_ps_spmv_time": (_I, [_P, _P, _P, _I, _I, _P]),
     "cup2d_ps_precond_device": (_I, [_P, _P, _P]),
+    "cup2d_ps_precond_bf16_device": (_I, [_P, _P, _P]),
     "cup2d_ps_residual_device": (_I, [_P, _P, _P, _P]),
     "cup2d_ps_sizes": (_I, [_P, _P, _P, _P, _P]),
     "cup2d_ps_get_csr": (_I, [_P, _P, _P, _P]),

@@ -7,8 +7,12 @@ namespace cup2d {
 
 constexpr int WAVES = 4;            // wavefronts per workgroup
 constexpr int WG = 64 * WAVES;      // threads per workgroup
+constexpr int TILE = 16;            // blocks per wavefront in the MFMA preconditioner
 
-// Device view of the operator in stencil form (see assembly.h).
+// Device view of the operator in stencil form (see assembly.h).  The
+// coefficients are FP64 in both precisions: FP32 coefficients would break the
+// operator's exact zero row sums (the Neumann null space) and make the
+// singular system inconsistent.
 struct OpView {
   int nb;
   int mean_row;                          // -1 when the mean row is not replaced
@@ -32,12 +36,17 @@ struct Scal {
   double alpha, beta, omega, rho, rho_prev;
   double err, err0, err_opt;
   double mean;                           // h^2-weighted mean of the solution
+  double mu;                             // mean of r removed at the top of the iteration
+  double err_ref;                        // last error that improved 2x on its predecessor
+  double n;                              // number of unknowns
   double tol_abs, tol_rel, eps, bd_tol;
   double red[8];                         // last reduction results (diagnostics)
   int iter, max_iter, restarts, max_restarts;
   int done, converged, improved, restart;
   int norm;
-  int pad;
+  int project;                           // remove the 1-component of r every iteration
+  int iter_ref;                          // iteration of err_ref
+  int stall;                             // end the solve after this many iterations without a 2x gain (0: never)
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -56,9 +65,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// max that propagates NaN (fmax would drop it and hide a failed iteration)
+__device__ __forceinline__ double nmax(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : fmax(a, b);
+}
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o >= 1; o >>= 1) v = nmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -79,7 +92,7 @@ __device__ void store_partials(double (&v)[NQ], double *part) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       double a = red[q][0];
-      for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, red[q][i]) : a + red[q][i];
+      for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? nmax(a, red[q][i]) : a + red[q][i];
       part[q * gridDim.x + blockIdx.x] = a;
     }
   }
@@ -97,7 +110,7 @@ __device__ void load_totals(const double *__restrict__ part, int G, double (&out
     double a = 0.0;
     for (int i = threadIdx.x; i < G; i += WG) {
       const double p = part[q * G + i];
-      a = ((MAXQ >> q) & 1) ? fmax(a, p) : a + p;
+      a = ((MAXQ >> q) & 1) ? nmax(a, p) : a + p;
     }
     a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
     if (lane == 0) fin[q][w] = a;
@@ -106,39 +119,42 @@ __device__ void load_totals(const double *__restrict__ part, int G, double (&out
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     double a = fin[q][0];
-    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? fmax(a, fin[q][i]) : a + fin[q][i];
+    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? nmax(a, fin[q][i]) : a + fin[q][i];
     out[q] = a;
   }
 }
 
 // 5-point stencil row of cell c of block b: sum over non-wall faces of
-// (x_nb - x_c).  xs is the wave's LDS copy of x on block b.
-__device__ __forceinline__ double stencil_row(const double *__restrict__ x, int4 nb, int c,
+// (x_nb - x_c), in FP64 whatever the storage type TV of x.  xs is the wave's
+// LDS copy of x on block b.
+template <class TV>
+__device__ __forceinline__ double stencil_row(const TV *__restrict__ x, int4 nb, int c,
                                               const double *xs, double xc) {
   const int ix = c & 7, iy = c >> 3;
   double y = 0.0;
   int deg = 0;
-  if (iy > 0) { y += xs[c - 8]; deg++; } else if (nb.z >= 0) { y += x[(long)nb.z * 64 + 56 + ix]; deg++; }
-  if (ix > 0) { y += xs[c - 1]; deg++; } else if (nb.x >= 0) { y += x[(long)nb.x * 64 + iy * 8 + 7]; deg++; }
-  if (ix < 7) { y += xs[c + 1]; deg++; } else if (nb.y >= 0) { y += x[(long)nb.y * 64 + iy * 8]; deg++; }
-  if (iy < 7) { y += xs[c + 8]; deg++; } else if (nb.w >= 0) { y += x[(long)nb.w * 64 + ix]; deg++; }
+  if (iy > 0) { y += xs[c - 8]; deg++; } else if (nb.z >= 0) { y += (double)x[(long)nb.z * 64 + 56 + ix]; deg++; }
+  if (ix > 0) { y += xs[c - 1]; deg++; } else if (nb.x >= 0) { y += (double)x[(long)nb.x * 64 + iy * 8 + 7]; deg++; }
+  if (ix < 7) { y += xs[c + 1]; deg++; } else if (nb.y >= 0) { y += (double)x[(long)nb.y * 64 + iy * 8]; deg++; }
+  if (iy < 7) { y += xs[c + 8]; deg++; } else if (nb.w >= 0) { y += (double)x[(long)nb.w * 64 + ix]; deg++; }
   return y - double(deg) * xc;
 }
 
-// y = A x on block b, one cell per lane.  xs: wave LDS scratch of 64 doubles
-// holding x on the block, ys: a second 64-double scratch.  The mean row, when
-// replaced, is left to the caller.
-__device__ __forceinline__ double apply_block(const OpView &A, const double *__restrict__ x, int b,
-                                              int c, double *xs, double *ys, double xc) {
+// y = A x on block b, one cell per lane, FP64 arithmetic.  xs: wave LDS
+// scratch of 64 doubles holding x on the block, ys: a second scratch.  The
+// mean row, when replaced, is left to the caller.
+template <class TV>
+__device__ __forceinline__ double apply_block(const OpView &A, const TV *__restrict__ x, int b, int c,
+                                              double *xs, double *ys, double xc) {
   const int4 nb = A.nbr[b];
-  double y = stencil_row(x, nb, c, xs, xc);
+  double y = stencil_row<TV>(x, nb, c, xs, xc);
   const unsigned long long m = A.mask[b];
   if (m) {
     const int e0 = A.bptr[b], e1 = A.bptr[b + 1];
     for (int e = e0 + c; e < e1; e += 64) {
       double acc = 0.0;
       const int k1 = A.rptr[e + 1];
-      for (int k = A.rptr[e]; k < k1; k++) acc += A.val[k] * x[A.col[k]];
+      for (int k = A.rptr[e]; k < k1; k++) acc += A.val[k] * (double)x[A.col[k]];
       ys[A.cell[e]] = acc;
     }
     wave_sync();
@@ -148,7 +164,7 @@ __device__ __forceinline__ double apply_block(const OpView &A, const double *__r
   return y;
 }
 
-// z = P v for NB blocks at once: lane c owns output row c; P (64x64,
+// z = P v for NB blocks at once (FP64): lane c owns output row c; P (64x64,
 // symmetric) sits in LDS, v holds the NB input vectors (64 each).
 template <int NB>
 __device__ __forceinline__ void prec_group(const double *Ps, const double *v, int c, double (&z)[NB]) {
@@ -161,5 +177,67 @@ __device__ __forceinline__ void prec_group(const double *Ps, const double *v, in
     for (int j = 0; j < NB; j++) z[j] = fma(pk, v[j * 64 + k], z[j]);
   }
 }
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA block preconditioner: Z(64 x 16 blocks) = P(64x64) R(64 x 16)
+// with v_mfma_f32_16x16x32_bf16.  A tile of TILE=16 blocks per wavefront:
+// lane l works on block tb + (l & 15), quarter q = l >> 4.
+//   input cells of the lane  : 32 ks + 8 q + j      (ks = 0,1; j = 0..7)
+//   output cells of the lane : 16 mt + 4 q + v      (mt = 0..3; v = 0..3)
+// A operand (P) of lane l for (mt, ks): P[16 mt + (l & 15)][32 ks + 8 q + j];
+// B operand for ks: R[32 ks + 8 q + j][block l & 15];
+// D of lane l, register v: Z[16 mt + 4 q + v][block l & 15].
+// The FP32 input is split into three bf16 terms (hi + mid + lo, three MFMAs
+// per step), which represent it to ~2^-24: a preconditioner whose input is
+// rounded coarser than that is no longer close enough to linear for
+// BiCGSTAB on the singular Neumann system.  P itself is the bf16 rounding of
+// P_inv, a fixed linear preconditioner.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct PFrag {
+  bf16x8 a[4][2];
+};
+
+__device__ __forceinline__ void load_pfrag(PFrag &P, const bf16x8 *__restrict__ pf, int lane) {
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) P.a[mt][ks] = pf[(mt * 2 + ks) * 64 + lane];
+}
+
+// in[ks*8 + j] = value at input cell 32 ks + 8 q + j; out[mt*4 + v] = value at
+// output cell 16 mt + 4 q + v.
+__device__ __forceinline__ void prec_mfma(const PFrag &P, const float (&in)[16], float (&out)[16]) {
+  bf16x8 hi[2], md[2], lo[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float v = in[ks * 8 + j];
+      const __bf16 h = (__bf16)v;
+      const float r1 = v - (float)h;
+      const __bf16 m = (__bf16)r1;
+      hi[ks][j] = h;
+      md[ks][j] = m;
+      lo[ks][j] = (__bf16)(r1 - (float)m);
+    }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.a[mt][ks], lo[ks], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.a[mt][ks], md[ks], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.a[mt][ks], hi[ks], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) out[mt * 4 + v] = acc[v];
+  }
+}
+
+__device__ __forceinline__ int in_cell(int q, int k) { return 32 * (k >> 3) + 8 * q + (k & 7); }
+__device__ __forceinline__ int out_cell(int q, int k) { return 16 * (k >> 2) + 4 * q + (k & 3); }
 
 }  // namespace cup2d

@@ -6,19 +6,26 @@
 // (ExpAMRSolver.cpp:25-95), restart on breakdown, best iterate x_opt, then
 // removal of the h^2-weighted mean (ExpAMRSolver.cpp:383-415).
 //
-// MI355X design: one wavefront per 8x8 block (lane = cell); every vector
-// pass of an iteration is fused into five kernels that keep all scalars on
-// the device; each reduction ends in the last-arriving workgroup, which also
-// runs the scalar recurrence (alpha, omega, beta, restart, convergence) so
-// that no iteration needs a host round trip.  Iterations are replayed in
-// fixed-size chunks from a hipGraph; kernels of a finished solve exit at
-// launch on the device-side `done` flag.
+// MI355X design:
+//  * one wavefront per 8x8 block in the operator kernels (lane = cell); the
+//    5-point part of the AMR operator is applied from a per-block neighbour
+//    table, only coarse-fine rows are read from a sparse list;
+//  * the vector passes of one iteration are fused into five kernels; every
+//    scalar (alpha, omega, beta, restart, convergence) lives on the device;
+//    a reduction is finished by its consumer kernel, which re-reduces the
+//    producer's per-workgroup partials (no device-wide fences or atomics);
+//  * iterations are replayed in chunks from a hipGraph; kernels of a
+//    finished solve exit at launch on the device-side `done` flag;
+//  * mixed precision: FP64 iterative refinement (r = b - A x and x += d in
+//    FP64) around an inner BiCGSTAB with FP32 vectors and SpMV, FP64
+//    accumulation of every inner product, and the block preconditioner as a
+//    BF16 MFMA contraction with FP32 accumulation.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -33,38 +40,48 @@ using namespace cup2d;
 
 namespace {
 
-constexpr int MAXG = 512;    // workgroups of the vector kernels (grid-stride beyond)
+constexpr int MAXG = 512;  // workgroups of the vector kernels (grid-stride beyond)
 
 // ----------------------------------------------------------------------------
-// kernels
+// scalar recurrence
 // ----------------------------------------------------------------------------
 
-__device__ __forceinline__ void load_P(double *Ps, const double *__restrict__ P) {
-  for (int i = threadIdx.x; i < 64 * 64; i += WG) Ps[i] = P[i];
-  __syncthreads();
-}
+constexpr int NQ5 = 6;  // totals of the x/r update: |r|_inf, |r|^2, (rhat,r), |rhat|^2, sum r, sum rhat
 
 // State at the top of iteration i from the state of iteration i-1 and the
-// totals o = (|r|_inf, |r|^2, (rhat,r), |rhat|^2) of iteration i-1's update
-// (BiCGSTAB.cu:471-509 and :572-600).  prev.iter < 0 marks the state after
-// r0 = b - A x0 (BiCGSTAB.cu:427-465).  Deterministic: recomputing it from
-// the same inputs gives the same state.
-__device__ Scal evaluate(const Scal &prev, const double (&o)[4]) {
+// totals o of iteration i-1's update (BiCGSTAB.cu:471-509 and :572-600).
+// prev.iter < 0 marks the state after r0 = b - A x0 (BiCGSTAB.cu:427-465).
+// With `project` (FP32-stored residuals) the component of r along the left
+// null vector 1 of the Neumann operator, mu = sum r / n, is removed: it is
+// rounding noise no iterate can cancel, and left in place it stalls the
+// iteration at ~1e-7 and then makes it diverge.  The totals are corrected
+// for it here and the kernels subtract mu from r.  Deterministic:
+// recomputing it from the same inputs gives the same state.
+__device__ Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
   Scal s = prev;
-  const double rr = o[1], rhr = o[2], hh = o[3];
-  const double err = prev.norm ? sqrt(rr) : o[0];
+  double rr = o[1], rhr = o[2], hh = o[3], amax = o[0];
+  s.mu = 0.0;
+  if (prev.project) {
+    const double mu = o[4] / prev.n;
+    s.mu = mu;
+    rr = fmax(rr - prev.n * mu * mu, 0.0);
+    rhr -= mu * o[5];
+    amax += fabs(mu);  // bound of max |r_i - mu|
+  }
+  const double err = prev.norm ? sqrt(rr) : amax;
   if (prev.iter < 0) {
-    s.err = s.err0 = s.err_opt = err;
-    s.iter = 0;
+    s.err = s.err0 = s.err_opt = s.err_ref = err;
+    s.iter = s.iter_ref = 0;
     s.alpha = s.omega = s.rho_prev = 1.0;
     s.rho = rhr;
     s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
     s.restarts = s.converged = s.improved = s.restart = 0;
-    s.done = (err == 0.0) ? 1 : 0;
+    s.done = (err == 0.0 || s.max_iter <= 0) ? 1 : 0;
     return s;
   }
   if (prev.done) {
     s.restart = 0;
+    s.mu = 0.0;
     return s;
   }
   s.err = err;
@@ -79,10 +96,19 @@ __device__ Scal evaluate(const Scal &prev, const double (&o)[4]) {
     }
   }
   if (!s.done && s.iter >= s.max_iter) s.done = 1;
+  // stagnation (mixed precision: the FP32 recurrence reached its floor; the
+  // caller restarts from the FP64 residual)
+  if (err < 0.5 * s.err_ref) {
+    s.err_ref = err;
+    s.iter_ref = s.iter;
+  }
+  if (!s.done && s.stall > 0 && s.iter - s.iter_ref > s.stall) s.done = 1;
   s.rho_prev = prev.rho;
   s.rho = rhr;
-  const bool breakdown = rhr * rhr < s.bd_tol * rr * hh;
   s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+  // breakdown test of BiCGSTAB.cu:483; a beta that would overflow FP32
+  // storage (omega ~ 0) or is not finite is handled the same way
+  const bool breakdown = rhr * rhr < s.bd_tol * rr * hh || !(fabs(s.beta) < 1e30);
   s.restart = 0;
   if (!s.done && breakdown && s.max_restarts > 0) {
     s.restarts += 1;
@@ -101,56 +127,86 @@ __device__ Scal evaluate(const Scal &prev, const double (&o)[4]) {
   return s;
 }
 
-// Partial sums of h^2 x0 (mean row of A x0 when the mean constraint is on).
-__global__ __launch_bounds__(WG) void k_init_mean(int nb, const double *__restrict__ x0,
-                                                  const double *__restrict__ h2, double *pm) {
+// Evaluates the state of this iteration in every workgroup (identically);
+// workgroup 0 stores it to slot `par`.
+__device__ __forceinline__ void top_of_iteration(Scal *st, int par, const double *p5, int G5,
+                                                 Scal &cur) {
+  double o[NQ5];
+  load_totals<NQ5, 1u>(p5, G5, o);
+  if (threadIdx.x == 0) cur = evaluate(st[1 - par], o);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) st[par] = cur;
+}
+
+// ----------------------------------------------------------------------------
+// initialisation
+// ----------------------------------------------------------------------------
+
+__device__ __forceinline__ void load_P(double *Ps, const double *__restrict__ P) {
+  for (int i = threadIdx.x; i < 64 * 64; i += WG) Ps[i] = P[i];
+  __syncthreads();
+}
+
+// Partial sums of h^2 x (the mean row of A x when the mean constraint is on).
+template <class T>
+__global__ __launch_bounds__(WG) void k_hsum(int nb, const T *__restrict__ x,
+                                             const double *__restrict__ h2, double *pm) {
   const long n = (long)nb * 64;
   double v[1] = {0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
-    v[0] += h2[i >> 6] * x0[i];
+    v[0] += h2[i >> 6] * (double)x[i];
   store_partials<1, 0u>(v, pm);
 }
 
-// r = b - A x0, rhat = r, p = nu = 0, x = x_opt = x0 (BiCGSTAB.cu:431-465);
-// partials of (|r|_inf, |r|^2, (rhat,r), |rhat|^2) for the first state.
+// r = b - A x0 in FP64, stored as TR to r and rhat; p = nu = 0; x = x_opt =
+// x0 (x may alias x0) (BiCGSTAB.cu:431-465); partials of the update totals
+// of the stored r.
+template <class TR>
 __global__ __launch_bounds__(WG) void k_init(OpView A, const double *__restrict__ b,
-                                             const double *__restrict__ x0, double *__restrict__ x,
-                                             double *__restrict__ xopt, double *__restrict__ r,
-                                             double *__restrict__ rhat, double *__restrict__ p,
-                                             double *__restrict__ nu, const double *pm, int Gm,
+                                             const double *x0, double *x,
+                                             double *__restrict__ xopt, TR *__restrict__ r,
+                                             TR *__restrict__ rhat, TR *__restrict__ p,
+                                             TR *__restrict__ nu, const double *pm, int Gm,
                                              double *p5) {
   __shared__ double xs[WAVES][64], ys[WAVES][64];
   double m0[1] = {0.0};
   if (A.mean_row >= 0) load_totals<1, 0u>(pm, Gm, m0);
   const int c = lane_id(), w = wave_id();
   const int nw = gridDim.x * WAVES;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double v[NQ5] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
     const long i = (long)bb * 64 + c;
     const double xc = x0[i];
     xs[w][c] = xc;
     wave_sync();
-    const double y = apply_block(A, x0, bb, c, xs[w], ys[w], xc);
+    const double y = apply_block<double>(A, x0, bb, c, xs[w], ys[w], xc);
     wave_sync();
     // mean row: b = 0 (ExpAMRSolver.cpp:340-345) and (A x)_row = sum h^2 x
     const double ri = (i == A.mean_row) ? 0.0 - m0[0] : b[i] - y;
-    r[i] = ri;
-    rhat[i] = ri;
-    p[i] = 0.0;
-    nu[i] = 0.0;
+    const TR rt = (TR)ri;
+    r[i] = rt;
+    rhat[i] = rt;
+    p[i] = TR(0);
+    nu[i] = TR(0);
     x[i] = xc;
     xopt[i] = xc;
-    v[0] = fmax(v[0], fabs(ri));
-    v[1] += ri * ri;
+    const double rd = (double)rt;
+    v[0] = nmax(v[0], fabs(rd));
+    v[1] += rd * rd;
+    v[4] += rd;
   }
   v[2] = v[1];
   v[3] = v[1];
-  store_partials<4, 1u>(v, p5);
+  v[5] = v[4];
+  store_partials<NQ5, 1u>(v, p5);
 }
 
-// Top of iteration: evaluate the state (every workgroup, identically; the
-// first one stores it), then p = r + beta (p - omega nu) (or p = rhat = r
-// after a restart), x_opt = x when the last iteration improved, z1 = P p
+// ----------------------------------------------------------------------------
+// iteration kernels, FP64 (vector-FMA preconditioner, P in LDS)
+// ----------------------------------------------------------------------------
+
+// Top of iteration: p = r + beta (p - omega nu) (or p = rhat = r after a
+// restart), x_opt = x when the last iteration improved, z1 = P p
 // (BiCGSTAB.cu:471-520).
 template <int NB>
 __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict__ P,
@@ -163,11 +219,7 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
   __shared__ Scal cur;
   __shared__ double Ps[64 * 64];
   __shared__ double vs[WAVES][NB * 64];
-  double o[4];
-  load_totals<4, 1u>(p5, G5, o);
-  if (threadIdx.x == 0) cur = evaluate(st[1 - par], o);
-  __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) st[par] = cur;
+  top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega;
   const int restart = cur.restart, improved = cur.improved;
@@ -203,39 +255,6 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
       if (bb < nb) z1[(long)bb * 64 + c] = z[j];
     }
   }
-}
-
-// State evaluation alone (end of an iteration chunk, one workgroup), so the
-// host sees convergence of the chunk's last iteration.
-__global__ __launch_bounds__(WG) void k_eval(Scal *st, int par, const double *p5, int G5) {
-  double o[4];
-  load_totals<4, 1u>(p5, G5, o);
-  if (threadIdx.x == 0) st[par] = evaluate(st[1 - par], o);
-}
-
-// nu = A z1; partials of (rhat, nu) and sum h^2 z1   (BiCGSTAB.cu:523-527)
-__global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, const double *__restrict__ z1,
-                                                double *__restrict__ nu,
-                                                const double *__restrict__ rhat, const Scal *st,
-                                                int par, double *p2) {
-  if (st[par].done) return;
-  __shared__ double xs[WAVES][64], ys[WAVES][64];
-  const int c = lane_id(), w = wave_id();
-  const int nw = gridDim.x * WAVES;
-  double v[2] = {0.0, 0.0};
-  for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
-    const long i = (long)bb * 64 + c;
-    const double zc = z1[i];
-    xs[w][c] = zc;
-    wave_sync();
-    double y = apply_block(A, z1, bb, c, xs[w], ys[w], zc);
-    wave_sync();
-    if (i == A.mean_row) y = 0.0;  // replaced by the consumer
-    nu[i] = y;
-    v[0] += rhat[i] * y;
-    v[1] += A.h2[bb] * zc;
-  }
-  store_partials<2, 0u>(v, p2);
 }
 
 // alpha = rho / ((rhat, nu) + eps); s = r - alpha nu (stored in r); z2 = P s
@@ -290,11 +309,187 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
   }
 }
 
+// ----------------------------------------------------------------------------
+// iteration kernels, FP32 inner solve (BF16 MFMA preconditioner)
+// ----------------------------------------------------------------------------
+
+// 16 values of the lane's input cells (32 ks + 8 q + j) of one block
+__device__ __forceinline__ void ld16(const float *__restrict__ a, long base, int q, float (&v)[16]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    const float4 *p4 = reinterpret_cast<const float4 *>(a + base + 32 * ks + 8 * q);
+    const float4 u0 = p4[0], u1 = p4[1];
+    v[ks * 8 + 0] = u0.x; v[ks * 8 + 1] = u0.y; v[ks * 8 + 2] = u0.z; v[ks * 8 + 3] = u0.w;
+    v[ks * 8 + 4] = u1.x; v[ks * 8 + 5] = u1.y; v[ks * 8 + 6] = u1.z; v[ks * 8 + 7] = u1.w;
+  }
+}
+__device__ __forceinline__ void st16_in(float *__restrict__ a, long base, int q, const float (&v)[16]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    float4 *p4 = reinterpret_cast<float4 *>(a + base + 32 * ks + 8 * q);
+    p4[0] = make_float4(v[ks * 8 + 0], v[ks * 8 + 1], v[ks * 8 + 2], v[ks * 8 + 3]);
+    p4[1] = make_float4(v[ks * 8 + 4], v[ks * 8 + 5], v[ks * 8 + 6], v[ks * 8 + 7]);
+  }
+}
+// 16 values of the lane's output cells (16 mt + 4 q + v)
+__device__ __forceinline__ void st16_out(float *__restrict__ a, long base, int q, const float (&v)[16]) {
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+    *reinterpret_cast<float4 *>(a + base + 16 * mt + 4 * q) =
+        make_float4(v[mt * 4 + 0], v[mt * 4 + 1], v[mt * 4 + 2], v[mt * 4 + 3]);
+}
+
+// Top of iteration (FP32 storage): r -= mu (null-space projection),
+// p = r + beta (p - omega nu) or p = rhat = r after a restart, x_opt = x
+// when the last iteration improved, z1 = P p on MFMA   (BiCGSTAB.cu:471-520)
+__global__ __launch_bounds__(WG) void k_pupdate_mf(int nb, const bf16x8 *__restrict__ pf,
+                                                   float *__restrict__ r,
+                                                   float *__restrict__ rhat, float *__restrict__ p,
+                                                   const float *__restrict__ nu,
+                                                   const double *__restrict__ x,
+                                                   double *__restrict__ xopt, float *__restrict__ z1,
+                                                   Scal *st, int par, const double *p5, int G5) {
+  __shared__ Scal cur;
+  top_of_iteration(st, par, p5, G5, cur);
+  if (cur.done) return;
+  const double beta = cur.beta, omega = cur.omega, mu = cur.mu;
+  const int restart = cur.restart, improved = cur.improved;
+  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
+  for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
+    const int bb = tile * TILE + j;
+    const bool ok = bb < nb;
+    const long base = (long)bb * 64;
+    float pin[16];
+    if (ok) {
+      float rv[16];
+      ld16(r, base, q, rv);
+      if (mu != 0.0) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) rv[k] = (float)((double)rv[k] - mu);
+        st16_in(r, base, q, rv);
+      }
+      if (restart) {
+        st16_in(rhat, base, q, rv);
+#pragma unroll
+        for (int k = 0; k < 16; k++) pin[k] = rv[k];
+      } else {
+        float pv[16], nv[16];
+        ld16(p, base, q, pv);
+        ld16(nu, base, q, nv);
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+          pin[k] = (float)((double)rv[k] + beta * ((double)pv[k] - omega * (double)nv[k]));
+      }
+      st16_in(p, base, q, pin);
+      if (improved) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) xopt[base + in_cell(q, k)] = x[base + in_cell(q, k)];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) pin[k] = 0.f;
+    }
+    float z[16];
+    prec_mfma(P, pin, z);
+    if (ok) st16_out(z1, base, q, z);
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restrict__ pf,
+                                                   float *__restrict__ r, float *__restrict__ nu,
+                                                   const float *__restrict__ rhat,
+                                                   float *__restrict__ z2, Scal *st, int par,
+                                                   const double *p2, int G2, long mean_row) {
+  if (st[par].done) return;
+  double o[2];
+  load_totals<2, 0u>(p2, G2, o);
+  double rn = o[0];
+  if (mean_row >= 0) rn += (double)rhat[mean_row] * o[1];
+  const double alpha = st[par].rho / (rn + st[par].eps);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[par].alpha = alpha;
+    st[par].red[2] = rn;
+  }
+  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
+  for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
+    const int bb = tile * TILE + j;
+    const bool ok = bb < nb;
+    const long base = (long)bb * 64;
+    float sv[16];
+    if (ok) {
+      float rv[16], nv[16];
+      ld16(r, base, q, rv);
+      ld16(nu, base, q, nv);
+      if (mean_row >= base && mean_row < base + 64) {
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+          if (base + in_cell(q, k) == mean_row) {
+            nv[k] = (float)o[1];
+            nu[mean_row] = nv[k];
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < 16; k++) sv[k] = (float)((double)rv[k] - alpha * (double)nv[k]);
+      st16_in(r, base, q, sv);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) sv[k] = 0.f;
+    }
+    float z[16];
+    prec_mfma(P, sv, z);
+    if (ok) st16_out(z2, base, q, z);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// iteration kernels shared by both precisions
+// ----------------------------------------------------------------------------
+
+// State evaluation alone (end of an iteration chunk, one workgroup), so the
+// host sees convergence of the chunk's last iteration.
+__global__ __launch_bounds__(WG) void k_eval(Scal *st, int par, const double *p5, int G5) {
+  double o[NQ5];
+  load_totals<NQ5, 1u>(p5, G5, o);
+  if (threadIdx.x == 0) st[par] = evaluate(st[1 - par], o);
+}
+
+// nu = A z1; partials of (rhat, nu) and sum h^2 z1   (BiCGSTAB.cu:523-527)
+template <class T>
+__global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, const T *__restrict__ z1,
+                                                T *__restrict__ nu, const T *__restrict__ rhat,
+                                                const Scal *st, int par, double *p2) {
+  if (st[par].done) return;
+  __shared__ double xs[WAVES][64], ys[WAVES][64];
+  const int c = lane_id(), w = wave_id();
+  const int nw = gridDim.x * WAVES;
+  double v[2] = {0.0, 0.0};
+  for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
+    const long i = (long)bb * 64 + c;
+    const double zc = (double)z1[i];
+    xs[w][c] = zc;
+    wave_sync();
+    double y = apply_block<T>(A, z1, bb, c, xs[w], ys[w], zc);
+    wave_sync();
+    if (i == A.mean_row) y = 0.0;  // replaced by the consumer
+    const T yt = (T)y;
+    nu[i] = yt;
+    v[0] += (double)rhat[i] * (double)yt;
+    v[1] += A.h2[bb] * zc;
+  }
+  store_partials<2, 0u>(v, p2);
+}
+
 // t = A z2; partials of (t, s), (t, t) and sum h^2 z2   (BiCGSTAB.cu:548-553)
-__global__ __launch_bounds__(WG) void k_spmv_t(OpView A, const double *__restrict__ z2,
-                                               double *__restrict__ t,
-                                               const double *__restrict__ sv, const Scal *st,
-                                               int par, double *p4) {
+template <class T>
+__global__ __launch_bounds__(WG) void k_spmv_t(OpView A, const T *__restrict__ z2,
+                                               T *__restrict__ t, const T *__restrict__ sv,
+                                               const Scal *st, int par, double *p4) {
   if (st[par].done) return;
   __shared__ double xs[WAVES][64], ys[WAVES][64];
   const int c = lane_id(), w = wave_id();
@@ -302,36 +497,38 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, const double *__restric
   double v[3] = {0.0, 0.0, 0.0};
   for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
     const long i = (long)bb * 64 + c;
-    const double zc = z2[i];
+    const double zc = (double)z2[i];
     xs[w][c] = zc;
     wave_sync();
-    double y = apply_block(A, z2, bb, c, xs[w], ys[w], zc);
+    double y = apply_block<T>(A, z2, bb, c, xs[w], ys[w], zc);
     wave_sync();
     if (i == A.mean_row) y = 0.0;
-    t[i] = y;
-    v[0] += y * sv[i];
-    v[1] += y * y;
+    const T yt = (T)y;
+    t[i] = yt;
+    const double yd = (double)yt;
+    v[0] += yd * (double)sv[i];
+    v[1] += yd * yd;
     v[2] += A.h2[bb] * zc;
   }
   store_partials<3, 0u>(v, p4);
 }
 
-// omega = (t, s) / ((t, t) + eps); x += alpha z1 + omega z2; r = s - omega t;
-// partials of (|r|_inf, |r|^2, (rhat, r), |rhat|^2)   (BiCGSTAB.cu:552-572)
+// omega = (t, s) / ((t, t) + eps); x += alpha z1 + omega z2 (x in FP64);
+// r = s - omega t; partials of the update totals   (BiCGSTAB.cu:552-572)
+template <class T>
 __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
-                                                 const double *__restrict__ z1,
-                                                 const double *__restrict__ z2,
-                                                 double *__restrict__ r,
-                                                 const double *__restrict__ t,
-                                                 const double *__restrict__ rhat, Scal *st,
-                                                 int par, const double *p4, int G4, double *p5,
+                                                 const T *__restrict__ z1,
+                                                 const T *__restrict__ z2, T *__restrict__ r,
+                                                 const T *__restrict__ t,
+                                                 const T *__restrict__ rhat, Scal *st, int par,
+                                                 const double *p4, int G4, double *p5,
                                                  long mean_row) {
   if (st[par].done) return;
   double o[3];
   load_totals<3, 0u>(p4, G4, o);
   double ts = o[0], tt = o[1];
   if (mean_row >= 0) {
-    ts += o[2] * r[mean_row];
+    ts += o[2] * (double)r[mean_row];
     tt += o[2] * o[2];
   }
   const double omega = ts / (tt + st[par].eps);
@@ -342,27 +539,44 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
     st[par].red[4] = tt;
   }
   const long n = (long)nb * 64;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double v[NQ5] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
-    x[i] = x[i] + alpha * z1[i] + omega * z2[i];
-    const double ti = (i == mean_row) ? o[2] : t[i];
-    const double ri = r[i] - omega * ti;
-    r[i] = ri;
-    const double rh = rhat[i];
-    v[0] = fmax(v[0], fabs(ri));
-    v[1] += ri * ri;
-    v[2] += rh * ri;
+    x[i] = x[i] + alpha * (double)z1[i] + omega * (double)z2[i];
+    const double ti = (i == mean_row) ? o[2] : (double)t[i];
+    const T rt = (T)((double)r[i] - omega * ti);
+    r[i] = rt;
+    const double rd = (double)rt, rh = (double)rhat[i];
+    v[0] = nmax(v[0], fabs(rd));
+    v[1] += rd * rd;
+    v[2] += rh * rd;
     v[3] += rh * rh;
+    v[4] += rd;
+    v[5] += rh;
   }
-  store_partials<4, 1u>(v, p5);
+  store_partials<NQ5, 1u>(v, p5);
 }
 
-// mean of the returned iterate (x if the last iteration improved, else x_opt)
+// ----------------------------------------------------------------------------
+// end of solve
+// ----------------------------------------------------------------------------
+
+// x = x_opt unless the last iteration was the best one (end of an inner
+// solve of the mixed solver, before the FP64 residual check)
+__global__ __launch_bounds__(WG) void k_select_best(int nb, double *__restrict__ x,
+                                                    const double *__restrict__ xopt, const Scal *st,
+                                                    int par) {
+  if (st[par].improved) return;
+  const long n = (long)nb * 64;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) x[i] = xopt[i];
+}
+
+// mean of the returned iterate (x if the last iteration improved or
+// `always_x`, else x_opt)
 __global__ __launch_bounds__(WG) void k_final_mean(int nb, const double *__restrict__ x,
                                                    const double *__restrict__ xopt,
                                                    const double *__restrict__ h2, const Scal *st,
-                                                   int par, double *pf) {
-  const double *src = st[par].improved ? x : xopt;
+                                                   int par, int always_x, double *pf) {
+  const double *src = (always_x || st[par].improved) ? x : xopt;
   const long n = (long)nb * 64;
   double v[2] = {0.0, 0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
@@ -376,32 +590,37 @@ __global__ __launch_bounds__(WG) void k_final_mean(int nb, const double *__restr
 __global__ __launch_bounds__(WG) void k_final_write(int nb, const double *__restrict__ x,
                                                     const double *__restrict__ xopt,
                                                     double *__restrict__ out, Scal *st, int par,
-                                                    const double *pf, int Gf) {
+                                                    int always_x, const double *pf, int Gf) {
   double o[2];
   load_totals<2, 0u>(pf, Gf, o);
   const double mean = o[0] / o[1];
-  const double *src = st[par].improved ? x : xopt;
+  const double *src = (always_x || st[par].improved) ? x : xopt;
   if (blockIdx.x == 0 && threadIdx.x == 0) st[par].mean = mean;
   const long n = (long)nb * 64;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
     out[i] = src[i] - mean;
 }
 
+// ----------------------------------------------------------------------------
+// operator pieces exposed by the C ABI
+// ----------------------------------------------------------------------------
+
 // y = A x (stencil form); partials of sum h^2 x for the mean row
-__global__ __launch_bounds__(WG) void k_spmv(OpView A, const double *__restrict__ x,
-                                             double *__restrict__ y, double *pm) {
+template <class T>
+__global__ __launch_bounds__(WG) void k_spmv(OpView A, const T *__restrict__ x,
+                                             T *__restrict__ y, double *pm) {
   __shared__ double xs[WAVES][64], ys[WAVES][64];
   const int c = lane_id(), w = wave_id();
   const int nw = gridDim.x * WAVES;
   double v[1] = {0.0};
   for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
     const long i = (long)bb * 64 + c;
-    const double xc = x[i];
+    const double xc = (double)x[i];
     xs[w][c] = xc;
     wave_sync();
-    const double yi = apply_block(A, x, bb, c, xs[w], ys[w], xc);
+    const double yi = apply_block<T>(A, x, bb, c, xs[w], ys[w], xc);
     wave_sync();
-    y[i] = yi;
+    y[i] = (T)yi;
     v[0] += A.h2[bb] * xc;
   }
   if (A.mean_row >= 0) store_partials<1, 0u>(v, pm);
@@ -422,14 +641,15 @@ __global__ __launch_bounds__(WG) void k_spmv_csr(long n, const int *__restrict__
 }
 
 // y[mean_row] = sum h^2 x from the partials (one workgroup)
+template <class T>
 __global__ __launch_bounds__(WG) void k_meanrow_fix(const double *pm, int G, long mean_row,
-                                                    double *__restrict__ y) {
+                                                    T *__restrict__ y) {
   double o[1];
   load_totals<1, 0u>(pm, G, o);
-  if (threadIdx.x == 0) y[mean_row] = o[0];
+  if (threadIdx.x == 0) y[mean_row] = (T)o[0];
 }
 
-// z = P r on every block
+// z = P r on every block, FP64
 template <int NB>
 __global__ __launch_bounds__(WG) void k_prec(int nb, const double *__restrict__ P,
                                              const double *__restrict__ r,
@@ -457,6 +677,31 @@ __global__ __launch_bounds__(WG) void k_prec(int nb, const double *__restrict__ 
   }
 }
 
+// z = P r on every block, BF16 MFMA (FP32 in/out)
+__global__ __launch_bounds__(WG) void k_prec_mf(int nb, const bf16x8 *__restrict__ pf,
+                                                const float *__restrict__ r,
+                                                float *__restrict__ z) {
+  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
+  for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
+    const int bb = tile * TILE + j;
+    const bool ok = bb < nb;
+    const long base = (long)bb * 64;
+    float v[16];
+    if (ok) {
+      ld16(r, base, q, v);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) v[k] = 0.f;
+    }
+    float out[16];
+    prec_mfma(P, v, out);
+    if (ok) st16_out(z, base, q, out);
+  }
+}
+
 // partials of |b - y|^2, |b - y|_inf, |b|^2, |b|_inf; y holds A x
 __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restrict__ b,
                                                  const double *__restrict__ y, long mean_row,
@@ -479,11 +724,14 @@ __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restric
 
 std::string g_create_err;
 
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-};
+// float -> bf16 bits, round to nearest even (finite inputs)
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return uint16_t(u >> 16);
+}
 
 }  // namespace
 
@@ -491,6 +739,7 @@ struct cup2d_ps {
   cup2d_ps_config cfg{};
   std::string err;
   int device = 0;
+  bool mixed = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_mesh = false;
@@ -503,16 +752,21 @@ struct cup2d_ps {
   int *d_csr_rp = nullptr, *d_csr_col = nullptr;
   double *d_csr_val = nullptr;
   double *d_P = nullptr;
-  // work vectors
+  bf16x8 *d_pf = nullptr;                // BF16 MFMA fragments of P_inv
+  // FP64 work vectors (x is the solution in both precisions)
   double *d_x = nullptr, *d_xopt = nullptr, *d_r = nullptr, *d_rhat = nullptr, *d_p = nullptr,
          *d_nu = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_t = nullptr, *d_b = nullptr,
          *d_xio = nullptr;
-  Scal *d_s = nullptr;                 // two state slots (iteration parity)
-  double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr, *d_pf = nullptr;
-  Scal *h_s = nullptr;                 // pinned copy of the state slots
-  int G = 1;                           // workgroups of the SpMV / vector kernels
-  int Gp = 1;                          // workgroups of the preconditioner kernels
-  int nbp = 2;                         // blocks per wave in the preconditioner kernels
+  // FP32-stored Krylov vectors of the mixed-precision solver (x stays FP64)
+  float *f_r = nullptr, *f_rhat = nullptr, *f_p = nullptr, *f_nu = nullptr, *f_z1 = nullptr,
+        *f_z2 = nullptr, *f_t = nullptr;
+  Scal *d_s = nullptr;                   // two state slots (iteration parity)
+  double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr, *d_pfin = nullptr;
+  Scal *h_s = nullptr;                   // pinned copy of the state slots
+  int G = 1;                             // workgroups of the SpMV / vector kernels
+  int Gp = 1;                            // workgroups of the FP64 preconditioner kernels
+  int Gm = 1;                            // workgroups of the MFMA preconditioner kernels
+  int nbp = 2;                           // blocks per wave in the FP64 preconditioner kernels
   // graph of one chunk of iterations
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
@@ -615,35 +869,57 @@ std::vector<double> make_pinv() {
   return P;
 }
 
-// Launches one BiCGSTAB iteration (parity `par` selects the state slot).
-void launch_iteration(cup2d_ps *h, int par) {
+// P in the A-operand layout of v_mfma_f32_16x16x32_bf16 (see kernels.h):
+// fragment (mt, ks), lane l, element j = P[16 mt + (l & 15)][32 ks + 8 (l >> 4) + j].
+std::vector<uint16_t> make_pfrag(const std::vector<double> &P) {
+  std::vector<uint16_t> f(8 * 64 * 8);
+  for (int mt = 0; mt < 4; mt++)
+    for (int ks = 0; ks < 2; ks++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 8; j++) {
+          const int row = 16 * mt + (l & 15), col = 32 * ks + 8 * (l >> 4) + j;
+          f[((mt * 2 + ks) * 64 + l) * 8 + j] = bf16_bits((float)P[row * 64 + col]);
+        }
+  return f;
+}
+
+template <int NB>
+void launch_iter_fp64(cup2d_ps *h, int par) {
   const OpView A = view(h);
-  const int G = h->G, Gp = h->Gp;
-  const int nb = (int)h->op.nb;
+  const int G = h->G, Gp = h->Gp, nb = (int)h->op.nb;
   const long mr = h->op.mean_row;
   hipStream_t st = h->stream;
-  if (h->nbp == 4)
-    k_pupdate<4><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
-                                    h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
-  else if (h->nbp == 2)
-    k_pupdate<2><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
-                                    h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
-  else
-    k_pupdate<1><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
-                                    h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
-  k_spmv_nu<<<G, WG, 0, st>>>(A, h->d_z1, h->d_nu, h->d_rhat, h->d_s, par, h->d_p2);
-  if (h->nbp == 4)
-    k_supdate<4><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
-                                    h->d_p2, G, mr);
-  else if (h->nbp == 2)
-    k_supdate<2><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
-                                    h->d_p2, G, mr);
-  else
-    k_supdate<1><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
-                                    h->d_p2, G, mr);
-  k_spmv_t<<<G, WG, 0, st>>>(A, h->d_z2, h->d_t, h->d_r, h->d_s, par, h->d_p4);
-  k_xrupdate<<<G, WG, 0, st>>>(nb, h->d_x, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat, h->d_s,
-                               par, h->d_p4, G, h->d_p5, mr);
+  k_pupdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
+                                   h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
+  k_spmv_nu<double><<<G, WG, 0, st>>>(A, h->d_z1, h->d_nu, h->d_rhat, h->d_s, par, h->d_p2);
+  k_supdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
+                                   h->d_p2, G, mr);
+  k_spmv_t<double><<<G, WG, 0, st>>>(A, h->d_z2, h->d_t, h->d_r, h->d_s, par, h->d_p4);
+  k_xrupdate<double><<<G, WG, 0, st>>>(nb, h->d_x, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat,
+                                       h->d_s, par, h->d_p4, G, h->d_p5, mr);
+}
+
+void launch_iter_mixed(cup2d_ps *h, int par) {
+  const OpView A = view(h);
+  const int G = h->G, Gm = h->Gm, nb = (int)h->op.nb;
+  const long mr = h->op.mean_row;
+  hipStream_t st = h->stream;
+  k_pupdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_rhat, h->f_p, h->f_nu, h->d_x,
+                                  h->d_xopt, h->f_z1, h->d_s, par, h->d_p5, G);
+  k_spmv_nu<float><<<G, WG, 0, st>>>(A, h->f_z1, h->f_nu, h->f_rhat, h->d_s, par, h->d_p2);
+  k_supdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_nu, h->f_rhat, h->f_z2, h->d_s, par,
+                                  h->d_p2, G, mr);
+  k_spmv_t<float><<<G, WG, 0, st>>>(A, h->f_z2, h->f_t, h->f_r, h->d_s, par, h->d_p4);
+  k_xrupdate<float><<<G, WG, 0, st>>>(nb, h->d_x, h->f_z1, h->f_z2, h->f_r, h->f_t, h->f_rhat,
+                                      h->d_s, par, h->d_p4, G, h->d_p5, mr);
+}
+
+// Launches one BiCGSTAB iteration (parity `par` selects the state slot).
+void launch_iteration(cup2d_ps *h, int par) {
+  if (h->mixed) launch_iter_mixed(h, par);
+  else if (h->nbp == 4) launch_iter_fp64<4>(h, par);
+  else if (h->nbp == 2) launch_iter_fp64<2>(h, par);
+  else launch_iter_fp64<1>(h, par);
 }
 
 // Iterations per chunk: even, so that every chunk starts on state slot 0.
@@ -678,31 +954,9 @@ int ensure_graph(cup2d_ps *h) {
   return CUP2D_PS_OK;
 }
 
-int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
-               double tol_rel, int max_restarts, cup2d_ps_stats *st) {
-  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "solve before cup2d_ps_set_mesh");
-  HIPCHK(h, hipSetDevice(h->device));
-  const int G = h->G;
-  const int nb = (int)h->op.nb;
-  // slot 1 holds the marker state from which the first evaluation starts
-  Scal init{};
-  init.tol_abs = tol_abs;
-  init.tol_rel = tol_rel;
-  init.eps = h->cfg.eps;
-  init.bd_tol = h->cfg.breakdown_tol;
-  init.max_iter = h->cfg.max_iter;
-  init.max_restarts = max_restarts;
-  init.norm = h->cfg.norm;
-  init.iter = -1;
-  h->h_s[0] = init;
-  h->h_s[1] = init;
-  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
-  const OpView A = view(h);
-  if (h->op.mean_row >= 0) k_init_mean<<<G, WG, 0, h->stream>>>(nb, d_x0, h->d_h2, h->d_pm);
-  k_init<<<G, WG, 0, h->stream>>>(A, d_b, d_x0, h->d_x, h->d_xopt, h->d_r, h->d_rhat, h->d_p,
-                                  h->d_nu, h->d_pm, G, h->d_p5);
-  HIPCHK(h, hipGetLastError());
+// Runs iteration chunks until the device state reports done (or max_iter +
+// one chunk have been launched).
+int run_iterations(cup2d_ps *h, int max_iter) {
   const int C = chunk_of(h);
   if (h->cfg.use_graph) {
     int rc = ensure_graph(h);
@@ -720,11 +974,53 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
     HIPCHK(h, hipMemcpyAsync(&h->h_s[0].done, &h->d_s[0].done, sizeof(int),
                              hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->h_s[0].done || launched > h->cfg.max_iter + C) break;
+    if (h->h_s[0].done || launched > max_iter + C) break;
   }
-  k_final_mean<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, h->d_h2, h->d_s, 0, h->d_pf);
-  k_final_write<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, d_out, h->d_s, 0, h->d_pf, G);
+  return CUP2D_PS_OK;
+}
+
+Scal marker_state(const cup2d_ps *h, double tol_abs, double tol_rel, int max_restarts, int max_iter) {
+  Scal s{};
+  s.n = (double)h->op.n;
+  // the projection applies to the singular Neumann operator only: with the
+  // mean constraint the replaced row makes A nonsingular and 1 is no longer
+  // its left null vector
+  s.project = (h->mixed && h->op.mean_row < 0) ? 1 : 0;
+  s.stall = h->mixed ? 150 : 0;
+  s.tol_abs = tol_abs;
+  s.tol_rel = tol_rel;
+  s.eps = h->cfg.eps;
+  s.bd_tol = h->cfg.breakdown_tol;
+  s.max_iter = max_iter;
+  s.max_restarts = max_restarts;
+  s.norm = h->cfg.norm;
+  s.iter = -1;
+  return s;
+}
+
+int finish(cup2d_ps *h, double *d_out, int always_x) {
+  const int G = h->G, nb = (int)h->op.nb;
+  k_final_mean<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, h->d_h2, h->d_s, 0, always_x,
+                                        h->d_pfin);
+  k_final_write<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, d_out, h->d_s, 0, always_x,
+                                         h->d_pfin, G);
   HIPCHK(h, hipGetLastError());
+  return CUP2D_PS_OK;
+}
+
+int solve_fp64(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
+               double tol_rel, int max_restarts, cup2d_ps_stats *st) {
+  const int G = h->G, nb = (int)h->op.nb;
+  h->h_s[0] = h->h_s[1] = marker_state(h, tol_abs, tol_rel, max_restarts, h->cfg.max_iter);
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
+  if (h->op.mean_row >= 0) k_hsum<double><<<G, WG, 0, h->stream>>>(nb, d_x0, h->d_h2, h->d_pm);
+  k_init<double><<<G, WG, 0, h->stream>>>(view(h), d_b, d_x0, h->d_x, h->d_xopt, h->d_r,
+                                          h->d_rhat, h->d_p, h->d_nu, h->d_pm, G, h->d_p5);
+  HIPCHK(h, hipGetLastError());
+  int rc = run_iterations(h, h->cfg.max_iter);
+  if (rc) return rc;
+  if ((rc = finish(h, d_out, 0))) return rc;
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, 2 * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -734,7 +1030,6 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
     st->iterations = f.iter;
     st->restarts = f.restarts;
     st->converged = f.converged;
-    st->refinements = 0;
     st->err_init = f.err0;
     st->err_final = f.err_opt;
     float ms = 0;
@@ -742,6 +1037,95 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
     st->ms = ms;
   }
   return CUP2D_PS_OK;
+}
+
+// FP64 norms of r = b - A x from the k_init partials (host side).
+int outer_residual(cup2d_ps *h, double &err) {
+  std::vector<double> part(NQ5 * (size_t)h->G);
+  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_p5, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double rinf = 0, r2 = 0;
+  for (int i = 0; i < h->G; i++) {
+    const double a = part[i];
+    rinf = (a != a || rinf != rinf) ? a : std::max(rinf, a);
+    r2 += part[h->G + i];
+  }
+  err = h->cfg.norm ? std::sqrt(r2) : rinf;
+  return CUP2D_PS_OK;
+}
+
+// Mixed precision: BiCGSTAB whose Krylov vectors (r, rhat, p, nu, z, t) are
+// stored in FP32 and whose preconditioner runs on BF16 MFMA, with the
+// solution x, every inner product and all arithmetic in FP64; r is projected
+// onto the range of the Neumann operator every iteration.  FP64 iterative
+// refinement wraps it: r = b - A x is recomputed in FP64 from x, the
+// tolerance is checked on that true residual, and if the FP32 recurrence
+// stopped short the iteration restarts from the current x.
+int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
+                double tol_rel, int max_restarts, cup2d_ps_stats *st) {
+  const int G = h->G, nb = (int)h->op.nb;
+  const size_t bytes = h->op.n * sizeof(double);
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_x, d_x0, bytes, hipMemcpyDeviceToDevice, h->stream));
+  const OpView A = view(h);
+  int total_iters = 0, total_restarts = 0, outer = 0;
+  double err0 = -1, err = 0;
+  bool conv = false;
+  const int max_outer = 16;
+  for (;;) {
+    if (h->op.mean_row >= 0) k_hsum<double><<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_h2, h->d_pm);
+    k_init<float><<<G, WG, 0, h->stream>>>(A, d_b, h->d_x, h->d_x, h->d_xopt, h->f_r, h->f_rhat,
+                                           h->f_p, h->f_nu, h->d_pm, G, h->d_p5);
+    HIPCHK(h, hipGetLastError());
+    int rc = outer_residual(h, err);
+    if (rc) return rc;
+    if (err != err) return fail(h, CUP2D_PS_EINVAL, "non-finite residual (NaN/Inf in b or x0)");
+    if (err0 < 0) err0 = err;
+    if (err == 0.0 || err <= tol_abs || err / err0 <= tol_rel) {
+      conv = true;
+      break;
+    }
+    if (outer >= max_outer || total_iters >= h->cfg.max_iter) break;
+    // iterate towards the full remaining reduction; the FP64 check above
+    // decides whether another pass is needed
+    const double rel = tol_rel > 0 ? tol_rel * err0 / err : 0.0;
+    h->h_s[0] = h->h_s[1] = marker_state(h, tol_abs, rel, max_restarts, h->cfg.max_iter - total_iters);
+    HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
+    if ((rc = run_iterations(h, h->cfg.max_iter - total_iters))) return rc;
+    k_select_best<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, h->d_s, 0);
+    HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    total_iters += h->h_s[0].iter;
+    total_restarts += h->h_s[0].restarts;
+    outer++;
+    if (h->h_s[0].iter == 0) break;  // no progress possible
+  }
+  int rc = finish(h, d_out, 1);
+  if (rc) return rc;
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->iterations = total_iters;
+    st->restarts = total_restarts;
+    st->converged = conv;
+    st->refinements = outer;
+    st->err_init = err0;
+    st->err_final = err;
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    st->ms = ms;
+  }
+  return CUP2D_PS_OK;
+}
+
+int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
+               double tol_rel, int max_restarts, cup2d_ps_stats *st) {
+  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "solve before cup2d_ps_set_mesh");
+  HIPCHK(h, hipSetDevice(h->device));
+  return h->mixed ? solve_mixed(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st)
+                  : solve_fp64(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st);
 }
 
 }  // namespace
@@ -759,7 +1143,7 @@ void cup2d_ps_config_default(cup2d_ps_config *cfg) {
   cfg->use_graph = 1;
   cfg->eps = 1e-100;
   cfg->breakdown_tol = 1e-16;
-  cfg->inner_tol = 1e-3;
+  cfg->inner_tol = 1e-4;
 }
 
 int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
@@ -768,12 +1152,10 @@ int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
   cup2d_ps_config c;
   if (cfg) c = *cfg;
   else cup2d_ps_config_default(&c);
-  if (c.max_iter <= 0 || c.chunk < 0 || c.eps < 0 || c.breakdown_tol < 0 ||
-      (c.precision != CUP2D_PS_FP64 && c.precision != CUP2D_PS_MIXED) ||
+  if (c.max_iter <= 0 || c.chunk < 0 || c.eps < 0 || c.breakdown_tol < 0 || c.inner_tol <= 0 ||
+      c.inner_tol >= 1 || (c.precision != CUP2D_PS_FP64 && c.precision != CUP2D_PS_MIXED) ||
       (c.norm != CUP2D_PS_NORM_INF && c.norm != CUP2D_PS_NORM_L2))
     return fail(nullptr, CUP2D_PS_EINVAL, "invalid configuration");
-  if (c.precision == CUP2D_PS_MIXED)
-    return fail(nullptr, CUP2D_PS_EINVAL, "mixed precision is not available in this build");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return fail(nullptr, CUP2D_PS_ENODEV, "no HIP device");
@@ -786,6 +1168,7 @@ int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
   auto h = std::make_unique<cup2d_ps>();
   h->cfg = c;
   h->device = c.device;
+  h->mixed = c.precision == CUP2D_PS_MIXED;
   if (hipSetDevice(c.device) != hipSuccess) return fail(nullptr, CUP2D_PS_EHIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
@@ -823,21 +1206,29 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
   for (long long b = 0; b < op.nb; b++)
     nbr[b] = make_int4(op.nbr[4 * b], op.nbr[4 * b + 1], op.nbr[4 * b + 2], op.nbr[4 * b + 3]);
   std::vector<unsigned long long> mask(op.exc_mask.begin(), op.exc_mask.end());
+  const std::vector<double> P = make_pinv();
+  const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = upload(h, &h->d_nbr, nbr)) || (rc = upload(h, &h->d_mask, mask)) ||
       (rc = upload(h, &h->d_bptr, op.exc_bptr)) || (rc = upload(h, &h->d_cell, op.exc_cell)) ||
       (rc = upload(h, &h->d_rptr, op.exc_rptr)) || (rc = upload(h, &h->d_col, op.exc_col)) ||
-      (rc = upload(h, &h->d_val, op.exc_val)) || (rc = upload(h, &h->d_h2, op.h2)) ||
-      (rc = upload(h, &h->d_csr_rp, op.rp)) || (rc = upload(h, &h->d_csr_col, op.col)) ||
-      (rc = upload(h, &h->d_csr_val, op.val)) || (rc = upload(h, &h->d_P, make_pinv())))
+      (rc = upload(h, &h->d_val, op.exc_val)) ||
+      (rc = upload(h, &h->d_h2, op.h2)) || (rc = upload(h, &h->d_csr_rp, op.rp)) ||
+      (rc = upload(h, &h->d_csr_col, op.col)) || (rc = upload(h, &h->d_csr_val, op.val)) ||
+      (rc = upload(h, &h->d_P, P)))
     return rc;
+  if ((rc = dalloc(h, &h->d_pf, 8 * 64))) return rc;
+  HIPCHK(h, hipMemcpy(h->d_pf, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   const size_t n = op.n;
   double **vecs[] = {&h->d_x, &h->d_xopt, &h->d_r, &h->d_rhat, &h->d_p, &h->d_nu,
                      &h->d_z1, &h->d_z2, &h->d_t, &h->d_b, &h->d_xio};
   for (double **v : vecs)
     if ((rc = dalloc(h, v, n))) return rc;
+  float **fvecs[] = {&h->f_r, &h->f_rhat, &h->f_p, &h->f_nu, &h->f_z1, &h->f_z2, &h->f_t};
+  for (float **v : fvecs)
+    if ((rc = dalloc(h, v, h->mixed ? n : 64))) return rc;
   if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_p2, (size_t)2 * MAXG)) ||
-      (rc = dalloc(h, &h->d_p4, (size_t)3 * MAXG)) || (rc = dalloc(h, &h->d_p5, (size_t)4 * MAXG)) ||
-      (rc = dalloc(h, &h->d_pm, (size_t)MAXG)) || (rc = dalloc(h, &h->d_pf, (size_t)4 * MAXG)))
+      (rc = dalloc(h, &h->d_p4, (size_t)3 * MAXG)) || (rc = dalloc(h, &h->d_p5, (size_t)NQ5 * MAXG)) ||
+      (rc = dalloc(h, &h->d_pm, (size_t)MAXG)) || (rc = dalloc(h, &h->d_pfin, (size_t)4 * MAXG)))
     return rc;
   HIPCHK(h, hipMemset(h->d_s, 0, 2 * sizeof(Scal)));
   h->G = (int)std::max<long long>(1, std::min<long long>(MAXG, (op.nb + WAVES - 1) / WAVES));
@@ -845,6 +1236,8 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
   if (h->nbp != 1 && h->nbp != 2 && h->nbp != 4) h->nbp = 2;
   const long long groups = (op.nb + h->nbp - 1) / h->nbp;
   h->Gp = (int)std::max<long long>(1, std::min<long long>(MAXG, (groups + WAVES - 1) / WAVES));
+  const long long tiles = (op.nb + TILE - 1) / TILE;
+  h->Gm = (int)std::max<long long>(1, std::min<long long>(MAXG, (tiles + WAVES - 1) / WAVES));
   h->have_mesh = true;
   return CUP2D_PS_OK;
 }
@@ -879,13 +1272,15 @@ int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind) 
   HIPCHK(h, hipSetDevice(h->device));
   const long n = (long)h->op.n;
   if (kind == 0) {
-    k_spmv<<<h->G, WG, 0, h->stream>>>(view(h), d_x, d_y, h->d_pm);
+    k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), d_x, d_y, h->d_pm);
   } else {
     const int G = (int)std::min<long>(4096, (n + WG - 1) / WG);
     k_spmv_csr<<<G, WG, 0, h->stream>>>(n, h->d_csr_rp, h->d_csr_col, h->d_csr_val, d_x, d_y);
-    if (h->op.mean_row >= 0) k_init_mean<<<h->G, WG, 0, h->stream>>>((int)h->op.nb, d_x, h->d_h2, h->d_pm);
+    if (h->op.mean_row >= 0)
+      k_hsum<double><<<h->G, WG, 0, h->stream>>>((int)h->op.nb, d_x, h->d_h2, h->d_pm);
   }
-  if (h->op.mean_row >= 0) k_meanrow_fix<<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, d_y);
+  if (h->op.mean_row >= 0)
+    k_meanrow_fix<double><<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, d_y);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
@@ -902,7 +1297,7 @@ int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, in
   const OpView A = view(h);
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int k = 0; k < reps; k++) {
-    if (kind == 0) k_spmv<<<h->G, WG, 0, h->stream>>>(A, d_x, d_y, h->d_pm);
+    if (kind == 0) k_spmv<double><<<h->G, WG, 0, h->stream>>>(A, d_x, d_y, h->d_pm);
     else k_spmv_csr<<<Gc, WG, 0, h->stream>>>(n, h->d_csr_rp, h->d_csr_col, h->d_csr_val, d_x, d_y);
   }
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -927,17 +1322,28 @@ int cup2d_ps_precond_device(cup2d_ps *h, const double *d_r, double *d_z) {
   return CUP2D_PS_OK;
 }
 
+int cup2d_ps_precond_bf16_device(cup2d_ps *h, const float *d_r, float *d_z) {
+  if (!h || !d_r || !d_z) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+  HIPCHK(h, hipSetDevice(h->device));
+  k_prec_mf<<<h->Gm, WG, 0, h->stream>>>((int)h->op.nb, h->d_pf, d_r, d_z);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
 int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]) {
   if (!h || !d_b || !d_x || !out) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  k_spmv<<<h->G, WG, 0, h->stream>>>(view(h), d_x, h->d_t, h->d_pm);
-  if (h->op.mean_row >= 0) k_meanrow_fix<<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, h->d_t);
-  k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pf);
+  k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), d_x, h->d_t, h->d_pm);
+  if (h->op.mean_row >= 0)
+    k_meanrow_fix<double><<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, h->d_t);
+  k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pfin);
   HIPCHK(h, hipGetLastError());
   std::vector<double> part(4 * (size_t)h->G);
-  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_pf, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_pfin, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   double r2 = 0, rinf = 0, b2 = 0, binf = 0;
   for (int i = 0; i < h->G; i++) {

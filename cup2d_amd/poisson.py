@@ -77,30 +77,32 @@ def assemble_csr(mesh: Mesh, mean_constraint=0):
 
 
 class DeviceArray:
-    """FP64 device buffer owned through a solver handle."""
+    """Device buffer (FP64 by default) owned through a solver handle."""
 
-    def __init__(self, solver: "Solver", n: int):
+    def __init__(self, solver: "Solver", n: int, dtype=np.float64):
         self.solver = solver
         self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
         p = ctypes.c_void_p()
-        _check(solver.h, _lib.load().cup2d_ps_dmalloc(solver.h, self.n * 8, ctypes.byref(p)))
+        _check(solver.h, _lib.load().cup2d_ps_dmalloc(solver.h, self.nbytes, ctypes.byref(p)))
         self.ptr = p.value
 
     @classmethod
-    def from_host(cls, solver, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        d = cls(solver, a.size)
+    def from_host(cls, solver, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(solver, a.size, dtype)
         d.upload(a)
         return d
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.size == self.n
-        _check(self.solver.h, _lib.load().cup2d_ps_h2d(self.solver.h, self.ptr, a.ctypes.data, self.n * 8))
+        _check(self.solver.h, _lib.load().cup2d_ps_h2d(self.solver.h, self.ptr, a.ctypes.data, self.nbytes))
 
     def numpy(self):
-        out = np.empty(self.n, np.float64)
-        _check(self.solver.h, _lib.load().cup2d_ps_d2h(self.solver.h, out.ctypes.data, self.ptr, self.n * 8))
+        out = np.empty(self.n, self.dtype)
+        _check(self.solver.h, _lib.load().cup2d_ps_d2h(self.solver.h, out.ctypes.data, self.ptr, self.nbytes))
         return out
 
     def free(self):
@@ -195,6 +197,13 @@ class Solver:
         dr = DeviceArray.from_host(self, r)
         dz = DeviceArray(self, dr.n)
         _check(self.h, _lib.load().cup2d_ps_precond_device(self.h, dr.ptr, dz.ptr))
+        return dz.numpy()
+
+    def precond_bf16(self, r):
+        """BF16-MFMA block preconditioner of the mixed solver on FP32 data."""
+        dr = DeviceArray.from_host(self, r, np.float32)
+        dz = DeviceArray(self, dr.n, np.float32)
+        _check(self.h, _lib.load().cup2d_ps_precond_bf16_device(self.h, dr.ptr, dz.ptr))
         return dz.numpy()
 
     def residual_device(self, d_b: DeviceArray, d_x: DeviceArray):

@@ -126,6 +126,11 @@ int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, in
  * BiCGSTAB.cu:520, :545). */
 int cup2d_ps_precond_device(cup2d_ps *h, const double *d_r, double *d_z);
 
+/* The same with the BF16 MFMA kernel of the mixed-precision solver: P_inv
+ * rounded to bf16, the input split into three bf16 terms, FP32
+ * accumulation; FP32 vectors (n floats each). */
+int cup2d_ps_precond_bf16_device(cup2d_ps *h, const float *d_r, float *d_z);
+
 /* True residual of x: out[0] = |b - A x|_2, out[1] = |b - A x|_inf,
  * out[2] = |b|_2, out[3] = |b|_inf, all in FP64 on the device (b's mean row
  * taken as 0 with mean_constraint). */

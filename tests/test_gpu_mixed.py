@@ -1,0 +1,107 @@
+"""GPU tests of the mixed-precision path: the BF16 MFMA block preconditioner
+and FP64 iterative refinement around the FP32 inner BiCGSTAB.
+
+Tolerances: the MFMA kernel applies P_inv rounded to bf16 (a fixed linear
+preconditioner) to the input split into bf16 hi + lo parts with FP32
+accumulation, so against the FP64 product with the same bf16-rounded P it
+agrees to ~1e-5 relative; the refined solution must meet the same FP64 true
+residual as the FP64 solver (1e-10 relative).
+"""
+
+import numpy as np
+import pytest
+
+from cup2d_amd import mesh as M
+from cup2d_amd import poisson as PS
+from cup2d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def mixed_solvers():
+    cache = {}
+
+    def get(name, mc=0, **cfg):
+        key = (name, mc, tuple(sorted(cfg.items())))
+        if key not in cache:
+            m = {"config1": W.config1_mesh, "config2": W.config2_mesh,
+                 "amr6_small": lambda: W.config3_mesh(target=3000),
+                 "tiny": lambda: M.uniform(4, 2, 2.0)}[name]()
+            s = PS.Solver(precision=PS.MIXED, mean_constraint=mc, max_iter=4000, **cfg)
+            s.set_mesh(m)
+            cache[key] = (m, s)
+        return cache[key]
+    return get
+
+
+@pytest.mark.parametrize("name", ["tiny", "config2"])
+def test_bf16_mfma_precond_layout(oracle, mixed_solvers, name):
+    m, s = mixed_solvers(name)
+    P = oracle.pinv()
+    Pb = bf16_round(P.astype(np.float32)).astype(np.float64)
+    r = np.random.default_rng(5).normal(size=m.n).astype(np.float32)
+    z = s.precond_bf16(r)
+    ref_b = (Pb @ r.astype(np.float64).reshape(-1, 64).T).T.reshape(-1)
+    ref = (P @ r.astype(np.float64).reshape(-1, 64).T).T.reshape(-1)
+    scale = np.abs(ref_b).max()
+    assert np.abs(z - ref_b).max() <= 2e-5 * scale           # exact up to input split + FP32
+    assert np.abs(z - ref).max() <= 2e-2 * scale             # and P itself is P_inv to bf16
+
+
+@pytest.mark.parametrize("name", ["config1", "config2", "amr6_small"])
+@pytest.mark.parametrize("norm", [PS.NORM_L2, PS.NORM_INF])
+def test_mixed_solve_reaches_fp64_residual(oracle, mixed_solvers, name, norm):
+    m, s = mixed_solvers(name, norm=norm)
+    A = oracle.assemble(m)
+    h2 = oracle.h2(m)
+    b = W.vortex_rhs(m, 3)
+    x, st = s.solve(b, rel_tol=1e-10, max_restarts=100)
+    print(name, norm, st)
+    assert st["converged"] and st["refinements"] >= 1, st
+    tr = oracle.true_residual(A, b, x)
+    assert (tr["rel2"] if norm == PS.NORM_L2 else tr["relinf"]) <= 1.05e-10, tr
+    xr, _ = oracle.bicgstab(A, b, rel_tol=1e-13, norm="l2", max_iter=4000)
+    xr = oracle.mean_remove(h2, xr)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-6
+
+
+def test_mixed_solve_matches_oracle_1e8(oracle, mixed_solvers):
+    m, s = mixed_solvers("config2", norm=PS.NORM_L2)
+    A = oracle.assemble(m)
+    h2 = oracle.h2(m)
+    b = W.vortex_rhs(m, 4)
+    x, st = s.solve(b, rel_tol=1e-12, max_restarts=100)
+    assert st["converged"], st
+    xr, _ = oracle.bicgstab(A, b, rel_tol=1e-13, norm="l2", max_iter=4000)
+    xr = oracle.mean_remove(h2, xr)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-8
+
+
+def test_mixed_mean_constraint(oracle, mixed_solvers):
+    m, s = mixed_solvers("config2", mc=1, norm=PS.NORM_L2)
+    A = oracle.assemble(m, 1)
+    h2 = oracle.h2(m)
+    b = W.vortex_rhs(m, 6)
+    bb = b.copy()
+    bb[A.mean_row] = 0.0
+    x, st = s.solve(b, rel_tol=1e-12, max_restarts=100)
+    assert st["converged"], st
+    xr, _ = oracle.bicgstab(A, bb, mean_mode=1, h2v=h2, rel_tol=1e-13, norm="l2", max_iter=4000)
+    xr = oracle.mean_remove(h2, xr)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-8
+
+
+def test_mixed_deterministic(mixed_solvers):
+    m, s = mixed_solvers("config2")
+    b = W.vortex_rhs(m, 8)
+    x1, st1 = s.solve(b, rel_tol=1e-10)
+    x2, st2 = s.solve(b, rel_tol=1e-10)
+    assert np.array_equal(x1, x2) and st1 == {**st2, "ms": st1["ms"]}
