@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <unordered_map>
 
 namespace cup2d {
@@ -303,6 +304,68 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
   }
   op.mean_row = (mean_constraint && op.corner_block >= 0) ? op.corner_block * BLEN : -1;
   return true;
+}
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+static uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return uint16_t(u >> 16);
+}
+
+
+// -K^-1 for the 8x8 block (Cholesky-Crout of K then L^-1 by Gauss-Jordan and
+// P = -(L^-T L^-1), the construction of ExpAMRSolver.cpp:37-95).
+std::vector<double> make_pinv() {
+  const int N = 64;
+  auto K = [](int a, int b) {
+    const int ya = a / 8, xa = a % 8, yb = b / 8, xb = b % 8;
+    if (a == b) return 4.0;
+    return (std::abs(xa - xb) + std::abs(ya - yb) == 1) ? -1.0 : 0.0;
+  };
+  std::vector<double> L(N * N, 0.0), Li(N * N, 0.0), P(N * N);
+  for (int i = 0; i < N; i++) {
+    double s1 = 0;
+    for (int k = 0; k < i; k++) s1 += L[i * N + k] * L[i * N + k];
+    L[i * N + i] = std::sqrt(K(i, i) - s1);
+    for (int j = i + 1; j < N; j++) {
+      double s2 = 0;
+      for (int k = 0; k < i; k++) s2 += L[i * N + k] * L[j * N + k];
+      L[j * N + i] = (K(j, i) - s2) / L[i * N + i];
+    }
+  }
+  for (int i = 0; i < N; i++) Li[i * N + i] = 1.0;
+  for (int br = 0; br < N; br++) {
+    const double f = 1.0 / L[br * N + br];
+    for (int c = 0; c <= br; c++) Li[br * N + c] *= f;
+    for (int wr = br + 1; wr < N; wr++) {
+      const double wf = L[wr * N + br];
+      for (int c = 0; c <= br; c++) Li[wr * N + c] -= wf * Li[br * N + c];
+    }
+  }
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++) {
+      double a = 0;
+      for (int k = std::max(i, j); k < N; k++) a += Li[k * N + i] * Li[k * N + j];
+      P[i * N + j] = -a;
+    }
+  return P;
+}
+
+// P in the A-operand layout of v_mfma_f32_16x16x32_bf16 (see kernels.h):
+// fragment (mt, ks), lane l, element j = P[16 mt + (l & 15)][32 ks + 8 (l >> 4) + j].
+std::vector<uint16_t> make_pfrag(const std::vector<double> &P) {
+  std::vector<uint16_t> f(8 * 64 * 8);
+  for (int mt = 0; mt < 4; mt++)
+    for (int ks = 0; ks < 2; ks++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 8; j++) {
+          const int row = 16 * mt + (l & 15), col = 32 * ks + 8 * (l >> 4) + j;
+          f[((mt * 2 + ks) * 64 + l) * 8 + j] = bf16_bits((float)P[row * 64 + col]);
+        }
+  return f;
 }
 
 }  // namespace cup2d

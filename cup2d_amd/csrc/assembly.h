@@ -61,4 +61,12 @@ struct Operator {
 // finer).
 bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err);
 
+// P_inv = -K^-1 of the 8x8-block 5-point matrix K, row-major 64x64.
+std::vector<double> make_pinv();
+
+// P in the A-operand layout of v_mfma_f32_16x16x32_bf16 rounded to bf16
+// (bit patterns): fragment (mt, ks), lane l, element j =
+// P[16 mt + (l & 15)][32 ks + 8 (l >> 4) + j].
+std::vector<uint16_t> make_pfrag(const std::vector<double> &P);
+
 }  // namespace cup2d

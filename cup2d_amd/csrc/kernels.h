@@ -240,4 +240,118 @@ __device__ __forceinline__ void prec_mfma(const PFrag &P, const float (&in)[16],
 __device__ __forceinline__ int in_cell(int q, int k) { return 32 * (k >> 3) + 8 * q + (k & 7); }
 __device__ __forceinline__ int out_cell(int q, int k) { return 16 * (k >> 2) + 4 * q + (k & 3); }
 
+constexpr int NQ5 = 6;  // totals of the x/r update: |r|_inf, |r|^2, (rhat,r), |rhat|^2, sum r, sum rhat
+
+// State at the top of iteration i from the state of iteration i-1 and the
+// totals o of iteration i-1's update (BiCGSTAB.cu:471-509 and :572-600).
+// prev.iter < 0 marks the state after r0 = b - A x0 (BiCGSTAB.cu:427-465).
+// With `project` (FP32-stored residuals) the component of r along the left
+// null vector 1 of the Neumann operator, mu = sum r / n, is removed: it is
+// rounding noise no iterate can cancel, and left in place it stalls the
+// iteration at ~1e-7 and then makes it diverge.  The totals are corrected
+// for it here and the kernels subtract mu from r.  Deterministic:
+// recomputing it from the same inputs gives the same state.
+__device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
+  Scal s = prev;
+  double rr = o[1], rhr = o[2], hh = o[3], amax = o[0];
+  s.mu = 0.0;
+  if (prev.project) {
+    const double mu = o[4] / prev.n;
+    s.mu = mu;
+    rr = fmax(rr - prev.n * mu * mu, 0.0);
+    rhr -= mu * o[5];
+    amax += fabs(mu);  // bound of max |r_i - mu|
+  }
+  const double err = prev.norm ? sqrt(rr) : amax;
+  if (prev.iter < 0) {
+    s.err = s.err0 = s.err_opt = s.err_ref = err;
+    s.iter = s.iter_ref = 0;
+    s.alpha = s.omega = s.rho_prev = 1.0;
+    s.rho = rhr;
+    s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+    s.restarts = s.converged = s.improved = s.restart = 0;
+    s.done = (err == 0.0 || s.max_iter <= 0) ? 1 : 0;
+    return s;
+  }
+  if (prev.done) {
+    s.restart = 0;
+    s.mu = 0.0;
+    return s;
+  }
+  s.err = err;
+  s.iter = prev.iter + 1;
+  s.improved = 0;
+  if (err < prev.err_opt) {
+    s.improved = 1;
+    s.err_opt = err;
+    if (err <= s.tol_abs || err / s.err0 <= s.tol_rel) {
+      s.converged = 1;
+      s.done = 1;
+    }
+  }
+  if (!s.done && s.iter >= s.max_iter) s.done = 1;
+  // stagnation (mixed precision: the FP32 recurrence reached its floor; the
+  // caller restarts from the FP64 residual)
+  if (err < 0.5 * s.err_ref) {
+    s.err_ref = err;
+    s.iter_ref = s.iter;
+  }
+  if (!s.done && s.stall > 0 && s.iter - s.iter_ref > s.stall) s.done = 1;
+  s.rho_prev = prev.rho;
+  s.rho = rhr;
+  s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+  // breakdown test of BiCGSTAB.cu:483; a beta that would overflow FP32
+  // storage (omega ~ 0) or is not finite is handled the same way
+  const bool breakdown = rhr * rhr < s.bd_tol * rr * hh || !(fabs(s.beta) < 1e30);
+  s.restart = 0;
+  if (!s.done && breakdown && s.max_restarts > 0) {
+    s.restarts += 1;
+    if (s.restarts >= s.max_restarts) {
+      s.done = 1;
+    } else {
+      s.restart = 1;
+      s.rho = rr;  // rhat = r
+      s.rho_prev = s.alpha = s.omega = 1.0;
+      s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
+    }
+  }
+  s.red[5] = rr;
+  s.red[6] = rhr;
+  s.red[7] = hh;
+  return s;
+}
+
+
+__device__ __forceinline__ void load_P(double *Ps, const double *__restrict__ P) {
+  for (int i = threadIdx.x; i < 64 * 64; i += WG) Ps[i] = P[i];
+  __syncthreads();
+}
+
+
+// 16 values of the lane's input cells (32 ks + 8 q + j) of one block
+__device__ __forceinline__ void ld16(const float *__restrict__ a, long base, int q, float (&v)[16]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    const float4 *p4 = reinterpret_cast<const float4 *>(a + base + 32 * ks + 8 * q);
+    const float4 u0 = p4[0], u1 = p4[1];
+    v[ks * 8 + 0] = u0.x; v[ks * 8 + 1] = u0.y; v[ks * 8 + 2] = u0.z; v[ks * 8 + 3] = u0.w;
+    v[ks * 8 + 4] = u1.x; v[ks * 8 + 5] = u1.y; v[ks * 8 + 6] = u1.z; v[ks * 8 + 7] = u1.w;
+  }
+}
+__device__ __forceinline__ void st16_in(float *__restrict__ a, long base, int q, const float (&v)[16]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    float4 *p4 = reinterpret_cast<float4 *>(a + base + 32 * ks + 8 * q);
+    p4[0] = make_float4(v[ks * 8 + 0], v[ks * 8 + 1], v[ks * 8 + 2], v[ks * 8 + 3]);
+    p4[1] = make_float4(v[ks * 8 + 4], v[ks * 8 + 5], v[ks * 8 + 6], v[ks * 8 + 7]);
+  }
+}
+// 16 values of the lane's output cells (16 mt + 4 q + v)
+__device__ __forceinline__ void st16_out(float *__restrict__ a, long base, int q, const float (&v)[16]) {
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+    *reinterpret_cast<float4 *>(a + base + 16 * mt + 4 * q) =
+        make_float4(v[mt * 4 + 0], v[mt * 4 + 1], v[mt * 4 + 2], v[mt * 4 + 3]);
+}
+
 }  // namespace cup2d

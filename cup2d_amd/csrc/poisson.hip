@@ -46,87 +46,6 @@ constexpr int MAXG = 512;  // workgroups of the vector kernels (grid-stride beyo
 // scalar recurrence
 // ----------------------------------------------------------------------------
 
-constexpr int NQ5 = 6;  // totals of the x/r update: |r|_inf, |r|^2, (rhat,r), |rhat|^2, sum r, sum rhat
-
-// State at the top of iteration i from the state of iteration i-1 and the
-// totals o of iteration i-1's update (BiCGSTAB.cu:471-509 and :572-600).
-// prev.iter < 0 marks the state after r0 = b - A x0 (BiCGSTAB.cu:427-465).
-// With `project` (FP32-stored residuals) the component of r along the left
-// null vector 1 of the Neumann operator, mu = sum r / n, is removed: it is
-// rounding noise no iterate can cancel, and left in place it stalls the
-// iteration at ~1e-7 and then makes it diverge.  The totals are corrected
-// for it here and the kernels subtract mu from r.  Deterministic:
-// recomputing it from the same inputs gives the same state.
-__device__ Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
-  Scal s = prev;
-  double rr = o[1], rhr = o[2], hh = o[3], amax = o[0];
-  s.mu = 0.0;
-  if (prev.project) {
-    const double mu = o[4] / prev.n;
-    s.mu = mu;
-    rr = fmax(rr - prev.n * mu * mu, 0.0);
-    rhr -= mu * o[5];
-    amax += fabs(mu);  // bound of max |r_i - mu|
-  }
-  const double err = prev.norm ? sqrt(rr) : amax;
-  if (prev.iter < 0) {
-    s.err = s.err0 = s.err_opt = s.err_ref = err;
-    s.iter = s.iter_ref = 0;
-    s.alpha = s.omega = s.rho_prev = 1.0;
-    s.rho = rhr;
-    s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
-    s.restarts = s.converged = s.improved = s.restart = 0;
-    s.done = (err == 0.0 || s.max_iter <= 0) ? 1 : 0;
-    return s;
-  }
-  if (prev.done) {
-    s.restart = 0;
-    s.mu = 0.0;
-    return s;
-  }
-  s.err = err;
-  s.iter = prev.iter + 1;
-  s.improved = 0;
-  if (err < prev.err_opt) {
-    s.improved = 1;
-    s.err_opt = err;
-    if (err <= s.tol_abs || err / s.err0 <= s.tol_rel) {
-      s.converged = 1;
-      s.done = 1;
-    }
-  }
-  if (!s.done && s.iter >= s.max_iter) s.done = 1;
-  // stagnation (mixed precision: the FP32 recurrence reached its floor; the
-  // caller restarts from the FP64 residual)
-  if (err < 0.5 * s.err_ref) {
-    s.err_ref = err;
-    s.iter_ref = s.iter;
-  }
-  if (!s.done && s.stall > 0 && s.iter - s.iter_ref > s.stall) s.done = 1;
-  s.rho_prev = prev.rho;
-  s.rho = rhr;
-  s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
-  // breakdown test of BiCGSTAB.cu:483; a beta that would overflow FP32
-  // storage (omega ~ 0) or is not finite is handled the same way
-  const bool breakdown = rhr * rhr < s.bd_tol * rr * hh || !(fabs(s.beta) < 1e30);
-  s.restart = 0;
-  if (!s.done && breakdown && s.max_restarts > 0) {
-    s.restarts += 1;
-    if (s.restarts >= s.max_restarts) {
-      s.done = 1;
-    } else {
-      s.restart = 1;
-      s.rho = rr;  // rhat = r
-      s.rho_prev = s.alpha = s.omega = 1.0;
-      s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
-    }
-  }
-  s.red[5] = rr;
-  s.red[6] = rhr;
-  s.red[7] = hh;
-  return s;
-}
-
 // Evaluates the state of this iteration in every workgroup (identically);
 // workgroup 0 stores it to slot `par`.
 __device__ __forceinline__ void top_of_iteration(Scal *st, int par, const double *p5, int G5,
@@ -141,11 +60,6 @@ __device__ __forceinline__ void top_of_iteration(Scal *st, int par, const double
 // ----------------------------------------------------------------------------
 // initialisation
 // ----------------------------------------------------------------------------
-
-__device__ __forceinline__ void load_P(double *Ps, const double *__restrict__ P) {
-  for (int i = threadIdx.x; i < 64 * 64; i += WG) Ps[i] = P[i];
-  __syncthreads();
-}
 
 // Partial sums of h^2 x (the mean row of A x when the mean constraint is on).
 template <class T>
@@ -312,32 +226,6 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
 // ----------------------------------------------------------------------------
 // iteration kernels, FP32 inner solve (BF16 MFMA preconditioner)
 // ----------------------------------------------------------------------------
-
-// 16 values of the lane's input cells (32 ks + 8 q + j) of one block
-__device__ __forceinline__ void ld16(const float *__restrict__ a, long base, int q, float (&v)[16]) {
-#pragma unroll
-  for (int ks = 0; ks < 2; ks++) {
-    const float4 *p4 = reinterpret_cast<const float4 *>(a + base + 32 * ks + 8 * q);
-    const float4 u0 = p4[0], u1 = p4[1];
-    v[ks * 8 + 0] = u0.x; v[ks * 8 + 1] = u0.y; v[ks * 8 + 2] = u0.z; v[ks * 8 + 3] = u0.w;
-    v[ks * 8 + 4] = u1.x; v[ks * 8 + 5] = u1.y; v[ks * 8 + 6] = u1.z; v[ks * 8 + 7] = u1.w;
-  }
-}
-__device__ __forceinline__ void st16_in(float *__restrict__ a, long base, int q, const float (&v)[16]) {
-#pragma unroll
-  for (int ks = 0; ks < 2; ks++) {
-    float4 *p4 = reinterpret_cast<float4 *>(a + base + 32 * ks + 8 * q);
-    p4[0] = make_float4(v[ks * 8 + 0], v[ks * 8 + 1], v[ks * 8 + 2], v[ks * 8 + 3]);
-    p4[1] = make_float4(v[ks * 8 + 4], v[ks * 8 + 5], v[ks * 8 + 6], v[ks * 8 + 7]);
-  }
-}
-// 16 values of the lane's output cells (16 mt + 4 q + v)
-__device__ __forceinline__ void st16_out(float *__restrict__ a, long base, int q, const float (&v)[16]) {
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++)
-    *reinterpret_cast<float4 *>(a + base + 16 * mt + 4 * q) =
-        make_float4(v[mt * 4 + 0], v[mt * 4 + 1], v[mt * 4 + 2], v[mt * 4 + 3]);
-}
 
 // Top of iteration (FP32 storage): r -= mu (null-space projection),
 // p = r + beta (p - omega nu) or p = rhat = r after a restart, x_opt = x
@@ -724,15 +612,6 @@ __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restric
 
 std::string g_create_err;
 
-// float -> bf16 bits, round to nearest even (finite inputs)
-uint16_t bf16_bits(float f) {
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  const uint32_t lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;
-  return uint16_t(u >> 16);
-}
-
 }  // namespace
 
 struct cup2d_ps {
@@ -829,58 +708,6 @@ OpView view(const cup2d_ps *h) {
   A.val = h->d_val;
   A.h2 = h->d_h2;
   return A;
-}
-
-// -K^-1 for the 8x8 block (Cholesky-Crout of K then L^-1 by Gauss-Jordan and
-// P = -(L^-T L^-1), the construction of ExpAMRSolver.cpp:37-95).
-std::vector<double> make_pinv() {
-  const int N = 64;
-  auto K = [](int a, int b) {
-    const int ya = a / 8, xa = a % 8, yb = b / 8, xb = b % 8;
-    if (a == b) return 4.0;
-    return (std::abs(xa - xb) + std::abs(ya - yb) == 1) ? -1.0 : 0.0;
-  };
-  std::vector<double> L(N * N, 0.0), Li(N * N, 0.0), P(N * N);
-  for (int i = 0; i < N; i++) {
-    double s1 = 0;
-    for (int k = 0; k < i; k++) s1 += L[i * N + k] * L[i * N + k];
-    L[i * N + i] = std::sqrt(K(i, i) - s1);
-    for (int j = i + 1; j < N; j++) {
-      double s2 = 0;
-      for (int k = 0; k < i; k++) s2 += L[i * N + k] * L[j * N + k];
-      L[j * N + i] = (K(j, i) - s2) / L[i * N + i];
-    }
-  }
-  for (int i = 0; i < N; i++) Li[i * N + i] = 1.0;
-  for (int br = 0; br < N; br++) {
-    const double f = 1.0 / L[br * N + br];
-    for (int c = 0; c <= br; c++) Li[br * N + c] *= f;
-    for (int wr = br + 1; wr < N; wr++) {
-      const double wf = L[wr * N + br];
-      for (int c = 0; c <= br; c++) Li[wr * N + c] -= wf * Li[br * N + c];
-    }
-  }
-  for (int i = 0; i < N; i++)
-    for (int j = 0; j < N; j++) {
-      double a = 0;
-      for (int k = std::max(i, j); k < N; k++) a += Li[k * N + i] * Li[k * N + j];
-      P[i * N + j] = -a;
-    }
-  return P;
-}
-
-// P in the A-operand layout of v_mfma_f32_16x16x32_bf16 (see kernels.h):
-// fragment (mt, ks), lane l, element j = P[16 mt + (l & 15)][32 ks + 8 (l >> 4) + j].
-std::vector<uint16_t> make_pfrag(const std::vector<double> &P) {
-  std::vector<uint16_t> f(8 * 64 * 8);
-  for (int mt = 0; mt < 4; mt++)
-    for (int ks = 0; ks < 2; ks++)
-      for (int l = 0; l < 64; l++)
-        for (int j = 0; j < 8; j++) {
-          const int row = 16 * mt + (l & 15), col = 32 * ks + 8 * (l >> 4) + j;
-          f[((mt * 2 + ks) * 64 + l) * 8 + j] = bf16_bits((float)P[row * 64 + col]);
-        }
-  return f;
 }
 
 template <int NB>
