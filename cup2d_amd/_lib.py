@@ -60,6 +60,15 @@ SIGNATURES = {
     "cup2d_ps_h2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "cup2d_ps_d2h": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "cup2d_ps_last_error": (ctypes.c_char_p, [_P]),
+    "cup2d_psb_create": (_I, [_P, _P]),
+    "cup2d_psb_destroy": (_I, [_P]),
+    "cup2d_psb_set_meshes": (_I, [_P, _I, _P]),
+    "cup2d_psb_offsets": (_I, [_P, _P]),
+    "cup2d_psb_solve": (_I, [_P, _P, _P, _D, _D, _I, _P]),
+    "cup2d_psb_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
+    "cup2d_psb_autotune": (_I, [_P, _P, _P, _P]),
+    "cup2d_psb_item_blocks": (_I, [_P]),
+    "cup2d_psb_last_error": (ctypes.c_char_p, [_P]),
 }
 
 _lib = None

@@ -48,7 +48,7 @@ def build_poisson():
         obj = os.path.join(bdir, src + ".o")
         _run(["g++", *CXX_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
-    for src in ("poisson.hip",):
+    for src in ("poisson.hip", "batch.hip"):
         obj = os.path.join(bdir, src + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)

@@ -318,6 +318,27 @@ static uint16_t bf16_bits(float f) {
 
 // -K^-1 for the 8x8 block (Cholesky-Crout of K then L^-1 by Gauss-Jordan and
 // P = -(L^-T L^-1), the construction of ExpAMRSolver.cpp:37-95).
+int ell_width(const Operator &op) {
+  int w = 0;
+  for (size_t e = 0; e + 1 < op.exc_rptr.size(); e++) w = std::max(w, op.exc_rptr[e + 1] - op.exc_rptr[e]);
+  return w;
+}
+
+void ell_append(const Operator &op, int W, long long row_off, std::vector<int> &erow,
+                std::vector<int> &ecol_rm, std::vector<double> &eval_rm) {
+  for (long long b = 0; b < op.nb; b++)
+    for (int e = op.exc_bptr[b]; e < op.exc_bptr[b + 1]; e++) {
+      const int row = (int)(b * BLEN + op.exc_cell[e] + row_off);
+      erow.push_back(row);
+      const int k0 = op.exc_rptr[e], k1 = op.exc_rptr[e + 1];
+      for (int w = 0; w < W; w++) {
+        const bool ok = k0 + w < k1;
+        ecol_rm.push_back(ok ? (int)(op.exc_col[k0 + w] + row_off) : row);
+        eval_rm.push_back(ok ? op.exc_val[k0 + w] : 0.0);
+      }
+    }
+}
+
 std::vector<double> make_pinv() {
   const int N = 64;
   auto K = [](int a, int b) {

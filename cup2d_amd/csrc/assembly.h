@@ -61,6 +61,23 @@ struct Operator {
 // finer).
 bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err);
 
+// Explicit rows in ELL form.  ell_width: most entries of an explicit row
+// (0 when there are none).  ell_append: appends op's explicit rows with
+// global row / column offset row_off, W entries per row (row-major), padded
+// with (own row, 0.0).  ell_transpose: row-major [n][W] to the kernels'
+// column-major [W][n].
+int ell_width(const Operator &op);
+void ell_append(const Operator &op, int W, long long row_off, std::vector<int> &erow,
+                std::vector<int> &ecol_rm, std::vector<double> &eval_rm);
+template <class T>
+std::vector<T> ell_transpose(const std::vector<T> &rm, int W) {
+  const size_t n = W > 0 ? rm.size() / W : 0;
+  std::vector<T> cm(rm.size());
+  for (size_t e = 0; e < n; e++)
+    for (int w = 0; w < W; w++) cm[(size_t)w * n + e] = rm[e * W + w];
+  return cm;
+}
+
 // P_inv = -K^-1 of the 8x8-block 5-point matrix K, row-major 64x64.
 std::vector<double> make_pinv();
 

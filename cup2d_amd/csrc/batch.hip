@@ -1,0 +1,1234 @@
+// batch.hip -- many independent AMR Poisson solves advanced together.
+//
+// Each instance is the solve of ExpAMRSolver::solve (ExpAMRSolver.cpp:354-416)
+// on its own mesh: BiCGSTAB (BiCGSTAB.cu:412-620) with the 64x64 block
+// preconditioner and h^2-weighted mean removal.  The instances' blocks are
+// concatenated into one block-diagonal operator and every iteration advances
+// all unfinished instances at once:
+//
+//  * work units: "items" (a run of up to `ib` blocks of one instance) for the
+//    SpMV and vector kernels, a workgroup per item; "tiles" (16 blocks of one
+//    instance) for the preconditioner kernels, a wavefront per tile.  The
+//    kernels walk the unit lists grid-stride with fixed grids (so a chunk of
+//    iterations is captured once as a hipGraph) and skip the units of
+//    instances whose device-side state says done: when few instances remain,
+//    their units are still spread over the whole GPU.
+//  * per-instance scalars: after each reduction point one kernel with a
+//    workgroup per instance re-reduces that instance's item partials in a
+//    fixed order and advances its scalar recurrence (deterministic, no
+//    atomics or fences).
+//  * the item size `ib` is picked by a timing autotuner and cached per
+//    batch shape.
+//
+// Precisions: FP64, or FP32-stored Krylov vectors with FP64 arithmetic and
+// the BF16 MFMA preconditioner inside per-instance FP64 iterative refinement
+// (the scheme of the single-solve mixed path).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cup2d_poisson.h"
+#include "assembly.h"
+#include "kernels.h"
+
+using namespace cup2d;
+
+namespace {
+
+constexpr int NQB = 7;        // update totals (NQ5) + sum h^2 x (initial mean row)
+constexpr int BGRID = 2048;   // workgroups of the item / tile kernels (grid-stride beyond)
+constexpr int MAX_INST = 4096;
+constexpr int MAX_PASSES = 16;  // FP64 refinement passes per instance (mixed)
+
+struct BView {
+  OpView A;                          // concatenated operator (A.mean_row unused)
+  const int *item_inst, *item_b0, *item_nb;  // item_nb < 0: explicit item
+  const int *item_begin;             // [K+1] first item of each instance
+  const int *tile_inst, *tile_b0, *tile_nb;
+  const long long *mean_row;         // [K] global row of the mean constraint or -1
+  int K, n_items, n_tiles;
+};
+
+// Workgroup reduction of NQ quantities; thread 0 stores them at
+// part[q*NI + idx].
+template <int NQ, unsigned MAXQ>
+__device__ void store_item_partials(double (&v)[NQ], double *part, int NI, int idx) {
+  __shared__ double red[NQ][WAVES];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) v[q] = ((MAXQ >> q) & 1) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[q][w] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      double a = red[q][0];
+      for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? nmax(a, red[q][i]) : a + red[q][i];
+      part[(long)q * NI + idx] = a;
+    }
+  }
+  __syncthreads();  // red[] is reused by the workgroup's next item
+}
+
+// Totals of partials i0..i1-1 of each quantity (fixed order; every thread
+// gets them).
+template <int NQ, unsigned MAXQ>
+__device__ void range_totals(const double *__restrict__ part, int NI, int i0, int i1,
+                             double (&out)[NQ]) {
+  __shared__ double fin[NQ][WAVES];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double a = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += WG) {
+      const double p = part[(long)q * NI + i];
+      a = ((MAXQ >> q) & 1) ? nmax(a, p) : a + p;
+    }
+    a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
+    if (lane == 0) fin[q][w] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double a = fin[q][0];
+    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? nmax(a, fin[q][i]) : a + fin[q][i];
+    out[q] = a;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// item kernels.  A block item (item_nb > 0) is taken by a workgroup, a
+// wavefront per UB blocks at a time (lane = cell); an explicit item
+// (item_nb < 0: -item_nb explicit rows from item_b0) by a workgroup, a
+// thread per row.  Every item of a running instance stores its partials,
+// zeros where it has nothing to add.
+// ----------------------------------------------------------------------------
+
+constexpr int UB = 2;  // blocks per wavefront in flight
+
+// Runs f(R, u, bl) for the item's blocks, UB per wave at a time, after
+// issuing every load of those blocks (stencil, h^2 and aux[i]).
+template <class TV, class TA, class F>
+__device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict__ x,
+                                            const TA *__restrict__ aux, int b0, int nbk, F &&f) {
+  const int c = lane_id();
+  for (int j = wave_id() * UB; j < nbk; j += WAVES * UB) {
+    int bl[UB];
+    bool ok[UB];
+#pragma unroll
+    for (int u = 0; u < UB; u++) {
+      ok[u] = j + u < nbk;
+      bl[u] = b0 + (ok[u] ? j + u : j);
+    }
+    BlockRows<UB, TV> R;
+    R.load(V.A, x, aux, bl, c);
+#pragma unroll
+    for (int u = 0; u < UB; u++)
+      if (ok[u]) f(R, u, bl[u]);
+  }
+}
+
+// r = b - A x (FP64), stored as T to r and rhat; p = nu = 0; x_opt = x;
+// partials of the update totals and of sum h^2 x.  The mean row's residual
+// is finished by kb_eval.
+template <class T>
+__global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict__ b,
+                                              const double *__restrict__ x,
+                                              double *__restrict__ xopt, T *__restrict__ r,
+                                              T *__restrict__ rhat, T *__restrict__ p,
+                                              T *__restrict__ nu, const Scal *st, double *p5) {
+  const int c = lane_id();
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int k = V.item_inst[item];
+    if (st[k].done) continue;
+    const int b0 = V.item_b0[item], cnt = V.item_nb[item];
+    const long long mr = V.mean_row[k];
+    double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
+    auto store_r = [&](long i, double y, double bi) {
+      const double ri = (i == mr) ? 0.0 : bi - y;
+      const T rt = (T)ri;
+      r[i] = rt;
+      rhat[i] = rt;
+      const double rd = (double)rt;
+      v[0] = nmax(v[0], fabs(rd));
+      v[1] += rd * rd;
+      v[4] += rd;
+    };
+    if (cnt > 0) {
+      item_blocks<double>(V, x, b, b0, cnt, [&](const BlockRows<UB, double> &R, int u, int bb) {
+        bool plain;
+        const double y = R.y(u, c, plain);
+        const long i = R.i[u];
+        p[i] = T(0);
+        nu[i] = T(0);
+        xopt[i] = R.xc[u];
+        v[6] += R.hh[u] * R.xc[u];
+        if (plain) store_r(i, y, R.a[u]);
+      });
+    } else if ((int)threadIdx.x < -cnt) {
+      const int e = b0 + threadIdx.x, row = V.A.erow[e];
+      store_r(row, explicit_row<double>(V.A, x, e), b[row]);
+    }
+    v[2] = v[1];
+    v[3] = v[1];
+    v[5] = v[4];
+    store_item_partials<NQB, 1u>(v, p5, V.n_items, item);
+  }
+}
+
+// nu = A z1; partials of (rhat, nu) and sum h^2 z1
+template <class T>
+__global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ z1,
+                                                 T *__restrict__ nu, const T *__restrict__ rhat,
+                                                 const Scal *st, double *p2) {
+  const int c = lane_id();
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int k = V.item_inst[item];
+    if (st[k].done) continue;
+    const int b0 = V.item_b0[item], cnt = V.item_nb[item];
+    const long long mr = V.mean_row[k];
+    double v[2] = {0.0, 0.0};
+    auto store_y = [&](long i, double y, double rh) {
+      if (i == mr) y = 0.0;  // completed by kb_alpha
+      const T yt = (T)y;
+      nu[i] = yt;
+      v[0] += rh * (double)yt;
+    };
+    if (cnt > 0) {
+      item_blocks<T>(V, z1, rhat, b0, cnt, [&](const BlockRows<UB, T> &R, int u, int bb) {
+        bool plain;
+        const double y = R.y(u, c, plain);
+        v[1] += R.hh[u] * R.xc[u];
+        if (plain) store_y(R.i[u], y, R.a[u]);
+      });
+    } else if ((int)threadIdx.x < -cnt) {
+      const int e = b0 + threadIdx.x, row = V.A.erow[e];
+      store_y(row, explicit_row<T>(V.A, z1, e), (double)rhat[row]);
+    }
+    store_item_partials<2, 0u>(v, p2, V.n_items, item);
+  }
+}
+
+// t = A z2; partials of (t, s), (t, t) and sum h^2 z2
+template <class T>
+__global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z2,
+                                                T *__restrict__ t, const T *__restrict__ sv,
+                                                const Scal *st, double *p4) {
+  const int c = lane_id();
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int k = V.item_inst[item];
+    if (st[k].done) continue;
+    const int b0 = V.item_b0[item], cnt = V.item_nb[item];
+    const long long mr = V.mean_row[k];
+    double v[3] = {0.0, 0.0, 0.0};
+    auto store_y = [&](long i, double y, double si) {
+      if (i == mr) y = 0.0;  // completed by kb_omega
+      const T yt = (T)y;
+      t[i] = yt;
+      const double yd = (double)yt;
+      v[0] += yd * si;
+      v[1] += yd * yd;
+    };
+    if (cnt > 0) {
+      item_blocks<T>(V, z2, sv, b0, cnt, [&](const BlockRows<UB, T> &R, int u, int bb) {
+        bool plain;
+        const double y = R.y(u, c, plain);
+        v[2] += R.hh[u] * R.xc[u];
+        if (plain) store_y(R.i[u], y, R.a[u]);
+      });
+    } else if ((int)threadIdx.x < -cnt) {
+      const int e = b0 + threadIdx.x, row = V.A.erow[e];
+      store_y(row, explicit_row<T>(V.A, z2, e), (double)sv[row]);
+    }
+    store_item_partials<3, 0u>(v, p4, V.n_items, item);
+  }
+}
+
+// x_opt = x when the previous iterate was the best so far; x += alpha z1 +
+// omega z2 (FP64 x); r = s - omega t; update partials
+template <class T>
+__global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ x,
+                                                  double *__restrict__ xopt,
+                                                  const T *__restrict__ z1,
+                                                  const T *__restrict__ z2, T *__restrict__ r,
+                                                  const T *__restrict__ t,
+                                                  const T *__restrict__ rhat, const Scal *st,
+                                                  double *p5) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int k = V.item_inst[item];
+    if (st[k].done) continue;
+    const double alpha = st[k].alpha, omega = st[k].omega;
+    const int improved = st[k].improved;
+    const int cnt = V.item_nb[item];
+    const long i0 = (long)V.item_b0[item] * 64, i1 = i0 + (cnt > 0 ? (long)cnt * 64 : 0);
+    double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) {
+      const double xo = x[i];
+      if (improved) xopt[i] = xo;
+      x[i] = xo + alpha * (double)z1[i] + omega * (double)z2[i];
+      const T rt = (T)((double)r[i] - omega * (double)t[i]);
+      r[i] = rt;
+      const double rd = (double)rt, rh = (double)rhat[i];
+      v[0] = nmax(v[0], fabs(rd));
+      v[1] += rd * rd;
+      v[2] += rh * rd;
+      v[3] += rh * rh;
+      v[4] += rd;
+      v[5] += rh;
+    }
+    store_item_partials<NQB, 1u>(v, p5, V.n_items, item);
+  }
+}
+
+// cell range of a block item (empty for an explicit item)
+__device__ __forceinline__ void item_cells(const BView &V, int item, long &i0, long &i1) {
+  const int cnt = V.item_nb[item];
+  i0 = (long)V.item_b0[item] * 64;
+  i1 = i0 + (cnt > 0 ? (long)cnt * 64 : 0);
+}
+
+// x = x_opt on the instances whose last iterate is not their best
+__global__ __launch_bounds__(WG) void kb_select(BView V, double *__restrict__ x,
+                                                const double *__restrict__ xopt, const Scal *st) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    if (st[V.item_inst[item]].improved) continue;
+    long i0, i1;
+    item_cells(V, item, i0, i1);
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) x[i] = xopt[i];
+  }
+}
+
+// partials of sum h^2 x and sum h^2 (mean removal, ExpAMRSolver.cpp:395-411)
+__global__ __launch_bounds__(WG) void kb_fmean(BView V, const double *__restrict__ x, double *pf) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    long i0, i1;
+    item_cells(V, item, i0, i1);
+    double v[2] = {0.0, 0.0};
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) {
+      const double hh = V.A.h2[i >> 6];
+      v[0] += x[i] * hh;
+      v[1] += hh;
+    }
+    store_item_partials<2, 0u>(v, pf, V.n_items, item);
+  }
+}
+
+// out = x - mean_k
+__global__ __launch_bounds__(WG) void kb_fwrite(BView V, const double *__restrict__ x,
+                                                double *__restrict__ out, const Scal *st) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const double mean = st[V.item_inst[item]].mean;
+    long i0, i1;
+    item_cells(V, item, i0, i1);
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) out[i] = x[i] - mean;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// tile kernels (preconditioner; a wavefront per 16-block tile)
+// ----------------------------------------------------------------------------
+
+// FP64: p = r + beta (p - omega nu), or on restart p = rhat = r; z1 = P p
+// (P in LDS, 4 blocks per step)
+__global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__restrict__ P,
+                                                   const double *__restrict__ r,
+                                                   double *__restrict__ rhat,
+                                                   double *__restrict__ p,
+                                                   const double *__restrict__ nu,
+                                                   double *__restrict__ z1, const Scal *st) {
+  __shared__ double Ps[64 * 64];
+  __shared__ double vs[WAVES][4 * 64];
+  load_P(Ps, P);
+  const int c = lane_id(), w = wave_id();
+  for (int tile = blockIdx.x * WAVES + w; tile < V.n_tiles; tile += gridDim.x * WAVES) {
+    const int k = V.tile_inst[tile];
+    if (st[k].done) continue;
+    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+    const double beta = st[k].beta, omega = st[k].omega;
+    const int restart = st[k].restart;
+    for (int g = 0; g < tnb; g += 4) {
+      long ii[4];
+      bool ok[4];
+      double rv[4], pv[4] = {}, nv[4] = {};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        ok[j] = g + j < tnb;
+        ii[j] = (long)(b0 + (ok[j] ? g + j : g)) * 64 + c;
+        rv[j] = r[ii[j]];
+      }
+      if (!restart)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          pv[j] = p[ii[j]];
+          nv[j] = nu[ii[j]];
+        }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const double pi = restart ? rv[j] : rv[j] + beta * (pv[j] - omega * nv[j]);
+        if (ok[j]) {
+          if (restart) rhat[ii[j]] = rv[j];
+          p[ii[j]] = pi;
+        }
+        vs[w][j * 64 + c] = ok[j] ? pi : 0.0;
+      }
+      wave_sync();
+      double z[4];
+      prec_group<4>(Ps, vs[w], c, z);
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (ok[j]) z1[ii[j]] = z[j];
+    }
+  }
+}
+
+// FP64: s = r - alpha nu (stored in r); z2 = P s
+__global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__restrict__ P,
+                                                   double *__restrict__ r,
+                                                   const double *__restrict__ nu,
+                                                   double *__restrict__ z2, const Scal *st) {
+  __shared__ double Ps[64 * 64];
+  __shared__ double vs[WAVES][4 * 64];
+  load_P(Ps, P);
+  const int c = lane_id(), w = wave_id();
+  for (int tile = blockIdx.x * WAVES + w; tile < V.n_tiles; tile += gridDim.x * WAVES) {
+    const int k = V.tile_inst[tile];
+    if (st[k].done) continue;
+    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+    const double alpha = st[k].alpha;
+    for (int g = 0; g < tnb; g += 4) {
+      long ii[4];
+      bool ok[4];
+      double rv[4], nv[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        ok[j] = g + j < tnb;
+        ii[j] = (long)(b0 + (ok[j] ? g + j : g)) * 64 + c;
+        rv[j] = r[ii[j]];
+        nv[j] = nu[ii[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const double si = rv[j] - alpha * nv[j];
+        if (ok[j]) r[ii[j]] = si;
+        vs[w][j * 64 + c] = ok[j] ? si : 0.0;
+      }
+      wave_sync();
+      double z[4];
+      prec_group<4>(Ps, vs[w], c, z);
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (ok[j]) z2[ii[j]] = z[j];
+    }
+  }
+}
+
+// FP32 storage, BF16 MFMA: r -= mu (projection); p update or restart;
+// z1 = P p
+__global__ __launch_bounds__(WG) void kb_pupdate32(BView V, const bf16x8 *__restrict__ pf,
+                                                   float *__restrict__ r, float *__restrict__ rhat,
+                                                   float *__restrict__ p,
+                                                   const float *__restrict__ nu,
+                                                   float *__restrict__ z1, const Scal *st) {
+  const int lane = lane_id(), q = lane >> 4, j = lane & 15;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  for (int tile = blockIdx.x * WAVES + wave_id(); tile < V.n_tiles; tile += gridDim.x * WAVES) {
+    const int k = V.tile_inst[tile];
+    if (st[k].done) continue;
+    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+    const double beta = st[k].beta, omega = st[k].omega, mu = st[k].mu;
+    const int restart = st[k].restart;
+    const bool ok = j < tnb;
+    const long base = (long)(b0 + (ok ? j : 0)) * 64;
+    float pin[16];
+    mf_pupdate(r, rhat, p, nu, base, q, ok, beta, omega, mu, restart, pin);
+    float z[16];
+    prec_mfma(P, pin, z);
+    if (ok) st16_out(z1, base, q, z);
+  }
+}
+
+// FP32 storage, BF16 MFMA: s = r - alpha nu (stored in r); z2 = P s
+__global__ __launch_bounds__(WG) void kb_supdate32(BView V, const bf16x8 *__restrict__ pf,
+                                                   float *__restrict__ r,
+                                                   const float *__restrict__ nu,
+                                                   float *__restrict__ z2, const Scal *st) {
+  const int lane = lane_id(), q = lane >> 4, j = lane & 15;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  for (int tile = blockIdx.x * WAVES + wave_id(); tile < V.n_tiles; tile += gridDim.x * WAVES) {
+    const int k = V.tile_inst[tile];
+    if (st[k].done) continue;
+    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+    const double alpha = st[k].alpha;
+    const bool ok = j < tnb;
+    const long base = (long)(b0 + (ok ? j : 0)) * 64;
+    float rv[16], nv[16], sv[16];
+    ld16(r, base, q, rv);
+    ld16(nu, base, q, nv);
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) sv[kk] = ok ? (float)((double)rv[kk] - alpha * (double)nv[kk]) : 0.f;
+    if (ok) st16_in(r, base, q, sv);
+    float z[16];
+    prec_mfma(P, sv, z);
+    if (ok) st16_out(z2, base, q, z);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// per-instance scalar kernels (a workgroup per instance)
+// ----------------------------------------------------------------------------
+
+// alpha_k = rho_k / ((rhat, nu)_k + eps) (BiCGSTAB.cu:523-531); completes
+// the mean row of nu
+template <class T>
+__global__ __launch_bounds__(WG) void kb_alpha(BView V, Scal *st, const double *p2,
+                                               T *__restrict__ nu, const T *__restrict__ rhat) {
+  const int k = blockIdx.x;
+  if (st[k].done) return;
+  double o[2];
+  range_totals<2, 0u>(p2, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (threadIdx.x == 0) {
+    double rn = o[0];
+    const long long mr = V.mean_row[k];
+    if (mr >= 0) {
+      const T m = (T)o[1];
+      nu[mr] = m;
+      rn += (double)rhat[mr] * (double)m;
+    }
+    st[k].alpha = st[k].rho / (rn + st[k].eps);
+    st[k].red[2] = rn;
+  }
+}
+
+// omega_k = (t, s)_k / ((t, t)_k + eps) (BiCGSTAB.cu:548-556); completes the
+// mean row of t
+template <class T>
+__global__ __launch_bounds__(WG) void kb_omega(BView V, Scal *st, const double *p4,
+                                               T *__restrict__ t, const T *__restrict__ sv) {
+  const int k = blockIdx.x;
+  if (st[k].done) return;
+  double o[3];
+  range_totals<3, 0u>(p4, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (threadIdx.x == 0) {
+    double ts = o[0], tt = o[1];
+    const long long mr = V.mean_row[k];
+    if (mr >= 0) {
+      const T m = (T)o[2];
+      t[mr] = m;
+      const double md = (double)m;
+      ts += md * (double)sv[mr];
+      tt += md * md;
+    }
+    st[k].omega = ts / (tt + st[k].eps);
+    st[k].red[3] = ts;
+    st[k].red[4] = tt;
+  }
+}
+
+// Next state of every unfinished instance; after kb_init, its initial state
+// (the mean row of r0 is completed here: b = 0 on that row,
+// ExpAMRSolver.cpp:340-345, and (A x)_row = sum h^2 x).  `init_check`: an
+// initial residual already within tolerance ends the instance (the FP64
+// residual check of a refinement pass).  A non-finite initial residual ends
+// it unconverged.
+template <class T>
+__global__ __launch_bounds__(WG) void kb_eval(BView V, Scal *st, const double *p5,
+                                              T *__restrict__ r, T *__restrict__ rhat,
+                                              int init_check) {
+  const int k = blockIdx.x;
+  if (st[k].done) return;
+  double o[NQB];
+  range_totals<NQB, 1u>(p5, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (threadIdx.x == 0) {
+    const Scal prev = st[k];
+    double oo[NQ5];
+    for (int q = 0; q < NQ5; q++) oo[q] = o[q];
+    const long long mr = V.mean_row[k];
+    if (prev.iter < 0 && mr >= 0) {
+      const T rm = (T)(0.0 - o[6]);
+      r[mr] = rm;
+      rhat[mr] = rm;
+      const double rd = (double)rm;
+      oo[0] = nmax(oo[0], fabs(rd));
+      oo[1] += rd * rd;
+      oo[2] += rd * rd;
+      oo[3] += rd * rd;
+      oo[4] += rd;
+      oo[5] += rd;
+    }
+    Scal s = evaluate(prev, oo);
+    if (prev.iter < 0) {
+      if (s.err != s.err || s.err > 1.79e308) {
+        s.done = 1;
+      } else if (s.err == 0.0 || (init_check && (s.err <= s.tol_abs || s.tol_rel >= 1.0))) {
+        s.converged = 1;
+        s.done = 1;
+      }
+    }
+    st[k] = s;
+  }
+}
+
+// all_done = no instance is unfinished
+__global__ __launch_bounds__(WG) void kb_alldone(int K, const Scal *st, int *all_done) {
+  int busy = 0;
+  for (int k = threadIdx.x; k < K; k += WG) busy |= st[k].done ? 0 : 1;
+  busy = __syncthreads_or(busy);
+  if (threadIdx.x == 0) *all_done = busy ? 0 : 1;
+}
+
+// mean_k = sum h^2 x / sum h^2
+__global__ __launch_bounds__(WG) void kb_mean(BView V, Scal *st, const double *pf) {
+  const int k = blockIdx.x;
+  double o[2];
+  range_totals<2, 0u>(pf, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (threadIdx.x == 0) st[k].mean = o[0] / o[1];
+}
+
+std::string g_berr;
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+
+struct cup2d_psb {
+  cup2d_ps_config cfg{};
+  std::string err;
+  bool mixed = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ready = false;
+  int K = 0;
+  long long nb = 0, n = 0;
+  int ib = 0;                       // blocks per item
+  std::vector<long long> boff;      // [K+1] block offset of each instance
+  std::vector<long long> eoff;      // [K+1] explicit-row offset of each instance
+  std::vector<long long> h_mean_row;
+  std::vector<int> item_inst, item_b0, item_nb, item_begin;
+  std::vector<int> tile_inst, tile_b0, tile_nb;
+  // device: operator
+  int4 *d_nbr = nullptr;
+  unsigned long long *d_mask = nullptr;
+  int *d_erow = nullptr, *d_ecol = nullptr;
+  double *d_eval = nullptr, *d_h2 = nullptr, *d_P = nullptr;
+  int ell_w = 0;
+  bf16x8 *d_pfrag = nullptr;
+  long long *d_mean_row = nullptr;
+  // device: work units
+  int *d_item_inst = nullptr, *d_item_b0 = nullptr, *d_item_nb = nullptr, *d_item_begin = nullptr;
+  int *d_tile_inst = nullptr, *d_tile_b0 = nullptr, *d_tile_nb = nullptr;
+  double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr;
+  // device: vectors and state
+  double *d_x = nullptr, *d_xopt = nullptr, *d_b = nullptr, *d_xio = nullptr;
+  void *v_r = nullptr, *v_rhat = nullptr, *v_p = nullptr, *v_nu = nullptr, *v_z1 = nullptr,
+       *v_z2 = nullptr, *v_t = nullptr;  // Krylov vectors, FP64 or FP32
+  Scal *d_st = nullptr;
+  int *d_all_done = nullptr;
+  int *h_all_done = nullptr;        // pinned
+  std::vector<Scal> h_st;
+  hipGraphExec_t gexec = nullptr;
+  int gchunk = 0;
+  std::vector<void *> allocs;       // mesh-dependent device memory
+  std::vector<void *> unit_allocs;  // work-unit lists and partials (depend on ib)
+};
+
+namespace {
+
+int bfail(cup2d_psb *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  else g_berr = msg;
+  return code;
+}
+
+#define BCHK(h, expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return bfail((h), _e == hipErrorOutOfMemory ? CUP2D_PS_ENOMEM : CUP2D_PS_EHIP,  \
+                   std::string(#expr) + ": " + hipGetErrorString(_e));                \
+  } while (0)
+
+template <class T>
+int balloc(cup2d_psb *h, std::vector<void *> &pool, T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  BCHK(h, hipMalloc((void **)p, count * sizeof(T)));
+  pool.push_back(*p);
+  return CUP2D_PS_OK;
+}
+
+template <class T>
+int bupload(cup2d_psb *h, std::vector<void *> &pool, T **p, const std::vector<T> &v) {
+  int rc = balloc(h, pool, p, v.size());
+  if (rc) return rc;
+  if (!v.empty()) BCHK(h, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return CUP2D_PS_OK;
+}
+
+void bfree_pool(std::vector<void *> &pool) {
+  for (void *p : pool) (void)hipFree(p);
+  pool.clear();
+}
+
+void drop_graph(cup2d_psb *h) {
+  if (h->gexec) {
+    (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+}
+
+BView bview(const cup2d_psb *h) {
+  BView V;
+  V.A.nb = (int)h->nb;
+  V.A.mean_row = -1;
+  V.A.nbr = h->d_nbr;
+  V.A.mask = h->d_mask;
+  V.A.n_exc = (int)h->eoff[h->K];
+  V.A.ell_w = h->ell_w;
+  V.A.erow = h->d_erow;
+  V.A.ecol = h->d_ecol;
+  V.A.eval = h->d_eval;
+  V.A.h2 = h->d_h2;
+  V.item_inst = h->d_item_inst;
+  V.item_b0 = h->d_item_b0;
+  V.item_nb = h->d_item_nb;
+  V.item_begin = h->d_item_begin;
+  V.tile_inst = h->d_tile_inst;
+  V.tile_b0 = h->d_tile_b0;
+  V.tile_nb = h->d_tile_nb;
+  V.mean_row = h->d_mean_row;
+  V.K = h->K;
+  V.n_items = (int)h->item_inst.size();
+  V.n_tiles = (int)h->tile_inst.size();
+  return V;
+}
+
+int grid_items(const cup2d_psb *h) {
+  return (int)std::min<size_t>(BGRID, std::max<size_t>(1, h->item_inst.size()));
+}
+int grid_tiles(const cup2d_psb *h) {
+  return (int)std::min<size_t>(BGRID, std::max<size_t>(1, (h->tile_inst.size() + WAVES - 1) / WAVES));
+}
+
+// Item and tile lists for `ib` blocks per item (tiles are TILE blocks; an
+// instance's last item and tile may be shorter); each instance's explicit
+// rows follow its block items as items of up to WG rows.
+int build_units(cup2d_psb *h, int ib) {
+  drop_graph(h);
+  bfree_pool(h->unit_allocs);
+  h->ib = ib;
+  h->item_inst.clear();
+  h->item_b0.clear();
+  h->item_nb.clear();
+  h->item_begin.assign(1, 0);
+  h->tile_inst.clear();
+  h->tile_b0.clear();
+  h->tile_nb.clear();
+  for (int k = 0; k < h->K; k++) {
+    const long long b0 = h->boff[k], b1 = h->boff[k + 1];
+    for (long long b = b0; b < b1; b += ib) {
+      h->item_inst.push_back(k);
+      h->item_b0.push_back((int)b);
+      h->item_nb.push_back((int)std::min<long long>(ib, b1 - b));
+    }
+    for (long long e = h->eoff[k]; e < h->eoff[k + 1]; e += WG) {
+      h->item_inst.push_back(k);
+      h->item_b0.push_back((int)e);
+      h->item_nb.push_back(-(int)std::min<long long>(WG, h->eoff[k + 1] - e));
+    }
+    h->item_begin.push_back((int)h->item_inst.size());
+    for (long long b = b0; b < b1; b += TILE) {
+      h->tile_inst.push_back(k);
+      h->tile_b0.push_back((int)b);
+      h->tile_nb.push_back((int)std::min<long long>(TILE, b1 - b));
+    }
+  }
+  const size_t NI = h->item_inst.size();
+  auto &pool = h->unit_allocs;
+  int rc;
+  if ((rc = bupload(h, pool, &h->d_item_inst, h->item_inst)) ||
+      (rc = bupload(h, pool, &h->d_item_b0, h->item_b0)) ||
+      (rc = bupload(h, pool, &h->d_item_nb, h->item_nb)) ||
+      (rc = bupload(h, pool, &h->d_item_begin, h->item_begin)) ||
+      (rc = bupload(h, pool, &h->d_tile_inst, h->tile_inst)) ||
+      (rc = bupload(h, pool, &h->d_tile_b0, h->tile_b0)) ||
+      (rc = bupload(h, pool, &h->d_tile_nb, h->tile_nb)) ||
+      (rc = balloc(h, pool, &h->d_p2, 2 * NI)) || (rc = balloc(h, pool, &h->d_p4, 3 * NI)) ||
+      (rc = balloc(h, pool, &h->d_p5, (size_t)NQB * NI)) || (rc = balloc(h, pool, &h->d_pm, 2 * NI)))
+    return rc;
+  return CUP2D_PS_OK;
+}
+
+// One BiCGSTAB iteration of every unfinished instance: 8 kernels.
+template <class T>
+void launch_iteration_b(cup2d_psb *h) {
+  const BView V = bview(h);
+  const int Gi = grid_items(h), Gt = grid_tiles(h), K = h->K;
+  hipStream_t s = h->stream;
+  T *r = (T *)h->v_r, *rhat = (T *)h->v_rhat, *p = (T *)h->v_p, *nu = (T *)h->v_nu,
+    *z1 = (T *)h->v_z1, *z2 = (T *)h->v_z2, *t = (T *)h->v_t;
+  Scal *st = h->d_st;
+  if constexpr (sizeof(T) == 8)
+    kb_pupdate64<<<Gt, WG, 0, s>>>(V, h->d_P, r, rhat, p, nu, z1, st);
+  else
+    kb_pupdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, rhat, p, nu, z1, st);
+  kb_spmv_nu<T><<<Gi, WG, 0, s>>>(V, z1, nu, rhat, st, h->d_p2);
+  kb_alpha<T><<<K, WG, 0, s>>>(V, st, h->d_p2, nu, rhat);
+  if constexpr (sizeof(T) == 8)
+    kb_supdate64<<<Gt, WG, 0, s>>>(V, h->d_P, r, nu, z2, st);
+  else
+    kb_supdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, nu, z2, st);
+  kb_spmv_t<T><<<Gi, WG, 0, s>>>(V, z2, t, r, st, h->d_p4);
+  kb_omega<T><<<K, WG, 0, s>>>(V, st, h->d_p4, t, r);
+  kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, z1, z2, r, t, rhat, st, h->d_p5);
+  kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0);
+}
+
+int chunk_b(const cup2d_psb *h) { return h->cfg.chunk > 0 ? h->cfg.chunk : 8; }
+
+void launch_chunk_b(cup2d_psb *h, int C) {
+  for (int i = 0; i < C; i++) {
+    if (h->mixed) launch_iteration_b<float>(h);
+    else launch_iteration_b<double>(h);
+  }
+  kb_alldone<<<1, WG, 0, h->stream>>>(h->K, h->d_st, h->d_all_done);
+}
+
+int ensure_graph_b(cup2d_psb *h) {
+  const int C = chunk_b(h);
+  if (h->gexec && h->gchunk == C) return CUP2D_PS_OK;
+  drop_graph(h);
+  hipGraph_t g = nullptr;
+  BCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  launch_chunk_b(h, C);
+  BCHK(h, hipStreamEndCapture(h->stream, &g));
+  hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return bfail(h, CUP2D_PS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  h->gchunk = C;
+  return CUP2D_PS_OK;
+}
+
+int read_all_done(cup2d_psb *h, int &done) {
+  BCHK(h, hipMemcpyAsync(h->h_all_done, h->d_all_done, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  BCHK(h, hipStreamSynchronize(h->stream));
+  done = *h->h_all_done;
+  return CUP2D_PS_OK;
+}
+
+// Chunks of iterations until every instance is done; `max_iter` bounds the
+// launches (each instance's own cap is in its state), `max_chunks` > 0 stops
+// earlier (timing runs).
+int run_chunks_b(cup2d_psb *h, int max_iter, int max_chunks) {
+  const int C = chunk_b(h);
+  if (h->cfg.use_graph) {
+    int rc = ensure_graph_b(h);
+    if (rc) return rc;
+  }
+  int launched = 0, chunks = 0;
+  for (;;) {
+    if (h->cfg.use_graph) {
+      BCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    } else {
+      launch_chunk_b(h, C);
+      BCHK(h, hipGetLastError());
+    }
+    launched += C;
+    chunks++;
+    int done = 0;
+    int rc = read_all_done(h, done);
+    if (rc) return rc;
+    if (done || launched > max_iter + C) break;
+    if (max_chunks > 0 && chunks >= max_chunks) break;
+  }
+  return CUP2D_PS_OK;
+}
+
+Scal marker_b(const cup2d_psb *h, int k, double tol_abs, double tol_rel, int max_restarts,
+              int max_iter) {
+  Scal s{};
+  s.n = 64.0 * double(h->boff[k + 1] - h->boff[k]);
+  s.project = (h->mixed && h->h_mean_row[k] < 0) ? 1 : 0;
+  s.stall = h->mixed ? 150 : 0;
+  s.tol_abs = tol_abs;
+  s.tol_rel = tol_rel;
+  s.eps = h->cfg.eps;
+  s.bd_tol = h->cfg.breakdown_tol;
+  s.max_iter = max_iter;
+  s.max_restarts = max_restarts;
+  s.norm = h->cfg.norm;
+  s.iter = -1;
+  return s;
+}
+
+// One pass over the states in h_st (marker states start, done states are
+// left alone): r0 = b - A x in FP64 for the starting instances, iterations
+// until every instance is done, x = x_opt where the last iterate is not the
+// best.  Returns with h_st holding the final states.
+template <class T>
+int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
+  const BView V = bview(h);
+  const int Gi = grid_items(h), K = h->K;
+  T *r = (T *)h->v_r, *rhat = (T *)h->v_rhat, *p = (T *)h->v_p, *nu = (T *)h->v_nu;
+  BCHK(h, hipMemcpyAsync(h->d_st, h->h_st.data(), K * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
+  kb_init<T><<<Gi, WG, 0, h->stream>>>(V, d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5);
+  kb_eval<T><<<K, WG, 0, h->stream>>>(V, h->d_st, h->d_p5, r, rhat, sizeof(T) == 4 ? 1 : 0);
+  kb_alldone<<<1, WG, 0, h->stream>>>(K, h->d_st, h->d_all_done);
+  BCHK(h, hipGetLastError());
+  int done = 0;
+  int rc = read_all_done(h, done);
+  if (rc) return rc;
+  if (!done && max_iter > 0) {
+    if ((rc = run_chunks_b(h, max_iter, max_chunks))) return rc;
+  }
+  kb_select<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_xopt, h->d_st);
+  BCHK(h, hipGetLastError());
+  BCHK(h, hipMemcpyAsync(h->h_st.data(), h->d_st, K * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
+  BCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
+// FP64: one pass.  Mixed: per-instance FP64 refinement -- each pass starts
+// from the FP64 residual of the instance's current x, an instance whose FP64
+// residual meets its target (or that cannot progress) leaves the batch, the
+// others iterate again towards the full remaining reduction.
+int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
+            double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "solve before cup2d_psb_set_meshes");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  const int K = h->K, max_iter = h->cfg.max_iter;
+  BCHK(h, hipEventRecord(h->ev0, h->stream));
+  BCHK(h, hipMemcpyAsync(h->d_x, d_x0, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  std::vector<int> live(K, 1), iters(K, 0), restarts(K, 0), passes(K, 0), conv(K, 0);
+  std::vector<double> err0(K, -1.0), errf(K, 0.0);
+  for (int k = 0; k < K; k++) h->h_st[k] = marker_b(h, k, tol_abs, tol_rel, max_restarts, max_iter);
+  if (!h->mixed) {
+    int rc = run_pass<double>(h, d_b, max_iter, -1);
+    if (rc) return rc;
+    for (int k = 0; k < K; k++) {
+      const Scal &s = h->h_st[k];
+      iters[k] = std::max(s.iter, 0);
+      restarts[k] = s.restarts;
+      conv[k] = s.converged;
+      err0[k] = s.err0;
+      errf[k] = s.iter > 0 ? s.err_opt : s.err0;
+    }
+  } else {
+    for (;;) {
+      int rc = run_pass<float>(h, d_b, max_iter, -1);
+      if (rc) return rc;
+      int any = 0;
+      for (int k = 0; k < K; k++) {
+        if (!live[k]) continue;
+        Scal &s = h->h_st[k];
+        if (err0[k] < 0) err0[k] = s.err0;
+        if (s.iter <= 0) {
+          // decided by the pass's FP64 residual check
+          live[k] = 0;
+          conv[k] = s.converged;
+          errf[k] = s.err0;
+          continue;
+        }
+        iters[k] += s.iter;
+        restarts[k] += s.restarts;
+        passes[k]++;
+        // next pass: check the FP64 residual of the new x against the
+        // caller's target; iterate again only within the pass and
+        // iteration budgets
+        const double target = std::max(tol_abs, tol_rel * err0[k]);
+        const int budget = (passes[k] >= MAX_PASSES) ? 0 : std::max(0, max_iter - iters[k]);
+        s = marker_b(h, k, target, 0.0, max_restarts, budget);
+        any = 1;
+      }
+      if (!any) break;
+      for (int k = 0; k < K; k++)
+        if (!live[k]) h->h_st[k].done = 1;
+    }
+  }
+  const BView V = bview(h);
+  const int Gi = grid_items(h);
+  kb_fmean<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_pm);
+  kb_mean<<<K, WG, 0, h->stream>>>(V, h->d_st, h->d_pm);
+  kb_fwrite<<<Gi, WG, 0, h->stream>>>(V, h->d_x, d_out, h->d_st);
+  BCHK(h, hipGetLastError());
+  BCHK(h, hipEventRecord(h->ev1, h->stream));
+  BCHK(h, hipStreamSynchronize(h->stream));
+  float ms = 0;
+  BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (stats)
+    for (int k = 0; k < K; k++) {
+      std::memset(&stats[k], 0, sizeof(cup2d_ps_stats));
+      stats[k].iterations = iters[k];
+      stats[k].restarts = restarts[k];
+      stats[k].converged = conv[k];
+      stats[k].refinements = passes[k];
+      stats[k].err_init = err0[k];
+      stats[k].err_final = errf[k];
+      stats[k].ms = ms;
+    }
+  return CUP2D_PS_OK;
+}
+
+// ms per batched iteration with the current units: a warm-up run, then a
+// timed run of 4 chunks with zero tolerances (x starts at 0)
+int time_units(cup2d_psb *h, const double *d_b, double &ms_per_iter) {
+  const int K = h->K, C = chunk_b(h);
+  float ms = 0;
+  for (int rep = 0; rep < 2; rep++) {
+    for (int k = 0; k < K; k++) h->h_st[k] = marker_b(h, k, 0.0, 0.0, 1 << 20, 1 << 20);
+    BCHK(h, hipMemsetAsync(h->d_x, 0, h->n * sizeof(double), h->stream));
+    BCHK(h, hipEventRecord(h->ev0, h->stream));
+    int rc = h->mixed ? run_pass<float>(h, d_b, 4 * C, 4) : run_pass<double>(h, d_b, 4 * C, 4);
+    if (rc) return rc;
+    BCHK(h, hipEventRecord(h->ev1, h->stream));
+    BCHK(h, hipEventSynchronize(h->ev1));
+    BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  }
+  ms_per_iter = ms / (4.0 * C);
+  return CUP2D_PS_OK;
+}
+
+// Tuned item sizes of this process, by (precision, instances, blocks).
+std::map<std::tuple<int, int, long long>, int> g_tuned;
+
+bool config_ok(const cup2d_ps_config &c) {
+  return c.max_iter > 0 && c.chunk >= 0 && c.eps >= 0 && c.breakdown_tol >= 0 &&
+         (c.precision == CUP2D_PS_FP64 || c.precision == CUP2D_PS_MIXED) &&
+         (c.norm == CUP2D_PS_NORM_INF || c.norm == CUP2D_PS_NORM_L2) && c.reserved_i[1] >= 0 &&
+         c.reserved_i[1] <= 1024;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cup2d_psb_create(const cup2d_ps_config *cfg, cup2d_psb **out) {
+  if (!out) return bfail(nullptr, CUP2D_PS_EINVAL, "out is NULL");
+  *out = nullptr;
+  cup2d_ps_config c;
+  if (cfg) c = *cfg;
+  else cup2d_ps_config_default(&c);
+  if (!config_ok(c)) return bfail(nullptr, CUP2D_PS_EINVAL, "invalid configuration");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return bfail(nullptr, CUP2D_PS_ENODEV, "no HIP device");
+  if (c.device < 0 || c.device >= ndev) return bfail(nullptr, CUP2D_PS_EINVAL, "bad device ordinal");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c.device) != hipSuccess)
+    return bfail(nullptr, CUP2D_PS_ENODEV, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return bfail(nullptr, CUP2D_PS_ENODEV, std::string("built for gfx950, device is ") + prop.gcnArchName);
+  auto h = std::make_unique<cup2d_psb>();
+  h->cfg = c;
+  h->mixed = c.precision == CUP2D_PS_MIXED;
+  if (hipSetDevice(c.device) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      hipHostMalloc((void **)&h->h_all_done, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    cup2d_psb_destroy(h.release());
+    return bfail(nullptr, CUP2D_PS_EHIP, "stream/event/pinned allocation failed");
+  }
+  *out = h.release();
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_destroy(cup2d_psb *h) {
+  if (!h) return CUP2D_PS_OK;
+  (void)hipSetDevice(h->cfg.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  bfree_pool(h->unit_allocs);
+  bfree_pool(h->allocs);
+  if (h->h_all_done) (void)hipHostFree(h->h_all_done);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!meshes || n_inst <= 0 || n_inst > MAX_INST)
+    return bfail(h, CUP2D_PS_EINVAL, "need 1.." + std::to_string(MAX_INST) + " meshes");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  BCHK(h, hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  bfree_pool(h->unit_allocs);
+  bfree_pool(h->allocs);
+  h->ready = false;
+  std::vector<cup2d::Operator> ops(n_inst);
+  std::vector<long long> boff(n_inst + 1, 0);
+  for (int k = 0; k < n_inst; k++) {
+    const cup2d_mesh_desc &m = meshes[k];
+    MeshDesc md{m.bpdx, m.bpdy, m.levels, m.periodic_x, m.periodic_y, m.extent, m.n_blocks,
+                m.level, m.bx, m.by};
+    std::string err;
+    if (!build_operator(md, h->cfg.mean_constraint, ops[k], err))
+      return bfail(h, CUP2D_PS_EINVAL, "mesh " + std::to_string(k) + ": " + err);
+    boff[k + 1] = boff[k] + ops[k].nb;
+  }
+  if (boff[n_inst] * 64 >= (1LL << 31) - 64)
+    return bfail(h, CUP2D_PS_EINVAL, "batch too large for 32-bit cell indices");
+  h->K = n_inst;
+  h->boff = boff;
+  h->nb = boff[n_inst];
+  h->n = h->nb * 64;
+  // concatenated operator: block ids, explicit rows and their columns shifted
+  // by each instance's offsets
+  int W = 0;
+  for (int k = 0; k < n_inst; k++) W = std::max(W, ell_width(ops[k]));
+  if (W > ELL_MAX)
+    return bfail(h, CUP2D_PS_EINVAL, "explicit row with more than " + std::to_string(ELL_MAX) + " entries");
+  std::vector<int4> nbr;
+  std::vector<unsigned long long> mask;
+  std::vector<double> h2;
+  std::vector<int> erow, ecol_rm;
+  std::vector<double> eval_rm;
+  nbr.reserve(h->nb);
+  mask.reserve(h->nb);
+  h2.reserve(h->nb);
+  h->h_mean_row.assign(n_inst, -1);
+  h->eoff.assign(n_inst + 1, 0);
+  for (int k = 0; k < n_inst; k++) {
+    const cup2d::Operator &op = ops[k];
+    const int bo = (int)boff[k];
+    const long long ro = boff[k] * 64;
+    auto shift = [bo](int v) { return v >= 0 ? v + bo : v; };
+    for (long long b = 0; b < op.nb; b++) {
+      int4 v;
+      v.x = shift(op.nbr[4 * b]);
+      v.y = shift(op.nbr[4 * b + 1]);
+      v.z = shift(op.nbr[4 * b + 2]);
+      v.w = shift(op.nbr[4 * b + 3]);
+      nbr.push_back(v);
+      mask.push_back(op.exc_mask[b]);
+      h2.push_back(op.h2[b]);
+    }
+    ell_append(op, W, ro, erow, ecol_rm, eval_rm);
+    h->eoff[k + 1] = (long long)erow.size();
+    if (op.mean_row >= 0) h->h_mean_row[k] = op.mean_row + ro;
+  }
+  h->ell_w = W;
+  ops.clear();
+  auto &pool = h->allocs;
+  int rc;
+  const std::vector<double> P = make_pinv();
+  const std::vector<uint16_t> pf = make_pfrag(P);
+  if ((rc = bupload(h, pool, &h->d_nbr, nbr)) || (rc = bupload(h, pool, &h->d_mask, mask)) ||
+      (rc = bupload(h, pool, &h->d_erow, erow)) ||
+      (rc = bupload(h, pool, &h->d_ecol, ell_transpose(ecol_rm, W))) ||
+      (rc = bupload(h, pool, &h->d_eval, ell_transpose(eval_rm, W))) ||
+      (rc = bupload(h, pool, &h->d_h2, h2)) ||
+      (rc = bupload(h, pool, &h->d_P, P)) ||
+      (rc = bupload(h, pool, &h->d_mean_row, h->h_mean_row)))
+    return rc;
+  if ((rc = balloc(h, pool, &h->d_pfrag, pf.size() / 8))) return rc;
+  BCHK(h, hipMemcpy(h->d_pfrag, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  const size_t n = (size_t)h->n, es = h->mixed ? sizeof(float) : sizeof(double);
+  double **dv[] = {&h->d_x, &h->d_xopt, &h->d_b, &h->d_xio};
+  for (double **v : dv)
+    if ((rc = balloc(h, pool, v, n))) return rc;
+  void **kv[] = {&h->v_r, &h->v_rhat, &h->v_p, &h->v_nu, &h->v_z1, &h->v_z2, &h->v_t};
+  for (void **v : kv) {
+    char *p = nullptr;
+    if ((rc = balloc(h, pool, &p, n * es))) return rc;
+    *v = p;
+  }
+  if ((rc = balloc(h, pool, &h->d_st, (size_t)n_inst)) || (rc = balloc(h, pool, &h->d_all_done, 1)))
+    return rc;
+  h->h_st.assign(n_inst, Scal{});
+  int ib = h->cfg.reserved_i[1];
+  if (ib <= 0) {
+    auto it = g_tuned.find({(int)h->mixed, n_inst, h->nb});
+    ib = it != g_tuned.end() ? it->second : 8;
+  }
+  if ((rc = build_units(h, ib))) return rc;
+  h->ready = true;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_offsets(const cup2d_psb *h, long long *offsets) {
+  if (!h || !offsets) return CUP2D_PS_EINVAL;
+  if (!h->ready) return CUP2D_PS_ENOMESH;
+  for (int k = 0; k <= h->K; k++) offsets[k] = h->boff[k] * 64;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_solve_device(cup2d_psb *h, const double *d_b, double *d_x, double tol_abs,
+                           double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!d_b || !d_x) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+  if (!(tol_abs >= 0) || !(tol_rel >= 0) || max_restarts < 0)
+    return bfail(h, CUP2D_PS_EINVAL, "negative tolerance or restart count");
+  return solve_b(h, d_b, d_x, d_x, tol_abs, tol_rel, max_restarts, stats);
+}
+
+int cup2d_psb_solve(cup2d_psb *h, const double *b, double *x, double tol_abs, double tol_rel,
+                    int max_restarts, cup2d_ps_stats *stats) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!b || !x) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "solve before cup2d_psb_set_meshes");
+  if (!(tol_abs >= 0) || !(tol_rel >= 0) || max_restarts < 0)
+    return bfail(h, CUP2D_PS_EINVAL, "negative tolerance or restart count");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  const size_t bytes = (size_t)h->n * sizeof(double);
+  BCHK(h, hipMemcpyAsync(h->d_b, b, bytes, hipMemcpyHostToDevice, h->stream));
+  BCHK(h, hipMemcpyAsync(h->d_xio, x, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = solve_b(h, h->d_b, h->d_xio, h->d_xio, tol_abs, tol_rel, max_restarts, stats);
+  if (rc) return rc;
+  BCHK(h, hipMemcpyAsync(x, h->d_xio, bytes, hipMemcpyDeviceToHost, h->stream));
+  BCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!d_b) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "autotune before cup2d_psb_set_meshes");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  static const int cand[] = {4, 8, 16, 32, 64};
+  int best = h->ib;
+  double best_ms = 1e300;
+  for (int ib : cand) {
+    int rc = build_units(h, ib);
+    if (rc) return rc;
+    double ms = 0;
+    if ((rc = time_units(h, d_b, ms))) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = ib;
+    }
+  }
+  int rc = build_units(h, best);
+  if (rc) return rc;
+  g_tuned[{(int)h->mixed, h->K, h->nb}] = best;
+  if (item_blocks) *item_blocks = best;
+  if (ms_per_iter) *ms_per_iter = best_ms;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_item_blocks(const cup2d_psb *h) { return h ? h->ib : 0; }
+
+const char *cup2d_psb_last_error(const cup2d_psb *h) {
+  return h ? h->err.c_str() : g_berr.c_str();
+}
+
+}  // extern "C"

@@ -9,22 +9,27 @@ constexpr int WAVES = 4;            // wavefronts per workgroup
 constexpr int WG = 64 * WAVES;      // threads per workgroup
 constexpr int TILE = 16;            // blocks per wavefront in the MFMA preconditioner
 
-// Device view of the operator in stencil form (see assembly.h).  The
-// coefficients are FP64 in both precisions: FP32 coefficients would break the
-// operator's exact zero row sums (the Neumann null space) and make the
-// singular system inconsistent.
+// Device view of the operator.  Rows without a coarse-fine face are the
+// plain 5-point stencil sum_f (x_nb(f) - x_self) over the non-wall faces,
+// given by one same-level neighbour id per block face (nbr: -1 wall, -2
+// coarse-fine).  Rows with a coarse-fine face ("explicit rows", bit c of
+// mask[b]) are stored as ELL: row e has ell_w entries ecol/eval[w * n_exc +
+// e] (padded with zero values).  The coefficients are FP64 in both
+// precisions: FP32 coefficients would break the operator's exact zero row
+// sums (the Neumann null space) and make the singular system inconsistent.
 struct OpView {
   int nb;
   int mean_row;                          // -1 when the mean row is not replaced
-  const int4 *nbr;                       // [nb] same-level neighbour per face, -1 wall, -2 coarse-fine
-  const unsigned long long *mask;        // [nb] explicit-row bitmask
-  const int *bptr;                       // [nb+1] explicit rows per block
-  const int *cell;                       // [n_exc] cell of each explicit row
-  const int *rptr;                       // [n_exc+1]
-  const int *col;                        // [nnz_exc]
-  const double *val;                     // [nnz_exc]
+  const int4 *nbr;                       // [nb] same-level neighbour per face (Xm, Xp, Ym, Yp)
+  const unsigned long long *mask;        // [nb] explicit rows of each block
+  int n_exc, ell_w;
+  const int *erow;                       // [n_exc] global row of each explicit row
+  const int *ecol;                       // [ell_w * n_exc]
+  const double *eval;                    // [ell_w * n_exc]
   const double *h2;                      // [nb]
 };
+
+constexpr int ELL_MAX = 16;              // widest explicit row the kernels take
 
 // Solver scalars, resident on the device for the whole solve.  The solver
 // keeps two copies, indexed by iteration parity: iteration i reads and
@@ -124,45 +129,98 @@ __device__ void load_totals(const double *__restrict__ part, int G, double (&out
   }
 }
 
-// 5-point stencil row of cell c of block b: sum over non-wall faces of
-// (x_nb - x_c), in FP64 whatever the storage type TV of x.  xs is the wave's
-// LDS copy of x on block b.
+// Loads of the stencil row of cell c of block b (lane = cell): its own value
+// and the values across the block faces it touches (ix = 0 or 7: the
+// horizontal neighbour block, iy = 0 or 7: the vertical one; 0 across a wall
+// or coarse-fine face).  All loads are independent, so a wave has them in
+// flight together.
 template <class TV>
-__device__ __forceinline__ double stencil_row(const TV *__restrict__ x, int4 nb, int c,
-                                              const double *xs, double xc) {
+__device__ __forceinline__ void stencil_load(const TV *__restrict__ x, int b, int4 nb, int c,
+                                             double &xc, double &xh, double &xv) {
   const int ix = c & 7, iy = c >> 3;
+  const int hb = (ix == 0) ? nb.x : ((ix == 7) ? nb.y : -1);
+  const int vb = (iy == 0) ? nb.z : ((iy == 7) ? nb.w : -1);
+  const int hc = (ix == 0) ? iy * 8 + 7 : iy * 8;
+  const int vc = (iy == 0) ? 56 + ix : ix;
+  xc = (double)x[(long)b * 64 + c];
+  xh = (hb >= 0) ? (double)x[(long)hb * 64 + hc] : 0.0;
+  xv = (vb >= 0) ? (double)x[(long)vb * 64 + vc] : 0.0;
+}
+
+// 5-point stencil row of cell c (FP64): sum over the non-wall faces of
+// (x_nb - x_c), in the face order Ym, Xm, Xp, Yp.  Neighbours inside the
+// block come from the other lanes of the wave.  Meaningless for explicit
+// rows (their coarse-fine faces are skipped), which the caller overwrites.
+__device__ __forceinline__ double stencil_eval(int4 nb, int c, double xc, double xh, double xv) {
+  const int ix = c & 7, iy = c >> 3;
+  const double xl = __shfl(xc, (c - 1) & 63), xr = __shfl(xc, (c + 1) & 63);
+  const double xd = __shfl(xc, (c - 8) & 63), xu = __shfl(xc, (c + 8) & 63);
   double y = 0.0;
   int deg = 0;
-  if (iy > 0) { y += xs[c - 8]; deg++; } else if (nb.z >= 0) { y += (double)x[(long)nb.z * 64 + 56 + ix]; deg++; }
-  if (ix > 0) { y += xs[c - 1]; deg++; } else if (nb.x >= 0) { y += (double)x[(long)nb.x * 64 + iy * 8 + 7]; deg++; }
-  if (ix < 7) { y += xs[c + 1]; deg++; } else if (nb.y >= 0) { y += (double)x[(long)nb.y * 64 + iy * 8]; deg++; }
-  if (iy < 7) { y += xs[c + 8]; deg++; } else if (nb.w >= 0) { y += (double)x[(long)nb.w * 64 + ix]; deg++; }
+  if (iy > 0) { y += xd; deg++; } else if (nb.z >= 0) { y += xv; deg++; }
+  if (ix > 0) { y += xl; deg++; } else if (nb.x >= 0) { y += xh; deg++; }
+  if (ix < 7) { y += xr; deg++; } else if (nb.y >= 0) { y += xh; deg++; }
+  if (iy < 7) { y += xu; deg++; } else if (nb.w >= 0) { y += xv; deg++; }
   return y - double(deg) * xc;
 }
 
-// y = A x on block b, one cell per lane, FP64 arithmetic.  xs: wave LDS
-// scratch of 64 doubles holding x on the block, ys: a second scratch.  The
-// mean row, when replaced, is left to the caller.
+// Explicit row e (one thread): sum_w val * x[col] in FP64, entries in
+// ascending column order; loads issued eight entries at a time.
 template <class TV>
-__device__ __forceinline__ double apply_block(const OpView &A, const TV *__restrict__ x, int b, int c,
-                                              double *xs, double *ys, double xc) {
-  const int4 nb = A.nbr[b];
-  double y = stencil_row<TV>(x, nb, c, xs, xc);
-  const unsigned long long m = A.mask[b];
-  if (m) {
-    const int e0 = A.bptr[b], e1 = A.bptr[b + 1];
-    for (int e = e0 + c; e < e1; e += 64) {
-      double acc = 0.0;
-      const int k1 = A.rptr[e + 1];
-      for (int k = A.rptr[e]; k < k1; k++) acc += A.val[k] * (double)x[A.col[k]];
-      ys[A.cell[e]] = acc;
+__device__ __forceinline__ double explicit_row(const OpView &A, const TV *__restrict__ x, int e) {
+  double acc = 0.0;
+  for (int w0 = 0; w0 < A.ell_w; w0 += 8) {
+    int col[8];
+    double val[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      const bool ok = w0 + w < A.ell_w;
+      const long k = (long)(ok ? w0 + w : 0) * A.n_exc + e;
+      col[w] = A.ecol[k];
+      val[w] = ok ? A.eval[k] : 0.0;
     }
-    wave_sync();
-    if ((m >> c) & 1ull) y = ys[c];
-    wave_sync();
+    double xv[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) xv[w] = (double)x[col[w]];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      if (w0 + w < A.ell_w) acc += val[w] * xv[w];
   }
-  return y;
+  return acc;
 }
+
+// The wave's U blocks b[u] (lane = cell): headers, then every load of the
+// U blocks at once -- the stencil loads, h^2 of the block and aux[i] (a
+// second vector the caller combines with the result; may be null).
+template <int U, class TV>
+struct BlockRows {
+  long i[U];
+  int4 nb[U];
+  unsigned long long m[U];
+  double xc[U], xh[U], xv[U], a[U], hh[U];
+
+  template <class TA>
+  __device__ __forceinline__ void load(const OpView &A, const TV *__restrict__ x,
+                                       const TA *__restrict__ aux, const int (&b)[U], int c) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      nb[u] = A.nbr[b[u]];
+      m[u] = A.mask[b[u]];
+      i[u] = (long)b[u] * 64 + c;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      stencil_load<TV>(x, b[u], nb[u], c, xc[u], xh[u], xv[u]);
+      a[u] = aux ? (double)aux[i[u]] : 0.0;
+      hh[u] = A.h2[b[u]];
+    }
+  }
+  // stencil value of block u; `plain` is false on explicit rows
+  __device__ __forceinline__ double y(int u, int c, bool &plain) const {
+    plain = !((m[u] >> c) & 1ull);
+    return stencil_eval(nb[u], c, xc[u], xh[u], xv[u]);
+  }
+};
 
 // z = P v for NB blocks at once (FP64): lane c owns output row c; P (64x64,
 // symmetric) sits in LDS, v holds the NB input vectors (64 each).
@@ -352,6 +410,43 @@ __device__ __forceinline__ void st16_out(float *__restrict__ a, long base, int q
   for (int mt = 0; mt < 4; mt++)
     *reinterpret_cast<float4 *>(a + base + 16 * mt + 4 * q) =
         make_float4(v[mt * 4 + 0], v[mt * 4 + 1], v[mt * 4 + 2], v[mt * 4 + 3]);
+}
+
+// Top-of-iteration update of the lane's 16 input cells of one block, FP32
+// storage with FP64 arithmetic: r -= mu (written back when mu != 0), then
+// p = r + beta (p - omega nu), or rhat = p = r after a restart.  Every load
+// is issued first.  pin receives p; a lane with ok false (past an
+// instance's last block; base is any valid block) stores nothing and gets
+// pin = 0.
+__device__ __forceinline__ void mf_pupdate(float *__restrict__ r, float *__restrict__ rhat,
+                                           float *__restrict__ p, const float *__restrict__ nu,
+                                           long base, int q, bool ok, double beta, double omega,
+                                           double mu, int restart, float (&pin)[16]) {
+  float rv[16], pv[16] = {}, nv[16] = {};
+  ld16(r, base, q, rv);
+  if (!restart) {
+    ld16(p, base, q, pv);
+    ld16(nu, base, q, nv);
+  }
+  if (mu != 0.0)
+#pragma unroll
+    for (int k = 0; k < 16; k++) rv[k] = (float)((double)rv[k] - mu);
+  if (restart) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) pin[k] = rv[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      pin[k] = (float)((double)rv[k] + beta * ((double)pv[k] - omega * (double)nv[k]));
+  }
+  if (ok) {
+    if (mu != 0.0) st16_in(r, base, q, rv);
+    if (restart) st16_in(rhat, base, q, rv);
+    st16_in(p, base, q, pin);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++) pin[k] = 0.f;
+  }
 }
 
 }  // namespace cup2d

@@ -40,7 +40,9 @@ using namespace cup2d;
 
 namespace {
 
-constexpr int MAXG = 512;  // workgroups of the vector kernels (grid-stride beyond)
+constexpr int MAXG = 512;  // workgroups of the block part of the vector kernels (grid-stride beyond)
+constexpr int MAXGE = 64;  // workgroups of the explicit-row part
+constexpr int SPMV_U = 2;  // blocks per wavefront in flight in the operator kernels
 
 // ----------------------------------------------------------------------------
 // scalar recurrence
@@ -74,40 +76,60 @@ __global__ __launch_bounds__(WG) void k_hsum(int nb, const T *__restrict__ x,
 
 // r = b - A x0 in FP64, stored as TR to r and rhat; p = nu = 0; x = x_opt =
 // x0 (x may alias x0) (BiCGSTAB.cu:431-465); partials of the update totals
-// of the stored r.
+// of the stored r.  Workgroups below Gb take the blocks (U per wave), the
+// others the explicit rows.
 template <class TR>
-__global__ __launch_bounds__(WG) void k_init(OpView A, const double *__restrict__ b,
+__global__ __launch_bounds__(WG) void k_init(OpView A, int Gb, const double *__restrict__ b,
                                              const double *x0, double *x,
                                              double *__restrict__ xopt, TR *__restrict__ r,
                                              TR *__restrict__ rhat, TR *__restrict__ p,
                                              TR *__restrict__ nu, const double *pm, int Gm,
                                              double *p5) {
-  __shared__ double xs[WAVES][64], ys[WAVES][64];
   double m0[1] = {0.0};
   if (A.mean_row >= 0) load_totals<1, 0u>(pm, Gm, m0);
-  const int c = lane_id(), w = wave_id();
-  const int nw = gridDim.x * WAVES;
   double v[NQ5] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
-    const long i = (long)bb * 64 + c;
-    const double xc = x0[i];
-    xs[w][c] = xc;
-    wave_sync();
-    const double y = apply_block<double>(A, x0, bb, c, xs[w], ys[w], xc);
-    wave_sync();
-    // mean row: b = 0 (ExpAMRSolver.cpp:340-345) and (A x)_row = sum h^2 x
-    const double ri = (i == A.mean_row) ? 0.0 - m0[0] : b[i] - y;
+  // mean row: b = 0 (ExpAMRSolver.cpp:340-345) and (A x)_row = sum h^2 x
+  auto store_r = [&](long i, double y, double bi) {
+    const double ri = (i == A.mean_row) ? 0.0 - m0[0] : bi - y;
     const TR rt = (TR)ri;
     r[i] = rt;
     rhat[i] = rt;
-    p[i] = TR(0);
-    nu[i] = TR(0);
-    x[i] = xc;
-    xopt[i] = xc;
     const double rd = (double)rt;
     v[0] = nmax(v[0], fabs(rd));
     v[1] += rd * rd;
     v[4] += rd;
+  };
+  if ((int)blockIdx.x < Gb) {
+    const int c = lane_id();
+    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
+      int bl[SPMV_U];
+      bool ok[SPMV_U];
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        ok[u] = g * SPMV_U + u < A.nb;
+        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
+      }
+      BlockRows<SPMV_U, double> R;
+      R.load(A, x0, b, bl, c);
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        if (!ok[u]) continue;
+        bool plain;
+        const double y = R.y(u, c, plain);
+        const long i = R.i[u];
+        p[i] = TR(0);
+        nu[i] = TR(0);
+        x[i] = R.xc[u];
+        xopt[i] = R.xc[u];
+        if (plain) store_r(i, y, R.a[u]);
+      }
+    }
+  } else {
+    const int ge = gridDim.x - Gb;
+    for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
+      const int row = A.erow[e];
+      store_r(row, explicit_row<double>(A, x0, e), b[row]);
+    }
   }
   v[2] = v[1];
   v[3] = v[1];
@@ -120,54 +142,58 @@ __global__ __launch_bounds__(WG) void k_init(OpView A, const double *__restrict_
 // ----------------------------------------------------------------------------
 
 // Top of iteration: p = r + beta (p - omega nu) (or p = rhat = r after a
-// restart), x_opt = x when the last iteration improved, z1 = P p
-// (BiCGSTAB.cu:471-520).
+// restart), z1 = P p (BiCGSTAB.cu:471-520).  The x_opt copy of the
+// iteration's state is made by k_xrupdate, which reads x anyway.
 template <int NB>
 __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict__ P,
                                                 const double *__restrict__ r,
                                                 double *__restrict__ rhat, double *__restrict__ p,
                                                 const double *__restrict__ nu,
-                                                const double *__restrict__ x,
-                                                double *__restrict__ xopt, double *__restrict__ z1,
-                                                Scal *st, int par, const double *p5, int G5) {
+                                                double *__restrict__ z1, Scal *st, int par,
+                                                const double *p5, int G5) {
   __shared__ Scal cur;
   __shared__ double Ps[64 * 64];
   __shared__ double vs[WAVES][NB * 64];
   top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega;
-  const int restart = cur.restart, improved = cur.improved;
+  const int restart = cur.restart;
   load_P(Ps, P);
   const int c = lane_id(), w = wave_id();
   const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
   for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
+    // every load of the group first
+    long ii[NB];
+    bool ok[NB];
+    double rv[NB], pv[NB] = {}, nv[NB] = {};
 #pragma unroll
     for (int j = 0; j < NB; j++) {
-      const int bb = g * NB + j;
-      double pi = 0.0;
-      if (bb < nb) {
-        const long i = (long)bb * 64 + c;
-        const double ri = r[i];
-        if (restart) {
-          rhat[i] = ri;
-          pi = ri;
-        } else {
-          pi = ri + beta * (p[i] - omega * nu[i]);
-        }
-        p[i] = pi;
-        if (improved) xopt[i] = x[i];
+      ok[j] = g * NB + j < nb;
+      ii[j] = (long)(ok[j] ? g * NB + j : g * NB) * 64 + c;
+      rv[j] = r[ii[j]];
+    }
+    if (!restart)
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        pv[j] = p[ii[j]];
+        nv[j] = nu[ii[j]];
       }
-      vs[w][j * 64 + c] = pi;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const double pi = restart ? rv[j] : rv[j] + beta * (pv[j] - omega * nv[j]);
+      if (ok[j]) {
+        if (restart) rhat[ii[j]] = rv[j];
+        p[ii[j]] = pi;
+      }
+      vs[w][j * 64 + c] = ok[j] ? pi : 0.0;
     }
     wave_sync();
     double z[NB];
     prec_group<NB>(Ps, vs[w], c, z);
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const int bb = g * NB + j;
-      if (bb < nb) z1[(long)bb * 64 + c] = z[j];
-    }
+    for (int j = 0; j < NB; j++)
+      if (ok[j]) z1[ii[j]] = z[j];
   }
 }
 
@@ -195,31 +221,33 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
   const int c = lane_id(), w = wave_id();
   const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
   for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
+    long ii[NB];
+    bool ok[NB];
+    double rv[NB], nv[NB];
 #pragma unroll
     for (int j = 0; j < NB; j++) {
-      const int bb = g * NB + j;
-      double si = 0.0;
-      if (bb < nb) {
-        const long i = (long)bb * 64 + c;
-        double ni = nu[i];
-        if (i == mean_row) {
-          ni = o[1];
-          nu[i] = ni;
-        }
-        si = r[i] - alpha * ni;
-        r[i] = si;
+      ok[j] = g * NB + j < nb;
+      ii[j] = (long)(ok[j] ? g * NB + j : g * NB) * 64 + c;
+      rv[j] = r[ii[j]];
+      nv[j] = nu[ii[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if (ii[j] == mean_row) nv[j] = o[1];
+      const double si = rv[j] - alpha * nv[j];
+      if (ok[j]) {
+        if (ii[j] == mean_row) nu[ii[j]] = nv[j];
+        r[ii[j]] = si;
       }
-      vs[w][j * 64 + c] = si;
+      vs[w][j * 64 + c] = ok[j] ? si : 0.0;
     }
     wave_sync();
     double z[NB];
     prec_group<NB>(Ps, vs[w], c, z);
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const int bb = g * NB + j;
-      if (bb < nb) z2[(long)bb * 64 + c] = z[j];
-    }
+    for (int j = 0; j < NB; j++)
+      if (ok[j]) z2[ii[j]] = z[j];
   }
 }
 
@@ -228,58 +256,28 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
 // ----------------------------------------------------------------------------
 
 // Top of iteration (FP32 storage): r -= mu (null-space projection),
-// p = r + beta (p - omega nu) or p = rhat = r after a restart, x_opt = x
-// when the last iteration improved, z1 = P p on MFMA   (BiCGSTAB.cu:471-520)
+// p = r + beta (p - omega nu) or p = rhat = r after a restart, z1 = P p on
+// MFMA   (BiCGSTAB.cu:471-520)
 __global__ __launch_bounds__(WG) void k_pupdate_mf(int nb, const bf16x8 *__restrict__ pf,
                                                    float *__restrict__ r,
                                                    float *__restrict__ rhat, float *__restrict__ p,
                                                    const float *__restrict__ nu,
-                                                   const double *__restrict__ x,
-                                                   double *__restrict__ xopt, float *__restrict__ z1,
-                                                   Scal *st, int par, const double *p5, int G5) {
+                                                   float *__restrict__ z1, Scal *st, int par,
+                                                   const double *p5, int G5) {
   __shared__ Scal cur;
   top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega, mu = cur.mu;
-  const int restart = cur.restart, improved = cur.improved;
+  const int restart = cur.restart;
   const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
   PFrag P;
   load_pfrag(P, pf, lane);
   const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
   for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
-    const int bb = tile * TILE + j;
-    const bool ok = bb < nb;
-    const long base = (long)bb * 64;
+    const bool ok = tile * TILE + j < nb;
+    const long base = (long)(ok ? tile * TILE + j : tile * TILE) * 64;
     float pin[16];
-    if (ok) {
-      float rv[16];
-      ld16(r, base, q, rv);
-      if (mu != 0.0) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) rv[k] = (float)((double)rv[k] - mu);
-        st16_in(r, base, q, rv);
-      }
-      if (restart) {
-        st16_in(rhat, base, q, rv);
-#pragma unroll
-        for (int k = 0; k < 16; k++) pin[k] = rv[k];
-      } else {
-        float pv[16], nv[16];
-        ld16(p, base, q, pv);
-        ld16(nu, base, q, nv);
-#pragma unroll
-        for (int k = 0; k < 16; k++)
-          pin[k] = (float)((double)rv[k] + beta * ((double)pv[k] - omega * (double)nv[k]));
-      }
-      st16_in(p, base, q, pin);
-      if (improved) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) xopt[base + in_cell(q, k)] = x[base + in_cell(q, k)];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 16; k++) pin[k] = 0.f;
-    }
+    mf_pupdate(r, rhat, p, nu, base, q, ok, beta, omega, mu, restart, pin);
     float z[16];
     prec_mfma(P, pin, z);
     if (ok) st16_out(z1, base, q, z);
@@ -306,29 +304,22 @@ __global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restr
   load_pfrag(P, pf, lane);
   const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
   for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
-    const int bb = tile * TILE + j;
-    const bool ok = bb < nb;
-    const long base = (long)bb * 64;
-    float sv[16];
-    if (ok) {
-      float rv[16], nv[16];
-      ld16(r, base, q, rv);
-      ld16(nu, base, q, nv);
-      if (mean_row >= base && mean_row < base + 64) {
+    const bool ok = tile * TILE + j < nb;
+    const long base = (long)(ok ? tile * TILE + j : tile * TILE) * 64;
+    float rv[16], nv[16], sv[16];
+    ld16(r, base, q, rv);
+    ld16(nu, base, q, nv);
+    if (ok && mean_row >= base && mean_row < base + 64) {
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-          if (base + in_cell(q, k) == mean_row) {
-            nv[k] = (float)o[1];
-            nu[mean_row] = nv[k];
-          }
-      }
-#pragma unroll
-      for (int k = 0; k < 16; k++) sv[k] = (float)((double)rv[k] - alpha * (double)nv[k]);
-      st16_in(r, base, q, sv);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 16; k++) sv[k] = 0.f;
+      for (int k = 0; k < 16; k++)
+        if (base + in_cell(q, k) == mean_row) {
+          nv[k] = (float)o[1];
+          nu[mean_row] = nv[k];
+        }
     }
+#pragma unroll
+    for (int k = 0; k < 16; k++) sv[k] = ok ? (float)((double)rv[k] - alpha * (double)nv[k]) : 0.f;
+    if (ok) st16_in(r, base, q, sv);
     float z[16];
     prec_mfma(P, sv, z);
     if (ok) st16_out(z2, base, q, z);
@@ -349,62 +340,100 @@ __global__ __launch_bounds__(WG) void k_eval(Scal *st, int par, const double *p5
 
 // nu = A z1; partials of (rhat, nu) and sum h^2 z1   (BiCGSTAB.cu:523-527)
 template <class T>
-__global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, const T *__restrict__ z1,
+__global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__restrict__ z1,
                                                 T *__restrict__ nu, const T *__restrict__ rhat,
                                                 const Scal *st, int par, double *p2) {
   if (st[par].done) return;
-  __shared__ double xs[WAVES][64], ys[WAVES][64];
-  const int c = lane_id(), w = wave_id();
-  const int nw = gridDim.x * WAVES;
   double v[2] = {0.0, 0.0};
-  for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
-    const long i = (long)bb * 64 + c;
-    const double zc = (double)z1[i];
-    xs[w][c] = zc;
-    wave_sync();
-    double y = apply_block<T>(A, z1, bb, c, xs[w], ys[w], zc);
-    wave_sync();
+  auto store_y = [&](long i, double y, double rh) {
     if (i == A.mean_row) y = 0.0;  // replaced by the consumer
     const T yt = (T)y;
     nu[i] = yt;
-    v[0] += (double)rhat[i] * (double)yt;
-    v[1] += A.h2[bb] * zc;
+    v[0] += rh * (double)yt;
+  };
+  if ((int)blockIdx.x < Gb) {
+    const int c = lane_id();
+    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
+      int bl[SPMV_U];
+      bool ok[SPMV_U];
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        ok[u] = g * SPMV_U + u < A.nb;
+        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
+      }
+      BlockRows<SPMV_U, T> R;
+      R.load(A, z1, rhat, bl, c);
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        if (!ok[u]) continue;
+        bool plain;
+        const double y = R.y(u, c, plain);
+        if (plain) store_y(R.i[u], y, R.a[u]);
+        v[1] += R.hh[u] * R.xc[u];
+      }
+    }
+  } else {
+    const int ge = gridDim.x - Gb;
+    for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
+      const int row = A.erow[e];
+      store_y(row, explicit_row<T>(A, z1, e), (double)rhat[row]);
+    }
   }
   store_partials<2, 0u>(v, p2);
 }
 
 // t = A z2; partials of (t, s), (t, t) and sum h^2 z2   (BiCGSTAB.cu:548-553)
 template <class T>
-__global__ __launch_bounds__(WG) void k_spmv_t(OpView A, const T *__restrict__ z2,
+__global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__restrict__ z2,
                                                T *__restrict__ t, const T *__restrict__ sv,
                                                const Scal *st, int par, double *p4) {
   if (st[par].done) return;
-  __shared__ double xs[WAVES][64], ys[WAVES][64];
-  const int c = lane_id(), w = wave_id();
-  const int nw = gridDim.x * WAVES;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
-    const long i = (long)bb * 64 + c;
-    const double zc = (double)z2[i];
-    xs[w][c] = zc;
-    wave_sync();
-    double y = apply_block<T>(A, z2, bb, c, xs[w], ys[w], zc);
-    wave_sync();
+  auto store_y = [&](long i, double y, double si) {
     if (i == A.mean_row) y = 0.0;
     const T yt = (T)y;
     t[i] = yt;
     const double yd = (double)yt;
-    v[0] += yd * (double)sv[i];
+    v[0] += yd * si;
     v[1] += yd * yd;
-    v[2] += A.h2[bb] * zc;
+  };
+  if ((int)blockIdx.x < Gb) {
+    const int c = lane_id();
+    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
+      int bl[SPMV_U];
+      bool ok[SPMV_U];
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        ok[u] = g * SPMV_U + u < A.nb;
+        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
+      }
+      BlockRows<SPMV_U, T> R;
+      R.load(A, z2, sv, bl, c);
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        if (!ok[u]) continue;
+        bool plain;
+        const double y = R.y(u, c, plain);
+        if (plain) store_y(R.i[u], y, R.a[u]);
+        v[2] += R.hh[u] * R.xc[u];
+      }
+    }
+  } else {
+    const int ge = gridDim.x - Gb;
+    for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
+      const int row = A.erow[e];
+      store_y(row, explicit_row<T>(A, z2, e), (double)sv[row]);
+    }
   }
   store_partials<3, 0u>(v, p4);
 }
 
-// omega = (t, s) / ((t, t) + eps); x += alpha z1 + omega z2 (x in FP64);
-// r = s - omega t; partials of the update totals   (BiCGSTAB.cu:552-572)
+// omega = (t, s) / ((t, t) + eps); x_opt = x when the previous iterate was
+// the best so far; x += alpha z1 + omega z2 (x in FP64); r = s - omega t;
+// partials of the update totals   (BiCGSTAB.cu:501-503, 552-572)
 template <class T>
 __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
+                                                 double *__restrict__ xopt,
                                                  const T *__restrict__ z1,
                                                  const T *__restrict__ z2, T *__restrict__ r,
                                                  const T *__restrict__ t,
@@ -421,6 +450,7 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
   }
   const double omega = ts / (tt + st[par].eps);
   const double alpha = st[par].alpha;
+  const int improved = st[par].improved;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[par].omega = omega;
     st[par].red[3] = ts;
@@ -429,7 +459,9 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
   const long n = (long)nb * 64;
   double v[NQ5] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
-    x[i] = x[i] + alpha * (double)z1[i] + omega * (double)z2[i];
+    const double xo = x[i];
+    if (improved) xopt[i] = xo;
+    x[i] = xo + alpha * (double)z1[i] + omega * (double)z2[i];
     const double ti = (i == mean_row) ? o[2] : (double)t[i];
     const T rt = (T)((double)r[i] - omega * ti);
     r[i] = rt;
@@ -493,23 +525,36 @@ __global__ __launch_bounds__(WG) void k_final_write(int nb, const double *__rest
 // operator pieces exposed by the C ABI
 // ----------------------------------------------------------------------------
 
-// y = A x (stencil form); partials of sum h^2 x for the mean row
+// y = A x (stencil + explicit rows); partials of sum h^2 x for the mean row
 template <class T>
-__global__ __launch_bounds__(WG) void k_spmv(OpView A, const T *__restrict__ x,
+__global__ __launch_bounds__(WG) void k_spmv(OpView A, int Gb, const T *__restrict__ x,
                                              T *__restrict__ y, double *pm) {
-  __shared__ double xs[WAVES][64], ys[WAVES][64];
-  const int c = lane_id(), w = wave_id();
-  const int nw = gridDim.x * WAVES;
   double v[1] = {0.0};
-  for (int bb = blockIdx.x * WAVES + w; bb < A.nb; bb += nw) {
-    const long i = (long)bb * 64 + c;
-    const double xc = (double)x[i];
-    xs[w][c] = xc;
-    wave_sync();
-    const double yi = apply_block<T>(A, x, bb, c, xs[w], ys[w], xc);
-    wave_sync();
-    y[i] = (T)yi;
-    v[0] += A.h2[bb] * xc;
+  if ((int)blockIdx.x < Gb) {
+    const int c = lane_id();
+    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
+      int bl[SPMV_U];
+      bool ok[SPMV_U];
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        ok[u] = g * SPMV_U + u < A.nb;
+        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
+      }
+      BlockRows<SPMV_U, T> R;
+      R.load(A, x, (const T *)nullptr, bl, c);
+#pragma unroll
+      for (int u = 0; u < SPMV_U; u++) {
+        if (!ok[u]) continue;
+        bool plain;
+        const double yi = R.y(u, c, plain);
+        if (plain) y[R.i[u]] = (T)yi;
+        v[0] += R.hh[u] * R.xc[u];
+      }
+    }
+  } else {
+    const int ge = gridDim.x - Gb;
+    for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG)
+      y[A.erow[e]] = (T)explicit_row<T>(A, x, e);
   }
   if (A.mean_row >= 0) store_partials<1, 0u>(v, pm);
 }
@@ -626,8 +671,9 @@ struct cup2d_ps {
   // device operator
   int4 *d_nbr = nullptr;
   unsigned long long *d_mask = nullptr;
-  int *d_bptr = nullptr, *d_cell = nullptr, *d_rptr = nullptr, *d_col = nullptr;
-  double *d_val = nullptr, *d_h2 = nullptr;
+  int *d_erow = nullptr, *d_ecol = nullptr;
+  double *d_eval = nullptr, *d_h2 = nullptr;
+  int ell_w = 0;
   int *d_csr_rp = nullptr, *d_csr_col = nullptr;
   double *d_csr_val = nullptr;
   double *d_P = nullptr;
@@ -643,6 +689,7 @@ struct cup2d_ps {
   double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr, *d_pfin = nullptr;
   Scal *h_s = nullptr;                   // pinned copy of the state slots
   int G = 1;                             // workgroups of the SpMV / vector kernels
+  int Gb = 1;                            // of which the block part of the operator kernels
   int Gp = 1;                            // workgroups of the FP64 preconditioner kernels
   int Gm = 1;                            // workgroups of the MFMA preconditioner kernels
   int nbp = 2;                           // blocks per wave in the FP64 preconditioner kernels
@@ -701,11 +748,11 @@ OpView view(const cup2d_ps *h) {
   A.mean_row = (int)h->op.mean_row;
   A.nbr = h->d_nbr;
   A.mask = h->d_mask;
-  A.bptr = h->d_bptr;
-  A.cell = h->d_cell;
-  A.rptr = h->d_rptr;
-  A.col = h->d_col;
-  A.val = h->d_val;
+  A.n_exc = (int)h->op.n_exc();
+  A.ell_w = h->ell_w;
+  A.erow = h->d_erow;
+  A.ecol = h->d_ecol;
+  A.eval = h->d_eval;
   A.h2 = h->d_h2;
   return A;
 }
@@ -716,13 +763,13 @@ void launch_iter_fp64(cup2d_ps *h, int par) {
   const int G = h->G, Gp = h->Gp, nb = (int)h->op.nb;
   const long mr = h->op.mean_row;
   hipStream_t st = h->stream;
-  k_pupdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_x,
-                                   h->d_xopt, h->d_z1, h->d_s, par, h->d_p5, G);
-  k_spmv_nu<double><<<G, WG, 0, st>>>(A, h->d_z1, h->d_nu, h->d_rhat, h->d_s, par, h->d_p2);
+  k_pupdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1,
+                                   h->d_s, par, h->d_p5, G);
+  k_spmv_nu<double><<<G, WG, 0, st>>>(A, h->Gb, h->d_z1, h->d_nu, h->d_rhat, h->d_s, par, h->d_p2);
   k_supdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
                                    h->d_p2, G, mr);
-  k_spmv_t<double><<<G, WG, 0, st>>>(A, h->d_z2, h->d_t, h->d_r, h->d_s, par, h->d_p4);
-  k_xrupdate<double><<<G, WG, 0, st>>>(nb, h->d_x, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat,
+  k_spmv_t<double><<<G, WG, 0, st>>>(A, h->Gb, h->d_z2, h->d_t, h->d_r, h->d_s, par, h->d_p4);
+  k_xrupdate<double><<<G, WG, 0, st>>>(nb, h->d_x, h->d_xopt, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat,
                                        h->d_s, par, h->d_p4, G, h->d_p5, mr);
 }
 
@@ -731,13 +778,13 @@ void launch_iter_mixed(cup2d_ps *h, int par) {
   const int G = h->G, Gm = h->Gm, nb = (int)h->op.nb;
   const long mr = h->op.mean_row;
   hipStream_t st = h->stream;
-  k_pupdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_rhat, h->f_p, h->f_nu, h->d_x,
-                                  h->d_xopt, h->f_z1, h->d_s, par, h->d_p5, G);
-  k_spmv_nu<float><<<G, WG, 0, st>>>(A, h->f_z1, h->f_nu, h->f_rhat, h->d_s, par, h->d_p2);
+  k_pupdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_rhat, h->f_p, h->f_nu, h->f_z1,
+                                  h->d_s, par, h->d_p5, G);
+  k_spmv_nu<float><<<G, WG, 0, st>>>(A, h->Gb, h->f_z1, h->f_nu, h->f_rhat, h->d_s, par, h->d_p2);
   k_supdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_nu, h->f_rhat, h->f_z2, h->d_s, par,
                                   h->d_p2, G, mr);
-  k_spmv_t<float><<<G, WG, 0, st>>>(A, h->f_z2, h->f_t, h->f_r, h->d_s, par, h->d_p4);
-  k_xrupdate<float><<<G, WG, 0, st>>>(nb, h->d_x, h->f_z1, h->f_z2, h->f_r, h->f_t, h->f_rhat,
+  k_spmv_t<float><<<G, WG, 0, st>>>(A, h->Gb, h->f_z2, h->f_t, h->f_r, h->d_s, par, h->d_p4);
+  k_xrupdate<float><<<G, WG, 0, st>>>(nb, h->d_x, h->d_xopt, h->f_z1, h->f_z2, h->f_r, h->f_t, h->f_rhat,
                                       h->d_s, par, h->d_p4, G, h->d_p5, mr);
 }
 
@@ -842,7 +889,7 @@ int solve_fp64(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
   if (h->op.mean_row >= 0) k_hsum<double><<<G, WG, 0, h->stream>>>(nb, d_x0, h->d_h2, h->d_pm);
-  k_init<double><<<G, WG, 0, h->stream>>>(view(h), d_b, d_x0, h->d_x, h->d_xopt, h->d_r,
+  k_init<double><<<G, WG, 0, h->stream>>>(view(h), h->Gb, d_b, d_x0, h->d_x, h->d_xopt, h->d_r,
                                           h->d_rhat, h->d_p, h->d_nu, h->d_pm, G, h->d_p5);
   HIPCHK(h, hipGetLastError());
   int rc = run_iterations(h, h->cfg.max_iter);
@@ -902,7 +949,7 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
   const int max_outer = 16;
   for (;;) {
     if (h->op.mean_row >= 0) k_hsum<double><<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_h2, h->d_pm);
-    k_init<float><<<G, WG, 0, h->stream>>>(A, d_b, h->d_x, h->d_x, h->d_xopt, h->f_r, h->f_rhat,
+    k_init<float><<<G, WG, 0, h->stream>>>(A, h->Gb, d_b, h->d_x, h->d_x, h->d_xopt, h->f_r, h->f_rhat,
                                            h->f_p, h->f_nu, h->d_pm, G, h->d_p5);
     HIPCHK(h, hipGetLastError());
     int rc = outer_residual(h, err);
@@ -1033,12 +1080,18 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
   for (long long b = 0; b < op.nb; b++)
     nbr[b] = make_int4(op.nbr[4 * b], op.nbr[4 * b + 1], op.nbr[4 * b + 2], op.nbr[4 * b + 3]);
   std::vector<unsigned long long> mask(op.exc_mask.begin(), op.exc_mask.end());
+  h->ell_w = ell_width(op);
+  if (h->ell_w > ELL_MAX)
+    return fail(h, CUP2D_PS_EINVAL, "explicit row with more than " + std::to_string(ELL_MAX) + " entries");
+  std::vector<int> erow, ecol_rm;
+  std::vector<double> eval_rm;
+  ell_append(op, h->ell_w, 0, erow, ecol_rm, eval_rm);
   const std::vector<double> P = make_pinv();
   const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = upload(h, &h->d_nbr, nbr)) || (rc = upload(h, &h->d_mask, mask)) ||
-      (rc = upload(h, &h->d_bptr, op.exc_bptr)) || (rc = upload(h, &h->d_cell, op.exc_cell)) ||
-      (rc = upload(h, &h->d_rptr, op.exc_rptr)) || (rc = upload(h, &h->d_col, op.exc_col)) ||
-      (rc = upload(h, &h->d_val, op.exc_val)) ||
+      (rc = upload(h, &h->d_erow, erow)) ||
+      (rc = upload(h, &h->d_ecol, ell_transpose(ecol_rm, h->ell_w))) ||
+      (rc = upload(h, &h->d_eval, ell_transpose(eval_rm, h->ell_w))) ||
       (rc = upload(h, &h->d_h2, op.h2)) || (rc = upload(h, &h->d_csr_rp, op.rp)) ||
       (rc = upload(h, &h->d_csr_col, op.col)) || (rc = upload(h, &h->d_csr_val, op.val)) ||
       (rc = upload(h, &h->d_P, P)))
@@ -1053,12 +1106,16 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
   float **fvecs[] = {&h->f_r, &h->f_rhat, &h->f_p, &h->f_nu, &h->f_z1, &h->f_z2, &h->f_t};
   for (float **v : fvecs)
     if ((rc = dalloc(h, v, h->mixed ? n : 64))) return rc;
-  if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_p2, (size_t)2 * MAXG)) ||
-      (rc = dalloc(h, &h->d_p4, (size_t)3 * MAXG)) || (rc = dalloc(h, &h->d_p5, (size_t)NQ5 * MAXG)) ||
-      (rc = dalloc(h, &h->d_pm, (size_t)MAXG)) || (rc = dalloc(h, &h->d_pfin, (size_t)4 * MAXG)))
+  const size_t GT = MAXG + MAXGE;
+  if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_p2, 2 * GT)) ||
+      (rc = dalloc(h, &h->d_p4, 3 * GT)) || (rc = dalloc(h, &h->d_p5, NQ5 * GT)) ||
+      (rc = dalloc(h, &h->d_pm, GT)) || (rc = dalloc(h, &h->d_pfin, 4 * GT)))
     return rc;
   HIPCHK(h, hipMemset(h->d_s, 0, 2 * sizeof(Scal)));
-  h->G = (int)std::max<long long>(1, std::min<long long>(MAXG, (op.nb + WAVES - 1) / WAVES));
+  const long long waves = (op.nb + SPMV_U - 1) / SPMV_U;
+  h->Gb = (int)std::max<long long>(1, std::min<long long>(MAXG, (waves + WAVES - 1) / WAVES));
+  const int Ge = op.n_exc() > 0 ? (int)std::min<long long>(MAXGE, (op.n_exc() + WG - 1) / WG) : 0;
+  h->G = h->Gb + Ge;
   h->nbp = h->cfg.reserved_i[0] > 0 ? h->cfg.reserved_i[0] : (op.nb >= 8192 ? 4 : 2);
   if (h->nbp != 1 && h->nbp != 2 && h->nbp != 4) h->nbp = 2;
   const long long groups = (op.nb + h->nbp - 1) / h->nbp;
@@ -1099,7 +1156,7 @@ int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind) 
   HIPCHK(h, hipSetDevice(h->device));
   const long n = (long)h->op.n;
   if (kind == 0) {
-    k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), d_x, d_y, h->d_pm);
+    k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), h->Gb, d_x, d_y, h->d_pm);
   } else {
     const int G = (int)std::min<long>(4096, (n + WG - 1) / WG);
     k_spmv_csr<<<G, WG, 0, h->stream>>>(n, h->d_csr_rp, h->d_csr_col, h->d_csr_val, d_x, d_y);
@@ -1124,7 +1181,7 @@ int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, in
   const OpView A = view(h);
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int k = 0; k < reps; k++) {
-    if (kind == 0) k_spmv<double><<<h->G, WG, 0, h->stream>>>(A, d_x, d_y, h->d_pm);
+    if (kind == 0) k_spmv<double><<<h->G, WG, 0, h->stream>>>(A, h->Gb, d_x, d_y, h->d_pm);
     else k_spmv_csr<<<Gc, WG, 0, h->stream>>>(n, h->d_csr_rp, h->d_csr_col, h->d_csr_val, d_x, d_y);
   }
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -1163,7 +1220,7 @@ int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, 
   if (!h || !d_b || !d_x || !out) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), d_x, h->d_t, h->d_pm);
+  k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), h->Gb, d_x, h->d_t, h->d_pm);
   if (h->op.mean_row >= 0)
     k_meanrow_fix<double><<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, h->d_t);
   k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pfin);

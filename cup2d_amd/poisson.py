@@ -217,6 +217,94 @@ class Solver:
         return self.residual_device(DeviceArray.from_host(self, b), DeviceArray.from_host(self, x))
 
 
+def _check_b(h, rc):
+    if rc != 0:
+        msg = _lib.load().cup2d_psb_last_error(h).decode()
+        raise _lib.SolverError(f"cup2d_psb error {rc}: {msg}")
+
+
+class BatchSolver:
+    """Batch handle (cup2d_psb_*): many independent meshes solved together.
+
+    Vectors are the instances' vectors concatenated in instance order;
+    ``offsets`` gives the first unknown of each instance.  Device buffers
+    come from an internal single-solve handle on the same device."""
+
+    def __init__(self, config: Optional[_lib.Config] = None, **kw):
+        lib = _lib.load()
+        self.config = config if config is not None else default_config(**kw)
+        h = ctypes.c_void_p()
+        rc = lib.cup2d_psb_create(ctypes.byref(self.config), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.SolverError(f"cup2d_psb_create error {rc}: {lib.cup2d_psb_last_error(None).decode()}")
+        self.h = h.value
+        self.mem = Solver(default_config(device=self.config.device))
+        self.meshes = []
+        self.offsets = None
+
+    def close(self):
+        if self.h:
+            _lib.load().cup2d_psb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_meshes(self, meshes):
+        descs, keep = [], []
+        for m in meshes:
+            d, k = _mesh_desc(m)
+            descs.append(d)
+            keep.append(k)
+        arr = (_lib.MeshDesc * len(descs))(*descs)
+        _check_b(self.h, _lib.load().cup2d_psb_set_meshes(self.h, len(descs), arr))
+        off = np.empty(len(descs) + 1, np.int64)
+        _check_b(self.h, _lib.load().cup2d_psb_offsets(self.h, off.ctypes.data))
+        self.meshes = list(meshes)
+        self.offsets = off
+        return self
+
+    @property
+    def n(self):
+        return int(self.offsets[-1])
+
+    def split(self, v):
+        """Per-instance views of a concatenated vector."""
+        return [v[self.offsets[k]:self.offsets[k + 1]] for k in range(len(self.meshes))]
+
+    def device_array(self, a=None, n=None):
+        if a is not None:
+            return DeviceArray.from_host(self.mem, a)
+        return DeviceArray(self.mem, self.n if n is None else n)
+
+    def solve(self, b, x0=None, tol=0.0, rel_tol=1e-10, max_restarts=100):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert b.size == self.n
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        st = (_lib.Stats * len(self.meshes))()
+        _check_b(self.h, _lib.load().cup2d_psb_solve(self.h, b.ctypes.data, x.ctypes.data, float(tol),
+                                                     float(rel_tol), int(max_restarts), st))
+        return x, [s.as_dict() for s in st]
+
+    def solve_device(self, d_b: DeviceArray, d_x: DeviceArray, tol=0.0, rel_tol=1e-10, max_restarts=100):
+        st = (_lib.Stats * len(self.meshes))()
+        _check_b(self.h, _lib.load().cup2d_psb_solve_device(self.h, d_b.ptr, d_x.ptr, float(tol),
+                                                            float(rel_tol), int(max_restarts), st))
+        return [s.as_dict() for s in st]
+
+    def autotune(self, d_b: DeviceArray):
+        ib = ctypes.c_int()
+        ms = ctypes.c_double()
+        _check_b(self.h, _lib.load().cup2d_psb_autotune(self.h, d_b.ptr, ctypes.byref(ib), ctypes.byref(ms)))
+        return ib.value, ms.value
+
+    def item_blocks(self):
+        return int(_lib.load().cup2d_psb_item_blocks(self.h))
+
+
 # ---------------------------------------------------------------------------
 # Mirror of the reference's solver plugin interface
 # ---------------------------------------------------------------------------

@@ -160,6 +160,48 @@ int cup2d_ps_d2h(cup2d_ps *h, void *dst, const void *src, size_t bytes);
  * h is NULL). */
 const char *cup2d_ps_last_error(const cup2d_ps *h);
 
+/* ------------------------------------------------------------------------
+ * Batched solves: many independent meshes solved together on one device.
+ * The reference solves one mesh per ExpAMRSolver::solve call
+ * (ExpAMRSolver.cpp:354-416); a batch handle runs that solve for every
+ * instance at once, each instance with its own scalars, stopping test,
+ * restarts and (mixed precision) FP64 refinement passes.  Vectors are the
+ * instances' vectors concatenated in instance order (offsets from
+ * cup2d_psb_offsets).  Config fields as for cup2d_ps_create; reserved_i[1]
+ * sets the blocks per work item (0: tuned value of this process, else 8).
+ * ------------------------------------------------------------------------ */
+typedef struct cup2d_psb cup2d_psb;
+
+int cup2d_psb_create(const cup2d_ps_config *cfg, cup2d_psb **out);
+int cup2d_psb_destroy(cup2d_psb *h);
+
+/* Assembles and uploads the operators of n_inst meshes (1..4096) with the
+ * handle's mean_constraint (per instance as cup2d_ps_set_mesh). */
+int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes);
+
+/* offsets[k] = first unknown of instance k, offsets[n_inst] = total (n_inst+1
+ * entries). */
+int cup2d_psb_offsets(const cup2d_psb *h, long long *offsets);
+
+/* Solves every instance (semantics of cup2d_ps_solve per instance, the same
+ * tol_abs / tol_rel / max_restarts for all); stats: n_inst entries or NULL
+ * (ms is the whole batch's time).  Host pointers. */
+int cup2d_psb_solve(cup2d_psb *h, const double *b, double *x, double tol_abs, double tol_rel,
+                    int max_restarts, cup2d_ps_stats *stats);
+/* Same with device pointers. */
+int cup2d_psb_solve_device(cup2d_psb *h, const double *d_b, double *d_x, double tol_abs,
+                           double tol_rel, int max_restarts, cup2d_ps_stats *stats);
+
+/* Times short runs of the batched iteration for several work-item sizes on
+ * right-hand side d_b, keeps the fastest (also for later handles of this
+ * process with the same precision and batch shape) and reports it and its
+ * ms per iteration.  No counterpart in the reference. */
+int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter);
+/* Blocks per work item in use. */
+int cup2d_psb_item_blocks(const cup2d_psb *h);
+
+const char *cup2d_psb_last_error(const cup2d_psb *h);
+
 #ifdef __cplusplus
 }
 #endif

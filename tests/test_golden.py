@@ -1,0 +1,56 @@
+"""The committed golden fixtures (tests/golden/, written by
+tests/golden/make_golden.py from the pinned oracle) against the oracle and
+the product's host assembly: any change of the operator or of the solver
+semantics shows up here."""
+
+import numpy as np
+import pytest
+
+from cup2d_amd import poisson as PS
+
+import golden_fixtures as G
+
+
+def test_pinv_fixture(oracle):
+    P = np.load(G.GOLDEN + "/pinv.npy", allow_pickle=False)
+    assert P.shape == (64, 64)
+    assert np.array_equal(oracle.pinv(), P)
+
+
+@pytest.mark.parametrize("name", G.CASES)
+def test_operator_fixture(oracle, name):
+    m, d = G.load(name)
+    A = oracle.assemble(m)
+    rp, col, val, mr = PS.assemble_csr(m, 0)
+    if "row_ptr" in d:
+        assert np.array_equal(A.row_ptr, d["row_ptr"])
+        assert np.array_equal(A.col, d["col"])
+        assert np.array_equal(A.val, d["val"])
+    else:
+        assert G.csr_digest(A.row_ptr, A.col, A.val) == G.digest(name)
+        assert A.nnz == int(d["nnz"])
+    # the product's host assembly gives the same bits
+    assert G.csr_digest(rp, col, val) == G.csr_digest(A.row_ptr, A.col, A.val)
+
+
+@pytest.mark.parametrize("name", G.CASES)
+def test_solution_fixture(oracle, name):
+    m, d = G.load(name)
+    A = oracle.assemble(m)
+    tr = oracle.true_residual(A, d["b"], d["x"])
+    assert tr["rel2"] <= 1e-13
+    h2 = np.repeat(oracle.h2(m), 64)
+    assert abs(np.dot(h2, d["x"])) <= 1e-13 * np.abs(d["x"]).max() * h2.sum()
+    # the oracle reproduces its own fixture
+    x, _ = oracle.bicgstab(A, d["b"], rel_tol=1e-14, norm="l2", max_iter=20000)
+    x = oracle.mean_remove(oracle.h2(m), x)
+    assert np.linalg.norm(x - d["x"]) <= 1e-10 * np.linalg.norm(d["x"])
+
+
+def test_mean_row_fixture(oracle):
+    m, d = G.load("uniform16x8")
+    A1 = oracle.assemble(m, 1)
+    assert A1.mean_row == int(d["mean_row"])
+    # both mean treatments give the same mean-free pressure
+    x0, x1 = d["x"], d["x_mean_row"]
+    assert np.linalg.norm(x0 - x1) <= 1e-8 * np.linalg.norm(x0)
