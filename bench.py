@@ -6,8 +6,14 @@ the 3-level graded AMR mesh (16x8 base blocks of 8x8 cells, disk at
 in HBM when the timed region starts.  Prints one JSON line.
 
 Also measured in the same run:
-* roofline of the SpMV (the operator application, timed live with HIP events
-  on the solver's stream) against the 8 TB/s HBM peak;
+* the same solve in mixed precision;
+* the batched workload: 64 independent AMR instances (block counts
+  log-uniform in [512, 8192], 216,896 blocks / 13.9M unknowns in total),
+  solved together in FP64 and in mixed precision, against the same 64
+  solves one after another with the single-solve FP64 solver;
+* roofline of the operator kernel (timed live with HIP events on the
+  solver's stream) on the bench mesh and on the DRAM-resident batch,
+  against the 8 TB/s HBM peak;
 * the CPU baseline: the oracle's restatement of the reference CPU solver
   (pipelined BiCGSTAB with the block-Cholesky preconditioner,
   AMRSolver.cpp:149-490) on a bounded sample, OpenMP on the host cores.
@@ -59,6 +65,69 @@ def cpu_baseline(mesh, b, budget_s=12.0):
                       f"each) in {dt:.1f} s on {threads} OpenMP threads"}
 
 
+def bench_batch(PS, W, steps, single_fp64_solves_per_s):
+    """The 64-instance batch: batched FP64 and mixed solves/s, the same 64
+    solves one after another with the single-solve FP64 solver, and the
+    roofline of the batched operator kernel (13.9M unknowns, DRAM-resident)."""
+    meshes = W.config4_meshes()
+    rhs = [W.config_rhs(m, 400 + k) for k, m in enumerate(meshes)]
+    b = np.concatenate(rhs)
+    out = {"workload": "64 AMR instances, N_b log-uniform in [512, 8192] (seed 4), 5 levels, one "
+                       "random disk each, seeded vortex RHS; all solved to 1e-10 rel 2-norm",
+           "instances": len(meshes), "n_blocks": int(sum(m.n_blocks for m in meshes))}
+    roof = None
+    for name, prec in (("fp64", PS.FP64), ("mixed", PS.MIXED)):
+        bs = PS.BatchSolver(PS.default_config(precision=prec, norm=PS.NORM_L2, max_iter=4000))
+        bs.set_meshes(meshes)
+        d_b = bs.device_array(b)
+        d_x = bs.device_array(np.zeros_like(b))
+        ib, ms_it = bs.autotune(d_b)
+        zeros = np.zeros_like(b)
+        d_x.upload(zeros)
+        bs.solve_device(d_b, d_x, rel_tol=1e-10)  # warm-up
+        t = []
+        for _ in range(steps):
+            d_x.upload(zeros)
+            t0 = time.perf_counter()
+            st = bs.solve_device(d_b, d_x, rel_tol=1e-10)
+            t.append(time.perf_counter() - t0)
+        its = [s["iterations"] for s in st]
+        sec = float(np.median(t))
+        out[name] = {"solves_per_s": len(meshes) / sec, "ms_per_batch": sec * 1e3,
+                     "item_blocks": ib, "ms_per_iteration_tuned": ms_it,
+                     "iterations_max": int(max(its)), "iterations_mean": float(np.mean(its)),
+                     "refinements_max": int(max(s["refinements"] for s in st)),
+                     "all_converged": bool(all(s["converged"] for s in st))}
+        if prec == PS.FP64:
+            sz = bs.sizes()
+            d_y = bs.device_array()
+            ms = bs.spmv_time(d_x, d_y, reps=20)
+            byt = spmv_bytes_csr(sz["n"], sz["nnz"])
+            ach = byt / (ms * 1e-3) / 1e9
+            roof = {"bound": "hbm", "kernel": "kb_spmv (batched stencil SpMV, FP64, 13.9M rows)",
+                    "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": byt,
+                    "ms_per_launch": ms}
+        del bs, d_b, d_x
+    # the same 64 solves one after another (single-solve FP64 solver)
+    s = PS.Solver(norm=PS.NORM_L2, max_iter=4000)
+    tot = 0.0
+    for m, bb in zip(meshes, rhs):
+        s.set_mesh(m)
+        d_b = PS.DeviceArray.from_host(s, bb)
+        d_x = PS.DeviceArray(s, m.n)
+        d_x.upload(np.zeros(m.n))
+        s.solve_device(d_b, d_x, rel_tol=1e-10)  # warm-up (graph capture)
+        d_x.upload(np.zeros(m.n))
+        t0 = time.perf_counter()
+        s.solve_device(d_b, d_x, rel_tol=1e-10)
+        tot += time.perf_counter() - t0
+    out["fp64_sequential"] = {"solves_per_s": len(meshes) / tot, "ms_total": tot * 1e3}
+    out["mixed_batched_vs_fp64_sequential"] = out["mixed"]["solves_per_s"] / out["fp64_sequential"]["solves_per_s"]
+    out["fp64_batched_vs_fp64_sequential"] = out["fp64"]["solves_per_s"] / out["fp64_sequential"]["solves_per_s"]
+    return out, roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +139,8 @@ def main():
                     help="1: replay iteration chunks from a hipGraph; 0: plain stream launches")
     ap.add_argument("--chunk", type=int, default=0, help="iterations per chunk (0: automatic)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
@@ -168,6 +239,10 @@ def main():
             "csr_kernel_ms_per_launch": ms_csr,
         },
     }
+    if not args.no_batch:
+        batch, roof_dram = bench_batch(PS, W, args.batch_steps, fp64["solves_per_s"])
+        out["config"]["batch"] = batch
+        out["roofline_dram"] = roof_dram
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, b)
     print(json.dumps(out))

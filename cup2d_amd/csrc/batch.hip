@@ -51,7 +51,8 @@ constexpr int MAX_PASSES = 16;  // FP64 refinement passes per instance (mixed)
 
 struct BView {
   OpView A;                          // concatenated operator (A.mean_row unused)
-  const int *item_inst, *item_b0, *item_nb;  // item_nb < 0: explicit item
+  const int4 *item;                  // {instance, first block / explicit row, count, 0};
+                                     // count < 0: explicit item of -count rows
   const int *item_begin;             // [K+1] first item of each instance
   const int *tile_inst, *tile_b0, *tile_nb;
   const long long *mean_row;         // [K] global row of the mean constraint or -1
@@ -108,31 +109,68 @@ __device__ void range_totals(const double *__restrict__ part, int NI, int i0, in
 }
 
 // ----------------------------------------------------------------------------
-// item kernels.  A block item (item_nb > 0) is taken by a workgroup, a
+// item kernels.  A block item (count > 0) is taken by a workgroup, a
 // wavefront per UB blocks at a time (lane = cell); an explicit item
-// (item_nb < 0: -item_nb explicit rows from item_b0) by a workgroup, a
+// (count < 0: -count explicit rows from the first one) by a workgroup, a
 // thread per row.  Every item of a running instance stores its partials,
 // zeros where it has nothing to add.
 // ----------------------------------------------------------------------------
 
-constexpr int UB = 2;  // blocks per wavefront in flight
+constexpr int UB = 4;  // blocks per wavefront in flight
+
+// Calls body(item, meta) for the workgroup's items (grid-stride) whose
+// instance is running (every item when st is null); the next item's meta is
+// loaded while the current one is processed.  body may synchronize the
+// workgroup (the skip is uniform).
+template <class F>
+__device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body) {
+  int item = blockIdx.x;
+  if (item >= V.n_items) return;
+  int4 mt = V.item[item];
+  for (;;) {
+    const int next = item + (int)gridDim.x;
+    const bool more = next < V.n_items;
+    const int4 nmt = V.item[more ? next : item];
+    if (!st || !st[mt.x].done) body(item, mt);
+    if (!more) break;
+    item = next;
+    mt = nmt;
+  }
+}
 
 // Runs f(R, u, bl) for the item's blocks, UB per wave at a time, after
-// issuing every load of those blocks (stencil, h^2 and aux[i]).
+// issuing every load of those blocks (stencil, h^2 and aux[i]).  The headers
+// of all the wave's blocks of the item are loaded up front, one per lane
+// (lane s*UB + u holds block (s*WAVES + wave)*UB + u; items have at most
+// 64*WAVES blocks), and read back with readlane.
 template <class TV, class TA, class F>
 __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict__ x,
                                             const TA *__restrict__ aux, int b0, int nbk, F &&f) {
-  const int c = lane_id();
-  for (int j = wave_id() * UB; j < nbk; j += WAVES * UB) {
+  const int c = lane_id(), w = wave_id();
+  const int jl = ((c / UB) * WAVES + w) * UB + (c % UB);
+  int4 hn = make_int4(-1, -1, -1, -1);
+  unsigned long long hm = 0;
+  if (jl < nbk) {
+    hn = V.A.nbr[b0 + jl];
+    hm = V.A.mask[b0 + jl];
+  }
+  const unsigned hm_lo = (unsigned)hm, hm_hi = (unsigned)(hm >> 32);
+  for (int s = 0; (s * WAVES + w) * UB < nbk; s++) {
+    const int j0 = (s * WAVES + w) * UB;
     int bl[UB];
     bool ok[UB];
+    BlockRows<UB, TV> R;
 #pragma unroll
     for (int u = 0; u < UB; u++) {
-      ok[u] = j + u < nbk;
-      bl[u] = b0 + (ok[u] ? j + u : j);
+      ok[u] = j0 + u < nbk;
+      bl[u] = b0 + (ok[u] ? j0 + u : j0);
+      const int src = s * UB + u;
+      R.nb[u] = make_int4(__builtin_amdgcn_readlane(hn.x, src), __builtin_amdgcn_readlane(hn.y, src),
+                          __builtin_amdgcn_readlane(hn.z, src), __builtin_amdgcn_readlane(hn.w, src));
+      R.m[u] = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hm_lo, src) |
+               ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hm_hi, src) << 32);
     }
-    BlockRows<UB, TV> R;
-    R.load(V.A, x, aux, bl, c);
+    R.load_data(V.A, x, aux, bl, c);
 #pragma unroll
     for (int u = 0; u < UB; u++)
       if (ok[u]) f(R, u, bl[u]);
@@ -149,11 +187,8 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
                                               T *__restrict__ rhat, T *__restrict__ p,
                                               T *__restrict__ nu, const Scal *st, double *p5) {
   const int c = lane_id();
-  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
-    const int k = V.item_inst[item];
-    if (st[k].done) continue;
-    const int b0 = V.item_b0[item], cnt = V.item_nb[item];
-    const long long mr = V.mean_row[k];
+  running_items(V, st, [&](int item, int4 mt) {
+    const long long mr = V.mean_row[mt.x];
     double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
     auto store_r = [&](long i, double y, double bi) {
       const double ri = (i == mr) ? 0.0 : bi - y;
@@ -165,8 +200,8 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
       v[1] += rd * rd;
       v[4] += rd;
     };
-    if (cnt > 0) {
-      item_blocks<double>(V, x, b, b0, cnt, [&](const BlockRows<UB, double> &R, int u, int bb) {
+    if (mt.z > 0) {
+      item_blocks<double>(V, x, b, mt.y, mt.z, [&](const BlockRows<UB, double> &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
         const long i = R.i[u];
@@ -176,15 +211,15 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
         v[6] += R.hh[u] * R.xc[u];
         if (plain) store_r(i, y, R.a[u]);
       });
-    } else if ((int)threadIdx.x < -cnt) {
-      const int e = b0 + threadIdx.x, row = V.A.erow[e];
+    } else if ((int)threadIdx.x < -mt.z) {
+      const int e = mt.y + threadIdx.x, row = V.A.erow[e];
       store_r(row, explicit_row<double>(V.A, x, e), b[row]);
     }
     v[2] = v[1];
     v[3] = v[1];
     v[5] = v[4];
     store_item_partials<NQB, 1u>(v, p5, V.n_items, item);
-  }
+  });
 }
 
 // nu = A z1; partials of (rhat, nu) and sum h^2 z1
@@ -193,11 +228,8 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
                                                  T *__restrict__ nu, const T *__restrict__ rhat,
                                                  const Scal *st, double *p2) {
   const int c = lane_id();
-  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
-    const int k = V.item_inst[item];
-    if (st[k].done) continue;
-    const int b0 = V.item_b0[item], cnt = V.item_nb[item];
-    const long long mr = V.mean_row[k];
+  running_items(V, st, [&](int item, int4 mt) {
+    const long long mr = V.mean_row[mt.x];
     double v[2] = {0.0, 0.0};
     auto store_y = [&](long i, double y, double rh) {
       if (i == mr) y = 0.0;  // completed by kb_alpha
@@ -205,19 +237,19 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
       nu[i] = yt;
       v[0] += rh * (double)yt;
     };
-    if (cnt > 0) {
-      item_blocks<T>(V, z1, rhat, b0, cnt, [&](const BlockRows<UB, T> &R, int u, int bb) {
+    if (mt.z > 0) {
+      item_blocks<T>(V, z1, rhat, mt.y, mt.z, [&](const BlockRows<UB, T> &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
         v[1] += R.hh[u] * R.xc[u];
         if (plain) store_y(R.i[u], y, R.a[u]);
       });
-    } else if ((int)threadIdx.x < -cnt) {
-      const int e = b0 + threadIdx.x, row = V.A.erow[e];
+    } else if ((int)threadIdx.x < -mt.z) {
+      const int e = mt.y + threadIdx.x, row = V.A.erow[e];
       store_y(row, explicit_row<T>(V.A, z1, e), (double)rhat[row]);
     }
     store_item_partials<2, 0u>(v, p2, V.n_items, item);
-  }
+  });
 }
 
 // t = A z2; partials of (t, s), (t, t) and sum h^2 z2
@@ -226,11 +258,8 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
                                                 T *__restrict__ t, const T *__restrict__ sv,
                                                 const Scal *st, double *p4) {
   const int c = lane_id();
-  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
-    const int k = V.item_inst[item];
-    if (st[k].done) continue;
-    const int b0 = V.item_b0[item], cnt = V.item_nb[item];
-    const long long mr = V.mean_row[k];
+  running_items(V, st, [&](int item, int4 mt) {
+    const long long mr = V.mean_row[mt.x];
     double v[3] = {0.0, 0.0, 0.0};
     auto store_y = [&](long i, double y, double si) {
       if (i == mr) y = 0.0;  // completed by kb_omega
@@ -240,23 +269,57 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
       v[0] += yd * si;
       v[1] += yd * yd;
     };
-    if (cnt > 0) {
-      item_blocks<T>(V, z2, sv, b0, cnt, [&](const BlockRows<UB, T> &R, int u, int bb) {
+    if (mt.z > 0) {
+      item_blocks<T>(V, z2, sv, mt.y, mt.z, [&](const BlockRows<UB, T> &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
         v[2] += R.hh[u] * R.xc[u];
         if (plain) store_y(R.i[u], y, R.a[u]);
       });
-    } else if ((int)threadIdx.x < -cnt) {
-      const int e = b0 + threadIdx.x, row = V.A.erow[e];
+    } else if ((int)threadIdx.x < -mt.z) {
+      const int e = mt.y + threadIdx.x, row = V.A.erow[e];
       store_y(row, explicit_row<T>(V.A, z2, e), (double)sv[row]);
     }
     store_item_partials<3, 0u>(v, p4, V.n_items, item);
-  }
+  });
+}
+
+// y = A x over every instance (the operator alone, FP64); partials of
+// sum h^2 x for the mean rows
+__global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict__ x,
+                                              double *__restrict__ y, double *pm) {
+  const int c = lane_id();
+  running_items(V, nullptr, [&](int item, int4 mt) {
+    double v[1] = {0.0};
+    if (mt.z > 0) {
+      item_blocks<double>(V, x, (const double *)nullptr, mt.y, mt.z,
+                          [&](const BlockRows<UB, double> &R, int u, int) {
+                            bool plain;
+                            const double yi = R.y(u, c, plain);
+                            v[0] += R.hh[u] * R.xc[u];
+                            if (plain) y[R.i[u]] = yi;
+                          });
+    } else if ((int)threadIdx.x < -mt.z) {
+      const int e = mt.y + threadIdx.x;
+      y[V.A.erow[e]] = explicit_row<double>(V.A, x, e);
+    }
+    store_item_partials<1, 0u>(v, pm, V.n_items, item);
+  });
+}
+
+// y[mean row of k] = sum h^2 x over instance k
+__global__ __launch_bounds__(WG) void kb_meanrow(BView V, const double *pm, double *__restrict__ y) {
+  const int k = blockIdx.x;
+  const long long mr = V.mean_row[k];
+  if (mr < 0) return;
+  double o[1];
+  range_totals<1, 0u>(pm, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (threadIdx.x == 0) y[mr] = o[0];
 }
 
 // x_opt = x when the previous iterate was the best so far; x += alpha z1 +
-// omega z2 (FP64 x); r = s - omega t; update partials
+// omega z2 (FP64 x); r = s - omega t; update partials.  Four cells per
+// thread per pass, all loads first.
 template <class T>
 __global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ x,
                                                   double *__restrict__ xopt,
@@ -265,46 +328,60 @@ __global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ 
                                                   const T *__restrict__ t,
                                                   const T *__restrict__ rhat, const Scal *st,
                                                   double *p5) {
-  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
-    const int k = V.item_inst[item];
-    if (st[k].done) continue;
-    const double alpha = st[k].alpha, omega = st[k].omega;
-    const int improved = st[k].improved;
-    const int cnt = V.item_nb[item];
-    const long i0 = (long)V.item_b0[item] * 64, i1 = i0 + (cnt > 0 ? (long)cnt * 64 : 0);
+  running_items(V, st, [&](int item, int4 mt) {
+    const double alpha = st[mt.x].alpha, omega = st[mt.x].omega;
+    const int improved = st[mt.x].improved;
     double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
-    for (long i = i0 + threadIdx.x; i < i1; i += WG) {
-      const double xo = x[i];
-      if (improved) xopt[i] = xo;
-      x[i] = xo + alpha * (double)z1[i] + omega * (double)z2[i];
-      const T rt = (T)((double)r[i] - omega * (double)t[i]);
-      r[i] = rt;
-      const double rd = (double)rt, rh = (double)rhat[i];
-      v[0] = nmax(v[0], fabs(rd));
-      v[1] += rd * rd;
-      v[2] += rh * rd;
-      v[3] += rh * rh;
-      v[4] += rd;
-      v[5] += rh;
+    const long i0 = (long)mt.y * 64, ncell = mt.z > 0 ? (long)mt.z * 64 : 0;
+    for (long off = threadIdx.x; off < ncell; off += 4 * WG) {
+      long ii[4];
+      bool ok[4];
+      double xo[4], a1[4], a2[4], rr[4], tt[4], rh[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        ok[q] = off + q * WG < ncell;
+        ii[q] = i0 + (ok[q] ? off + q * WG : off);
+        xo[q] = x[ii[q]];
+        a1[q] = (double)z1[ii[q]];
+        a2[q] = (double)z2[ii[q]];
+        rr[q] = (double)r[ii[q]];
+        tt[q] = (double)t[ii[q]];
+        rh[q] = (double)rhat[ii[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (!ok[q]) continue;
+        if (improved) xopt[ii[q]] = xo[q];
+        x[ii[q]] = xo[q] + alpha * a1[q] + omega * a2[q];
+        const T rt = (T)(rr[q] - omega * tt[q]);
+        r[ii[q]] = rt;
+        const double rd = (double)rt;
+        v[0] = nmax(v[0], fabs(rd));
+        v[1] += rd * rd;
+        v[2] += rh[q] * rd;
+        v[3] += rh[q] * rh[q];
+        v[4] += rd;
+        v[5] += rh[q];
+      }
     }
     store_item_partials<NQB, 1u>(v, p5, V.n_items, item);
-  }
+  });
 }
 
 // cell range of a block item (empty for an explicit item)
-__device__ __forceinline__ void item_cells(const BView &V, int item, long &i0, long &i1) {
-  const int cnt = V.item_nb[item];
-  i0 = (long)V.item_b0[item] * 64;
-  i1 = i0 + (cnt > 0 ? (long)cnt * 64 : 0);
+__device__ __forceinline__ void item_cells(const int4 &mt, long &i0, long &i1) {
+  i0 = (long)mt.y * 64;
+  i1 = i0 + (mt.z > 0 ? (long)mt.z * 64 : 0);
 }
 
 // x = x_opt on the instances whose last iterate is not their best
 __global__ __launch_bounds__(WG) void kb_select(BView V, double *__restrict__ x,
                                                 const double *__restrict__ xopt, const Scal *st) {
   for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
-    if (st[V.item_inst[item]].improved) continue;
+    const int4 mt = V.item[item];
+    if (st[mt.x].improved) continue;
     long i0, i1;
-    item_cells(V, item, i0, i1);
+    item_cells(mt, i0, i1);
     for (long i = i0 + threadIdx.x; i < i1; i += WG) x[i] = xopt[i];
   }
 }
@@ -313,7 +390,7 @@ __global__ __launch_bounds__(WG) void kb_select(BView V, double *__restrict__ x,
 __global__ __launch_bounds__(WG) void kb_fmean(BView V, const double *__restrict__ x, double *pf) {
   for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
     long i0, i1;
-    item_cells(V, item, i0, i1);
+    item_cells(V.item[item], i0, i1);
     double v[2] = {0.0, 0.0};
     for (long i = i0 + threadIdx.x; i < i1; i += WG) {
       const double hh = V.A.h2[i >> 6];
@@ -328,9 +405,10 @@ __global__ __launch_bounds__(WG) void kb_fmean(BView V, const double *__restrict
 __global__ __launch_bounds__(WG) void kb_fwrite(BView V, const double *__restrict__ x,
                                                 double *__restrict__ out, const Scal *st) {
   for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
-    const double mean = st[V.item_inst[item]].mean;
+    const int4 mt = V.item[item];
+    const double mean = st[mt.x].mean;
     long i0, i1;
-    item_cells(V, item, i0, i1);
+    item_cells(mt, i0, i1);
     for (long i = i0 + threadIdx.x; i < i1; i += WG) out[i] = x[i] - mean;
   }
 }
@@ -615,7 +693,7 @@ struct cup2d_psb {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool ready = false;
   int K = 0;
-  long long nb = 0, n = 0;
+  long long nb = 0, n = 0, nnz = 0;
   int ib = 0;                       // blocks per item
   std::vector<long long> boff;      // [K+1] block offset of each instance
   std::vector<long long> eoff;      // [K+1] explicit-row offset of each instance
@@ -631,7 +709,8 @@ struct cup2d_psb {
   bf16x8 *d_pfrag = nullptr;
   long long *d_mean_row = nullptr;
   // device: work units
-  int *d_item_inst = nullptr, *d_item_b0 = nullptr, *d_item_nb = nullptr, *d_item_begin = nullptr;
+  int4 *d_item = nullptr;
+  int *d_item_begin = nullptr;
   int *d_tile_inst = nullptr, *d_tile_b0 = nullptr, *d_tile_nb = nullptr;
   double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr;
   // device: vectors and state
@@ -705,9 +784,7 @@ BView bview(const cup2d_psb *h) {
   V.A.ecol = h->d_ecol;
   V.A.eval = h->d_eval;
   V.A.h2 = h->d_h2;
-  V.item_inst = h->d_item_inst;
-  V.item_b0 = h->d_item_b0;
-  V.item_nb = h->d_item_nb;
+  V.item = h->d_item;
   V.item_begin = h->d_item_begin;
   V.tile_inst = h->d_tile_inst;
   V.tile_b0 = h->d_tile_b0;
@@ -762,9 +839,9 @@ int build_units(cup2d_psb *h, int ib) {
   const size_t NI = h->item_inst.size();
   auto &pool = h->unit_allocs;
   int rc;
-  if ((rc = bupload(h, pool, &h->d_item_inst, h->item_inst)) ||
-      (rc = bupload(h, pool, &h->d_item_b0, h->item_b0)) ||
-      (rc = bupload(h, pool, &h->d_item_nb, h->item_nb)) ||
+  std::vector<int4> item4(NI);
+  for (size_t i = 0; i < NI; i++) item4[i] = make_int4(h->item_inst[i], h->item_b0[i], h->item_nb[i], 0);
+  if ((rc = bupload(h, pool, &h->d_item, item4)) ||
       (rc = bupload(h, pool, &h->d_item_begin, h->item_begin)) ||
       (rc = bupload(h, pool, &h->d_tile_inst, h->tile_inst)) ||
       (rc = bupload(h, pool, &h->d_tile_b0, h->tile_b0)) ||
@@ -1076,6 +1153,7 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
   h->ready = false;
   std::vector<cup2d::Operator> ops(n_inst);
   std::vector<long long> boff(n_inst + 1, 0);
+  long long nnz = 0;
   for (int k = 0; k < n_inst; k++) {
     const cup2d_mesh_desc &m = meshes[k];
     MeshDesc md{m.bpdx, m.bpdy, m.levels, m.periodic_x, m.periodic_y, m.extent, m.n_blocks,
@@ -1084,11 +1162,13 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
     if (!build_operator(md, h->cfg.mean_constraint, ops[k], err))
       return bfail(h, CUP2D_PS_EINVAL, "mesh " + std::to_string(k) + ": " + err);
     boff[k + 1] = boff[k] + ops[k].nb;
+    nnz += ops[k].nnz();
   }
   if (boff[n_inst] * 64 >= (1LL << 31) - 64)
     return bfail(h, CUP2D_PS_EINVAL, "batch too large for 32-bit cell indices");
   h->K = n_inst;
   h->boff = boff;
+  h->nnz = nnz;
   h->nb = boff[n_inst];
   h->n = h->nb * 64;
   // concatenated operator: block ids, explicit rows and their columns shifted
@@ -1222,6 +1302,47 @@ int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double
   g_tuned[{(int)h->mixed, h->K, h->nb}] = best;
   if (item_blocks) *item_blocks = best;
   if (ms_per_iter) *ms_per_iter = best_ms;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_spmv_device(cup2d_psb *h, const double *d_x, double *d_y) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!d_x || !d_y) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "spmv before cup2d_psb_set_meshes");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  const BView V = bview(h);
+  kb_spmv<<<grid_items(h), WG, 0, h->stream>>>(V, d_x, d_y, h->d_pm);
+  kb_meanrow<<<h->K, WG, 0, h->stream>>>(V, h->d_pm, d_y);
+  BCHK(h, hipGetLastError());
+  BCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_spmv_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
+                        double *ms_per_launch) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!d_x || !d_y || !ms_per_launch || reps <= 0) return bfail(h, CUP2D_PS_EINVAL, "bad argument");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "spmv before cup2d_psb_set_meshes");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  const BView V = bview(h);
+  const int G = grid_items(h);
+  BCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int k = 0; k < reps; k++) kb_spmv<<<G, WG, 0, h->stream>>>(V, d_x, d_y, h->d_pm);
+  BCHK(h, hipEventRecord(h->ev1, h->stream));
+  BCHK(h, hipEventSynchronize(h->ev1));
+  BCHK(h, hipGetLastError());
+  float ms = 0;
+  BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *ms_per_launch = ms / reps;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_sizes(const cup2d_psb *h, long long *n, long long *nnz, long long *n_explicit) {
+  if (!h) return CUP2D_PS_EINVAL;
+  if (!h->ready) return CUP2D_PS_ENOMESH;
+  if (n) *n = h->n;
+  if (nnz) *nnz = h->nnz;
+  if (n_explicit) *n_explicit = h->eoff[h->K];
   return CUP2D_PS_OK;
 }
 

@@ -206,10 +206,16 @@ struct BlockRows {
     for (int u = 0; u < U; u++) {
       nb[u] = A.nbr[b[u]];
       m[u] = A.mask[b[u]];
-      i[u] = (long)b[u] * 64 + c;
     }
+    load_data(A, x, aux, b, c);
+  }
+  // the same with nb[] and m[] already set
+  template <class TA>
+  __device__ __forceinline__ void load_data(const OpView &A, const TV *__restrict__ x,
+                                            const TA *__restrict__ aux, const int (&b)[U], int c) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
+      i[u] = (long)b[u] * 64 + c;
       stencil_load<TV>(x, b[u], nb[u], c, xc[u], xh[u], xv[u]);
       a[u] = aux ? (double)aux[i[u]] : 0.0;
       hh[u] = A.h2[b[u]];

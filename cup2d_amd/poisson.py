@@ -304,6 +304,24 @@ class BatchSolver:
     def item_blocks(self):
         return int(_lib.load().cup2d_psb_item_blocks(self.h))
 
+    def sizes(self):
+        n, nnz, ne = (ctypes.c_longlong() for _ in range(3))
+        _check_b(self.h, _lib.load().cup2d_psb_sizes(self.h, ctypes.byref(n), ctypes.byref(nnz),
+                                                     ctypes.byref(ne)))
+        return {"n": n.value, "nnz": nnz.value, "n_explicit": ne.value}
+
+    def spmv(self, x):
+        dx = self.device_array(x)
+        dy = self.device_array()
+        _check_b(self.h, _lib.load().cup2d_psb_spmv_device(self.h, dx.ptr, dy.ptr))
+        return dy.numpy()
+
+    def spmv_time(self, d_x: DeviceArray, d_y: DeviceArray, reps=50):
+        ms = ctypes.c_double()
+        _check_b(self.h, _lib.load().cup2d_psb_spmv_time(self.h, d_x.ptr, d_y.ptr, int(reps),
+                                                         ctypes.byref(ms)))
+        return ms.value
+
 
 # ---------------------------------------------------------------------------
 # Mirror of the reference's solver plugin interface

@@ -200,6 +200,17 @@ int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double
 /* Blocks per work item in use. */
 int cup2d_psb_item_blocks(const cup2d_psb *h);
 
+/* y = A x over every instance (device pointers, concatenated vectors; the
+ * mean rows with mean_constraint), and a timing of `reps` back-to-back
+ * launches of the operator kernel on the handle's stream (HIP events; mean
+ * ms per launch).  Reference: BiCGSTABSolver::hd_cusparseSpMV
+ * (BiCGSTAB.cu:315-410), applied per instance. */
+int cup2d_psb_spmv_device(cup2d_psb *h, const double *d_x, double *d_y);
+int cup2d_psb_spmv_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
+                        double *ms_per_launch);
+/* Totals over the instances: unknowns, nonzeros, explicit (coarse-fine) rows. */
+int cup2d_psb_sizes(const cup2d_psb *h, long long *n, long long *nnz, long long *n_explicit);
+
 const char *cup2d_psb_last_error(const cup2d_psb *h);
 
 #ifdef __cplusplus

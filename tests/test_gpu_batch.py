@@ -178,3 +178,20 @@ def test_batch_rejects_bad_input(batch_meshes):
     bs = make_batch(batch_meshes[:2])
     with pytest.raises(PS._lib.SolverError):
         bs.solve(np.zeros(bs.n), rel_tol=-1.0)
+
+
+@pytest.mark.parametrize("mc", [0, 1])
+def test_batch_spmv_matches_oracle(oracle, batch_meshes, mc):
+    bs = make_batch(batch_meshes, mean_constraint=mc)
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=bs.n)
+    y = bs.spmv(x)
+    sz = bs.sizes()
+    assert sz["n"] == bs.n
+    nnz = 0
+    for k, m in enumerate(batch_meshes):
+        A = oracle.assemble(m, mc)
+        nnz += A.nnz
+        ref = oracle.spmv(A, bs.split(x)[k], mean_mode=mc, h2v=oracle.h2(m))
+        assert np.abs(bs.split(y)[k] - ref).max() <= 1e-14 * np.abs(ref).max()
+    assert sz["nnz"] == nnz
