@@ -65,7 +65,7 @@ def cpu_baseline(mesh, b, budget_s=12.0):
                       f"each) in {dt:.1f} s on {threads} OpenMP threads"}
 
 
-def bench_batch(PS, W, steps, single_fp64_solves_per_s):
+def bench_batch(PS, W, steps, use_graph):
     """The 64-instance batch: batched FP64 and mixed solves/s, the same 64
     solves one after another with the single-solve FP64 solver, and the
     roofline of the batched operator kernel (13.9M unknowns, DRAM-resident)."""
@@ -77,7 +77,8 @@ def bench_batch(PS, W, steps, single_fp64_solves_per_s):
            "instances": len(meshes), "n_blocks": int(sum(m.n_blocks for m in meshes))}
     roof = None
     for name, prec in (("fp64", PS.FP64), ("mixed", PS.MIXED)):
-        bs = PS.BatchSolver(PS.default_config(precision=prec, norm=PS.NORM_L2, max_iter=4000))
+        bs = PS.BatchSolver(PS.default_config(precision=prec, norm=PS.NORM_L2, max_iter=4000,
+                                              use_graph=use_graph))
         bs.set_meshes(meshes)
         d_b = bs.device_array(b)
         d_x = bs.device_array(np.zeros_like(b))
@@ -110,7 +111,7 @@ def bench_batch(PS, W, steps, single_fp64_solves_per_s):
                     "ms_per_launch": ms}
         del bs, d_b, d_x
     # the same 64 solves one after another (single-solve FP64 solver)
-    s = PS.Solver(norm=PS.NORM_L2, max_iter=4000)
+    s = PS.Solver(norm=PS.NORM_L2, max_iter=4000, use_graph=use_graph)
     tot = 0.0
     for m, bb in zip(meshes, rhs):
         s.set_mesh(m)
@@ -128,6 +129,39 @@ def bench_batch(PS, W, steps, single_fp64_solves_per_s):
     return out, roof
 
 
+def kernel_only(PS, W, which, reps):
+    """Launches one kernel `reps` times (for rocprofv3 counter runs) and
+    prints its mean time and algorithmic bytes."""
+    if which == "copy":
+        import torch
+        n = 64 << 20  # 64 Mi doubles = 512 MiB each way
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(reps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                          "launches": reps}))
+        return
+    if which == "spmv":
+        mesh = W.config2_mesh()
+        s = PS.Solver(norm=PS.NORM_L2)
+        s.set_mesh(mesh)
+        sz = s.sizes()
+        d_x = PS.DeviceArray.from_host(s, np.random.default_rng(0).normal(size=mesh.n))
+        d_y = PS.DeviceArray(s, mesh.n)
+        ms = s.spmv_time(d_x, d_y, kind=0, reps=reps)
+    else:
+        meshes = W.config4_meshes()
+        s = PS.BatchSolver(PS.default_config(norm=PS.NORM_L2)).set_meshes(meshes)
+        sz = s.sizes()
+        d_x = s.device_array(np.random.default_rng(0).normal(size=s.n))
+        d_y = s.device_array()
+        ms = s.spmv_time(d_x, d_y, reps=reps)
+    print(json.dumps({"kernel": which, "ms_per_launch": ms, "launches": reps, "n": sz["n"],
+                      "nnz": sz["nnz"], "algorithmic_bytes": spmv_bytes_csr(sz["n"], sz["nnz"])}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +175,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "copy"], default=None,
+                    help="profiling aid: launch only this kernel --steps times and exit "
+                         "(spmv: operator on the bench mesh; batch_spmv: batched operator on the "
+                         "64-instance batch; copy: a plain 512 MiB device copy, the traffic "
+                         "counters' calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
@@ -149,6 +188,10 @@ def main():
 
     from cup2d_amd import poisson as PS
     from cup2d_amd import workloads as W
+
+    if args.kernel_only:
+        kernel_only(PS, W, args.kernel_only, args.steps)
+        return
 
     mesh = W.config2_mesh()
     b = W.config_rhs(mesh, 2, with_sources=True)
@@ -240,7 +283,7 @@ def main():
         },
     }
     if not args.no_batch:
-        batch, roof_dram = bench_batch(PS, W, args.batch_steps, fp64["solves_per_s"])
+        batch, roof_dram = bench_batch(PS, W, args.batch_steps, args.graph)
         out["config"]["batch"] = batch
         out["roofline_dram"] = roof_dram
     if not args.no_cpu:

@@ -48,13 +48,14 @@ constexpr int NQB = 7;        // update totals (NQ5) + sum h^2 x (initial mean r
 constexpr int BGRID = 2048;   // workgroups of the item / tile kernels (grid-stride beyond)
 constexpr int MAX_INST = 4096;
 constexpr int MAX_PASSES = 16;  // FP64 refinement passes per instance (mixed)
+constexpr int MAX_ITEM_BLOCKS = 256;  // item_blocks keeps one block header per lane: 64 * WAVES
 
 struct BView {
   OpView A;                          // concatenated operator (A.mean_row unused)
   const int4 *item;                  // {instance, first block / explicit row, count, 0};
                                      // count < 0: explicit item of -count rows
   const int *item_begin;             // [K+1] first item of each instance
-  const int *tile_inst, *tile_b0, *tile_nb;
+  const int4 *tile;                  // {instance, first block, blocks, 0}
   const long long *mean_row;         // [K] global row of the mean constraint or -1
   int K, n_items, n_tiles;
 };
@@ -417,6 +418,26 @@ __global__ __launch_bounds__(WG) void kb_fwrite(BView V, const double *__restric
 // tile kernels (preconditioner; a wavefront per 16-block tile)
 // ----------------------------------------------------------------------------
 
+// Calls body(meta) for the wave's tiles (grid-stride over waves) whose
+// instance is running; the next tile's meta is loaded while the current one
+// is processed.
+template <class F>
+__device__ __forceinline__ void running_tiles(const BView &V, const Scal *st, F &&body) {
+  const int stride = gridDim.x * WAVES;
+  int tile = blockIdx.x * WAVES + wave_id();
+  if (tile >= V.n_tiles) return;
+  int4 tm = V.tile[tile];
+  for (;;) {
+    const int next = tile + stride;
+    const bool more = next < V.n_tiles;
+    const int4 ntm = V.tile[more ? next : tile];
+    if (!st[tm.x].done) body(tm);
+    if (!more) break;
+    tile = next;
+    tm = ntm;
+  }
+}
+
 // FP64: p = r + beta (p - omega nu), or on restart p = rhat = r; z1 = P p
 // (P in LDS, 4 blocks per step)
 __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__restrict__ P,
@@ -429,10 +450,8 @@ __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__rest
   __shared__ double vs[WAVES][4 * 64];
   load_P(Ps, P);
   const int c = lane_id(), w = wave_id();
-  for (int tile = blockIdx.x * WAVES + w; tile < V.n_tiles; tile += gridDim.x * WAVES) {
-    const int k = V.tile_inst[tile];
-    if (st[k].done) continue;
-    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+  running_tiles(V, st, [&](int4 tm) {
+    const int k = tm.x, b0 = tm.y, tnb = tm.z;
     const double beta = st[k].beta, omega = st[k].omega;
     const int restart = st[k].restart;
     for (int g = 0; g < tnb; g += 4) {
@@ -468,7 +487,7 @@ __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__rest
       for (int j = 0; j < 4; j++)
         if (ok[j]) z1[ii[j]] = z[j];
     }
-  }
+  });
 }
 
 // FP64: s = r - alpha nu (stored in r); z2 = P s
@@ -480,10 +499,8 @@ __global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__rest
   __shared__ double vs[WAVES][4 * 64];
   load_P(Ps, P);
   const int c = lane_id(), w = wave_id();
-  for (int tile = blockIdx.x * WAVES + w; tile < V.n_tiles; tile += gridDim.x * WAVES) {
-    const int k = V.tile_inst[tile];
-    if (st[k].done) continue;
-    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+  running_tiles(V, st, [&](int4 tm) {
+    const int k = tm.x, b0 = tm.y, tnb = tm.z;
     const double alpha = st[k].alpha;
     for (int g = 0; g < tnb; g += 4) {
       long ii[4];
@@ -510,7 +527,7 @@ __global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__rest
       for (int j = 0; j < 4; j++)
         if (ok[j]) z2[ii[j]] = z[j];
     }
-  }
+  });
 }
 
 // FP32 storage, BF16 MFMA: r -= mu (projection); p update or restart;
@@ -523,20 +540,20 @@ __global__ __launch_bounds__(WG) void kb_pupdate32(BView V, const bf16x8 *__rest
   const int lane = lane_id(), q = lane >> 4, j = lane & 15;
   PFrag P;
   load_pfrag(P, pf, lane);
-  for (int tile = blockIdx.x * WAVES + wave_id(); tile < V.n_tiles; tile += gridDim.x * WAVES) {
-    const int k = V.tile_inst[tile];
-    if (st[k].done) continue;
-    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+  running_tiles(V, st, [&](int4 tm) {
+    const int k = tm.x, b0 = tm.y, tnb = tm.z;
     const double beta = st[k].beta, omega = st[k].omega, mu = st[k].mu;
     const int restart = st[k].restart;
     const bool ok = j < tnb;
     const long base = (long)(b0 + (ok ? j : 0)) * 64;
     float pin[16];
-    mf_pupdate(r, rhat, p, nu, base, q, ok, beta, omega, mu, restart, pin);
+    MfIn in;
+    mf_load(r, p, nu, base, q, in);
+    mf_pupdate(in, r, rhat, p, base, q, ok, beta, omega, mu, restart, pin);
     float z[16];
     prec_mfma(P, pin, z);
     if (ok) st16_out(z1, base, q, z);
-  }
+  });
 }
 
 // FP32 storage, BF16 MFMA: s = r - alpha nu (stored in r); z2 = P s
@@ -547,10 +564,8 @@ __global__ __launch_bounds__(WG) void kb_supdate32(BView V, const bf16x8 *__rest
   const int lane = lane_id(), q = lane >> 4, j = lane & 15;
   PFrag P;
   load_pfrag(P, pf, lane);
-  for (int tile = blockIdx.x * WAVES + wave_id(); tile < V.n_tiles; tile += gridDim.x * WAVES) {
-    const int k = V.tile_inst[tile];
-    if (st[k].done) continue;
-    const int b0 = V.tile_b0[tile], tnb = V.tile_nb[tile];
+  running_tiles(V, st, [&](int4 tm) {
+    const int k = tm.x, b0 = tm.y, tnb = tm.z;
     const double alpha = st[k].alpha;
     const bool ok = j < tnb;
     const long base = (long)(b0 + (ok ? j : 0)) * 64;
@@ -563,7 +578,7 @@ __global__ __launch_bounds__(WG) void kb_supdate32(BView V, const bf16x8 *__rest
     float z[16];
     prec_mfma(P, sv, z);
     if (ok) st16_out(z2, base, q, z);
-  }
+  });
 }
 
 // ----------------------------------------------------------------------------
@@ -711,7 +726,7 @@ struct cup2d_psb {
   // device: work units
   int4 *d_item = nullptr;
   int *d_item_begin = nullptr;
-  int *d_tile_inst = nullptr, *d_tile_b0 = nullptr, *d_tile_nb = nullptr;
+  int4 *d_tile = nullptr;
   double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr;
   // device: vectors and state
   double *d_x = nullptr, *d_xopt = nullptr, *d_b = nullptr, *d_xio = nullptr;
@@ -786,9 +801,7 @@ BView bview(const cup2d_psb *h) {
   V.A.h2 = h->d_h2;
   V.item = h->d_item;
   V.item_begin = h->d_item_begin;
-  V.tile_inst = h->d_tile_inst;
-  V.tile_b0 = h->d_tile_b0;
-  V.tile_nb = h->d_tile_nb;
+  V.tile = h->d_tile;
   V.mean_row = h->d_mean_row;
   V.K = h->K;
   V.n_items = (int)h->item_inst.size();
@@ -839,13 +852,14 @@ int build_units(cup2d_psb *h, int ib) {
   const size_t NI = h->item_inst.size();
   auto &pool = h->unit_allocs;
   int rc;
+  std::vector<int4> tile4(h->tile_inst.size());
+  for (size_t i = 0; i < tile4.size(); i++)
+    tile4[i] = make_int4(h->tile_inst[i], h->tile_b0[i], h->tile_nb[i], 0);
   std::vector<int4> item4(NI);
   for (size_t i = 0; i < NI; i++) item4[i] = make_int4(h->item_inst[i], h->item_b0[i], h->item_nb[i], 0);
   if ((rc = bupload(h, pool, &h->d_item, item4)) ||
       (rc = bupload(h, pool, &h->d_item_begin, h->item_begin)) ||
-      (rc = bupload(h, pool, &h->d_tile_inst, h->tile_inst)) ||
-      (rc = bupload(h, pool, &h->d_tile_b0, h->tile_b0)) ||
-      (rc = bupload(h, pool, &h->d_tile_nb, h->tile_nb)) ||
+      (rc = bupload(h, pool, &h->d_tile, tile4)) ||
       (rc = balloc(h, pool, &h->d_p2, 2 * NI)) || (rc = balloc(h, pool, &h->d_p4, 3 * NI)) ||
       (rc = balloc(h, pool, &h->d_p5, (size_t)NQB * NI)) || (rc = balloc(h, pool, &h->d_pm, 2 * NI)))
     return rc;
@@ -995,7 +1009,10 @@ int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, 
   BCHK(h, hipMemcpyAsync(h->d_x, d_x0, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
   std::vector<int> live(K, 1), iters(K, 0), restarts(K, 0), passes(K, 0), conv(K, 0);
   std::vector<double> err0(K, -1.0), errf(K, 0.0);
-  for (int k = 0; k < K; k++) h->h_st[k] = marker_b(h, k, tol_abs, tol_rel, max_restarts, max_iter);
+  // mixed: a pass ends at inner_tol of its initial residual (0: no limit)
+  const double inner = h->mixed ? h->cfg.inner_tol : 0.0;
+  for (int k = 0; k < K; k++)
+    h->h_st[k] = marker_b(h, k, tol_abs, std::max(tol_rel, inner), max_restarts, max_iter);
   if (!h->mixed) {
     int rc = run_pass<double>(h, d_b, max_iter, -1);
     if (rc) return rc;
@@ -1031,7 +1048,7 @@ int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, 
         // iteration budgets
         const double target = std::max(tol_abs, tol_rel * err0[k]);
         const int budget = (passes[k] >= MAX_PASSES) ? 0 : std::max(0, max_iter - iters[k]);
-        s = marker_b(h, k, target, 0.0, max_restarts, budget);
+        s = marker_b(h, k, target, inner, max_restarts, budget);
         any = 1;
       }
       if (!any) break;
@@ -1087,9 +1104,10 @@ std::map<std::tuple<int, int, long long>, int> g_tuned;
 
 bool config_ok(const cup2d_ps_config &c) {
   return c.max_iter > 0 && c.chunk >= 0 && c.eps >= 0 && c.breakdown_tol >= 0 &&
+         c.inner_tol >= 0 && c.inner_tol < 1 &&
          (c.precision == CUP2D_PS_FP64 || c.precision == CUP2D_PS_MIXED) &&
          (c.norm == CUP2D_PS_NORM_INF || c.norm == CUP2D_PS_NORM_L2) && c.reserved_i[1] >= 0 &&
-         c.reserved_i[1] <= 1024;
+         c.reserved_i[1] <= MAX_ITEM_BLOCKS;
 }
 
 }  // namespace

@@ -419,35 +419,39 @@ __device__ __forceinline__ void st16_out(float *__restrict__ a, long base, int q
 }
 
 // Top-of-iteration update of the lane's 16 input cells of one block, FP32
-// storage with FP64 arithmetic: r -= mu (written back when mu != 0), then
-// p = r + beta (p - omega nu), or rhat = p = r after a restart.  Every load
-// is issued first.  pin receives p; a lane with ok false (past an
-// instance's last block; base is any valid block) stores nothing and gets
-// pin = 0.
-__device__ __forceinline__ void mf_pupdate(float *__restrict__ r, float *__restrict__ rhat,
-                                           float *__restrict__ p, const float *__restrict__ nu,
+// storage with FP64 arithmetic, in two steps so that the loads can be
+// issued early: mf_load reads r, p and nu (p and nu unused after a
+// restart); mf_pupdate then applies r -= mu (written back when mu != 0) and
+// p = r + beta (p - omega nu), or rhat = p = r after a restart.  pin receives
+// p; a lane with ok false (past the last block; base is any valid block)
+// stores nothing and gets pin = 0.
+struct MfIn {
+  float r[16], p[16], nu[16];
+};
+__device__ __forceinline__ void mf_load(const float *__restrict__ r, const float *__restrict__ p,
+                                        const float *__restrict__ nu, long base, int q, MfIn &in) {
+  ld16(r, base, q, in.r);
+  ld16(p, base, q, in.p);
+  ld16(nu, base, q, in.nu);
+}
+__device__ __forceinline__ void mf_pupdate(MfIn &in, float *__restrict__ r,
+                                           float *__restrict__ rhat, float *__restrict__ p,
                                            long base, int q, bool ok, double beta, double omega,
                                            double mu, int restart, float (&pin)[16]) {
-  float rv[16], pv[16] = {}, nv[16] = {};
-  ld16(r, base, q, rv);
-  if (!restart) {
-    ld16(p, base, q, pv);
-    ld16(nu, base, q, nv);
-  }
   if (mu != 0.0)
 #pragma unroll
-    for (int k = 0; k < 16; k++) rv[k] = (float)((double)rv[k] - mu);
+    for (int k = 0; k < 16; k++) in.r[k] = (float)((double)in.r[k] - mu);
   if (restart) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) pin[k] = rv[k];
+    for (int k = 0; k < 16; k++) pin[k] = in.r[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 16; k++)
-      pin[k] = (float)((double)rv[k] + beta * ((double)pv[k] - omega * (double)nv[k]));
+      pin[k] = (float)((double)in.r[k] + beta * ((double)in.p[k] - omega * (double)in.nu[k]));
   }
   if (ok) {
-    if (mu != 0.0) st16_in(r, base, q, rv);
-    if (restart) st16_in(rhat, base, q, rv);
+    if (mu != 0.0) st16_in(r, base, q, in.r);
+    if (restart) st16_in(rhat, base, q, in.r);
     st16_in(p, base, q, pin);
   } else {
 #pragma unroll

@@ -141,9 +141,33 @@ __global__ __launch_bounds__(WG) void k_init(OpView A, int Gb, const double *__r
 // iteration kernels, FP64 (vector-FMA preconditioner, P in LDS)
 // ----------------------------------------------------------------------------
 
+// P (64x64 FP64) into registers, 16 values per thread of the workgroup, so
+// the load is in flight while the scalars are reduced; stored to LDS later.
+__device__ __forceinline__ void prefetch_P(const double *__restrict__ P, double (&pr)[16]) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) pr[k] = P[threadIdx.x + k * WG];
+}
+__device__ __forceinline__ void store_P(double *Ps, const double (&pr)[16]) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) Ps[threadIdx.x + k * WG] = pr[k];
+  __syncthreads();
+}
+
+// The wave's group g of NB blocks (lane = cell): indices and validity.
+template <int NB>
+__device__ __forceinline__ void group_index(int g, int nb, int c, long (&ii)[NB], bool (&ok)[NB]) {
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    ok[j] = g * NB + j < nb;
+    ii[j] = (long)(ok[j] ? g * NB + j : g * NB) * 64 + c;
+  }
+}
+
 // Top of iteration: p = r + beta (p - omega nu) (or p = rhat = r after a
 // restart), z1 = P p (BiCGSTAB.cu:471-520).  The x_opt copy of the
-// iteration's state is made by k_xrupdate, which reads x anyway.
+// iteration's state is made by k_xrupdate, which reads x anyway.  The loads
+// of P and of the wave's first group are issued before the state is
+// evaluated.
 template <int NB>
 __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict__ P,
                                                 const double *__restrict__ r,
@@ -154,30 +178,30 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
   __shared__ Scal cur;
   __shared__ double Ps[64 * 64];
   __shared__ double vs[WAVES][NB * 64];
+  const int c = lane_id(), w = wave_id();
+  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
+  double pr[16];
+  prefetch_P(P, pr);
+  int g = blockIdx.x * WAVES + w;
+  long ii[NB];
+  bool ok[NB];
+  double rv[NB], pv[NB], nv[NB];
+  auto load_group = [&](int gg) {
+    group_index<NB>(gg, nb, c, ii, ok);
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      rv[j] = r[ii[j]];
+      pv[j] = p[ii[j]];
+      nv[j] = nu[ii[j]];
+    }
+  };
+  if (g < ng) load_group(g);
   top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega;
   const int restart = cur.restart;
-  load_P(Ps, P);
-  const int c = lane_id(), w = wave_id();
-  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
-  for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
-    // every load of the group first
-    long ii[NB];
-    bool ok[NB];
-    double rv[NB], pv[NB] = {}, nv[NB] = {};
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      ok[j] = g * NB + j < nb;
-      ii[j] = (long)(ok[j] ? g * NB + j : g * NB) * 64 + c;
-      rv[j] = r[ii[j]];
-    }
-    if (!restart)
-#pragma unroll
-      for (int j = 0; j < NB; j++) {
-        pv[j] = p[ii[j]];
-        nv[j] = nu[ii[j]];
-      }
+  store_P(Ps, pr);
+  for (; g < ng; g += nw) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       const double pi = restart ? rv[j] : rv[j] + beta * (pv[j] - omega * nv[j]);
@@ -194,43 +218,49 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
 #pragma unroll
     for (int j = 0; j < NB; j++)
       if (ok[j]) z1[ii[j]] = z[j];
+    if (g + nw < ng) load_group(g + nw);
   }
 }
 
 // alpha = rho / ((rhat, nu) + eps); s = r - alpha nu (stored in r); z2 = P s
-// (BiCGSTAB.cu:527-545)
+// (BiCGSTAB.cu:527-545).  Loads of P and of the first group first.
 template <int NB>
 __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict__ P,
                                                 double *__restrict__ r, double *__restrict__ nu,
                                                 const double *__restrict__ rhat,
                                                 double *__restrict__ z2, Scal *st, int par,
                                                 const double *p2, int G2, long mean_row) {
-  if (st[par].done) return;
   __shared__ double Ps[64 * 64];
   __shared__ double vs[WAVES][NB * 64];
+  const int c = lane_id(), w = wave_id();
+  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
+  double pr[16];
+  prefetch_P(P, pr);
+  int g = blockIdx.x * WAVES + w;
+  long ii[NB];
+  bool ok[NB];
+  double rv[NB], nv[NB];
+  auto load_group = [&](int gg) {
+    group_index<NB>(gg, nb, c, ii, ok);
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      rv[j] = r[ii[j]];
+      nv[j] = nu[ii[j]];
+    }
+  };
+  if (g < ng) load_group(g);
+  const double rh_mr = mean_row >= 0 ? rhat[mean_row] : 0.0;
   double o[2];
   load_totals<2, 0u>(p2, G2, o);
-  double rn = o[0];
-  if (mean_row >= 0) rn += rhat[mean_row] * o[1];
+  if (st[par].done) return;
+  const double rn = o[0] + (mean_row >= 0 ? rh_mr * o[1] : 0.0);
   const double alpha = st[par].rho / (rn + st[par].eps);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[par].alpha = alpha;
     st[par].red[2] = rn;
   }
-  load_P(Ps, P);
-  const int c = lane_id(), w = wave_id();
-  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
-  for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
-    long ii[NB];
-    bool ok[NB];
-    double rv[NB], nv[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      ok[j] = g * NB + j < nb;
-      ii[j] = (long)(ok[j] ? g * NB + j : g * NB) * 64 + c;
-      rv[j] = r[ii[j]];
-      nv[j] = nu[ii[j]];
-    }
+  store_P(Ps, pr);
+  for (; g < ng; g += nw) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       if (ii[j] == mean_row) nv[j] = o[1];
@@ -248,6 +278,7 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
 #pragma unroll
     for (int j = 0; j < NB; j++)
       if (ok[j]) z2[ii[j]] = z[j];
+    if (g + nw < ng) load_group(g + nw);
   }
 }
 
@@ -257,7 +288,8 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
 
 // Top of iteration (FP32 storage): r -= mu (null-space projection),
 // p = r + beta (p - omega nu) or p = rhat = r after a restart, z1 = P p on
-// MFMA   (BiCGSTAB.cu:471-520)
+// MFMA   (BiCGSTAB.cu:471-520).  The P fragments and the wave's first tile are
+// loaded before the state is evaluated.
 __global__ __launch_bounds__(WG) void k_pupdate_mf(int nb, const bf16x8 *__restrict__ pf,
                                                    float *__restrict__ r,
                                                    float *__restrict__ rhat, float *__restrict__ p,
@@ -265,50 +297,67 @@ __global__ __launch_bounds__(WG) void k_pupdate_mf(int nb, const bf16x8 *__restr
                                                    float *__restrict__ z1, Scal *st, int par,
                                                    const double *p5, int G5) {
   __shared__ Scal cur;
+  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
+  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  int tile = blockIdx.x * WAVES + w;
+  bool ok = false;
+  long base = 0;
+  MfIn in;
+  auto load_tile = [&](int t) {
+    ok = t * TILE + j < nb;
+    base = (long)(ok ? t * TILE + j : t * TILE) * 64;
+    mf_load(r, p, nu, base, q, in);
+  };
+  if (tile < ntiles) load_tile(tile);
   top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega, mu = cur.mu;
   const int restart = cur.restart;
-  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
-  PFrag P;
-  load_pfrag(P, pf, lane);
-  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
-  for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
-    const bool ok = tile * TILE + j < nb;
-    const long base = (long)(ok ? tile * TILE + j : tile * TILE) * 64;
+  for (; tile < ntiles; tile += nw) {
     float pin[16];
-    mf_pupdate(r, rhat, p, nu, base, q, ok, beta, omega, mu, restart, pin);
+    mf_pupdate(in, r, rhat, p, base, q, ok, beta, omega, mu, restart, pin);
     float z[16];
     prec_mfma(P, pin, z);
     if (ok) st16_out(z1, base, q, z);
+    if (tile + nw < ntiles) load_tile(tile + nw);
   }
 }
 
+// alpha = rho / ((rhat, nu) + eps); s = r - alpha nu (stored in r); z2 = P s
+// on MFMA (BiCGSTAB.cu:527-545).  Loads first.
 __global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restrict__ pf,
                                                    float *__restrict__ r, float *__restrict__ nu,
                                                    const float *__restrict__ rhat,
                                                    float *__restrict__ z2, Scal *st, int par,
                                                    const double *p2, int G2, long mean_row) {
-  if (st[par].done) return;
+  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
+  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
+  PFrag P;
+  load_pfrag(P, pf, lane);
+  int tile = blockIdx.x * WAVES + w;
+  bool ok = false;
+  long base = 0;
+  float rv[16], nv[16];
+  auto load_tile = [&](int t) {
+    ok = t * TILE + j < nb;
+    base = (long)(ok ? t * TILE + j : t * TILE) * 64;
+    ld16(r, base, q, rv);
+    ld16(nu, base, q, nv);
+  };
+  if (tile < ntiles) load_tile(tile);
+  const double rh_mr = mean_row >= 0 ? (double)rhat[mean_row] : 0.0;
   double o[2];
   load_totals<2, 0u>(p2, G2, o);
-  double rn = o[0];
-  if (mean_row >= 0) rn += (double)rhat[mean_row] * o[1];
+  if (st[par].done) return;
+  const double rn = o[0] + (mean_row >= 0 ? rh_mr * o[1] : 0.0);
   const double alpha = st[par].rho / (rn + st[par].eps);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[par].alpha = alpha;
     st[par].red[2] = rn;
   }
-  const int lane = lane_id(), w = wave_id(), q = lane >> 4, j = lane & 15;
-  PFrag P;
-  load_pfrag(P, pf, lane);
-  const int ntiles = (nb + TILE - 1) / TILE, nw = gridDim.x * WAVES;
-  for (int tile = blockIdx.x * WAVES + w; tile < ntiles; tile += nw) {
-    const bool ok = tile * TILE + j < nb;
-    const long base = (long)(ok ? tile * TILE + j : tile * TILE) * 64;
-    float rv[16], nv[16], sv[16];
-    ld16(r, base, q, rv);
-    ld16(nu, base, q, nv);
+  for (; tile < ntiles; tile += nw) {
     if (ok && mean_row >= base && mean_row < base + 64) {
 #pragma unroll
       for (int k = 0; k < 16; k++)
@@ -317,12 +366,14 @@ __global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restr
           nu[mean_row] = nv[k];
         }
     }
+    float sv[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) sv[k] = ok ? (float)((double)rv[k] - alpha * (double)nv[k]) : 0.f;
     if (ok) st16_in(r, base, q, sv);
     float z[16];
     prec_mfma(P, sv, z);
     if (ok) st16_out(z2, base, q, z);
+    if (tile + nw < ntiles) load_tile(tile + nw);
   }
 }
 
@@ -343,7 +394,7 @@ template <class T>
 __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__restrict__ z1,
                                                 T *__restrict__ nu, const T *__restrict__ rhat,
                                                 const Scal *st, int par, double *p2) {
-  if (st[par].done) return;
+  const int done = st[par].done;  // checked once the first loads are issued
   double v[2] = {0.0, 0.0};
   auto store_y = [&](long i, double y, double rh) {
     if (i == A.mean_row) y = 0.0;  // replaced by the consumer
@@ -363,6 +414,7 @@ __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__res
       }
       BlockRows<SPMV_U, T> R;
       R.load(A, z1, rhat, bl, c);
+      if (done) return;
 #pragma unroll
       for (int u = 0; u < SPMV_U; u++) {
         if (!ok[u]) continue;
@@ -374,21 +426,26 @@ __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__res
     }
   } else {
     const int ge = gridDim.x - Gb;
+    if (done) return;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
       const int row = A.erow[e];
       store_y(row, explicit_row<T>(A, z1, e), (double)rhat[row]);
     }
   }
+  if (done) return;
   store_partials<2, 0u>(v, p2);
 }
 
-// t = A z2; partials of (t, s), (t, t) and sum h^2 z2   (BiCGSTAB.cu:548-553)
+// t = A z2; partials of (t, s), (t, t), sum h^2 z2 and, from workgroup 0,
+// s at the mean row (BiCGSTAB.cu:548-553).  s is stable during this kernel,
+// unlike in k_xrupdate, which overwrites it.
 template <class T>
 __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__restrict__ z2,
                                                T *__restrict__ t, const T *__restrict__ sv,
                                                const Scal *st, int par, double *p4) {
-  if (st[par].done) return;
-  double v[3] = {0.0, 0.0, 0.0};
+  const int done = st[par].done;  // checked once the first loads are issued
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (A.mean_row >= 0 && blockIdx.x == 0 && threadIdx.x == 0) v[3] = (double)sv[A.mean_row];
   auto store_y = [&](long i, double y, double si) {
     if (i == A.mean_row) y = 0.0;
     const T yt = (T)y;
@@ -409,6 +466,7 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__rest
       }
       BlockRows<SPMV_U, T> R;
       R.load(A, z2, sv, bl, c);
+      if (done) return;
 #pragma unroll
       for (int u = 0; u < SPMV_U; u++) {
         if (!ok[u]) continue;
@@ -420,12 +478,14 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__rest
     }
   } else {
     const int ge = gridDim.x - Gb;
+    if (done) return;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
       const int row = A.erow[e];
       store_y(row, explicit_row<T>(A, z2, e), (double)sv[row]);
     }
   }
-  store_partials<3, 0u>(v, p4);
+  if (done) return;
+  store_partials<4, 0u>(v, p4);
 }
 
 // omega = (t, s) / ((t, t) + eps); x_opt = x when the previous iterate was
@@ -440,12 +500,18 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
                                                  const T *__restrict__ rhat, Scal *st, int par,
                                                  const double *p4, int G4, double *p5,
                                                  long mean_row) {
+  const long n = (long)nb * 64;
+  // the first cell's loads are issued before the scalars are reduced
+  long i = (long)blockIdx.x * WG + threadIdx.x;
+  const long i0 = i < n ? i : 0;
+  double xo = x[i0], a1 = (double)z1[i0], a2 = (double)z2[i0], rr = (double)r[i0],
+         ti = (double)t[i0], rh = (double)rhat[i0];
+  double o[4];
+  load_totals<4, 0u>(p4, G4, o);
   if (st[par].done) return;
-  double o[3];
-  load_totals<3, 0u>(p4, G4, o);
   double ts = o[0], tt = o[1];
   if (mean_row >= 0) {
-    ts += o[2] * (double)r[mean_row];
+    ts += o[2] * o[3];  // t and s at the mean row
     tt += o[2] * o[2];
   }
   const double omega = ts / (tt + st[par].eps);
@@ -456,22 +522,31 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
     st[par].red[3] = ts;
     st[par].red[4] = tt;
   }
-  const long n = (long)nb * 64;
   double v[NQ5] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
-    const double xo = x[i];
+  const long stride = (long)gridDim.x * WG;
+  for (; i < n; i += stride) {
     if (improved) xopt[i] = xo;
-    x[i] = xo + alpha * (double)z1[i] + omega * (double)z2[i];
-    const double ti = (i == mean_row) ? o[2] : (double)t[i];
-    const T rt = (T)((double)r[i] - omega * ti);
+    x[i] = xo + alpha * a1 + omega * a2;
+    if (i == mean_row) ti = o[2];
+    const T rt = (T)(rr - omega * ti);
     r[i] = rt;
-    const double rd = (double)rt, rh = (double)rhat[i];
+    const double rd = (double)rt;
+    const double rhi = rh;
+    if (i + stride < n) {
+      const long in = i + stride;
+      xo = x[in];
+      a1 = (double)z1[in];
+      a2 = (double)z2[in];
+      rr = (double)r[in];
+      ti = (double)t[in];
+      rh = (double)rhat[in];
+    }
     v[0] = nmax(v[0], fabs(rd));
     v[1] += rd * rd;
-    v[2] += rh * rd;
-    v[3] += rh * rh;
+    v[2] += rhi * rd;
+    v[3] += rhi * rhi;
     v[4] += rd;
-    v[5] += rh;
+    v[5] += rhi;
   }
   store_partials<NQ5, 1u>(v, p5);
 }
@@ -961,9 +1036,10 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
       break;
     }
     if (outer >= max_outer || total_iters >= h->cfg.max_iter) break;
-    // iterate towards the full remaining reduction; the FP64 check above
-    // decides whether another pass is needed
-    const double rel = tol_rel > 0 ? tol_rel * err0 / err : 0.0;
+    // iterate towards the remaining reduction, at most inner_tol of this
+    // pass's initial residual (0: no limit); the FP64 check above decides
+    // whether another pass is needed
+    const double rel = std::max(tol_rel > 0 ? tol_rel * err0 / err : 0.0, h->cfg.inner_tol);
     h->h_s[0] = h->h_s[1] = marker_state(h, tol_abs, rel, max_restarts, h->cfg.max_iter - total_iters);
     HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
     if ((rc = run_iterations(h, h->cfg.max_iter - total_iters))) return rc;
@@ -1017,7 +1093,7 @@ void cup2d_ps_config_default(cup2d_ps_config *cfg) {
   cfg->use_graph = 1;
   cfg->eps = 1e-100;
   cfg->breakdown_tol = 1e-16;
-  cfg->inner_tol = 1e-4;
+  cfg->inner_tol = 0.0;
 }
 
 int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
@@ -1026,7 +1102,7 @@ int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
   cup2d_ps_config c;
   if (cfg) c = *cfg;
   else cup2d_ps_config_default(&c);
-  if (c.max_iter <= 0 || c.chunk < 0 || c.eps < 0 || c.breakdown_tol < 0 || c.inner_tol <= 0 ||
+  if (c.max_iter <= 0 || c.chunk < 0 || c.eps < 0 || c.breakdown_tol < 0 || c.inner_tol < 0 ||
       c.inner_tol >= 1 || (c.precision != CUP2D_PS_FP64 && c.precision != CUP2D_PS_MIXED) ||
       (c.norm != CUP2D_PS_NORM_INF && c.norm != CUP2D_PS_NORM_L2))
     return fail(nullptr, CUP2D_PS_EINVAL, "invalid configuration");
@@ -1108,7 +1184,7 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
     if ((rc = dalloc(h, v, h->mixed ? n : 64))) return rc;
   const size_t GT = MAXG + MAXGE;
   if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_p2, 2 * GT)) ||
-      (rc = dalloc(h, &h->d_p4, 3 * GT)) || (rc = dalloc(h, &h->d_p5, NQ5 * GT)) ||
+      (rc = dalloc(h, &h->d_p4, 4 * GT)) || (rc = dalloc(h, &h->d_p5, NQ5 * GT)) ||
       (rc = dalloc(h, &h->d_pm, GT)) || (rc = dalloc(h, &h->d_pfin, 4 * GT)))
     return rc;
   HIPCHK(h, hipMemset(h->d_s, 0, 2 * sizeof(Scal)));

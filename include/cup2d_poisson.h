@@ -68,8 +68,10 @@ typedef struct {
   double eps;             /* denominator guard; reference GPU 1e-21 (BiCGSTAB.cu:427),
                              reference CPU 1e-100 (AMRSolver.cpp:167) */
   double breakdown_tol;   /* restart when rho^2 < breakdown_tol |r|^2 |rhat|^2 (BiCGSTAB.cu:483) */
-  double inner_tol;       /* mixed precision: relative reduction of the FP32 recurrence residual
-                             that triggers an FP64 residual update */
+  double inner_tol;       /* mixed precision: an FP32 inner pass ends when its recurrence residual
+                             has dropped to inner_tol times the pass's initial (FP64) residual,
+                             and the next pass restarts from the FP64 residual; 0 = no limit
+                             (a pass runs until it stops improving) */
   double reserved_d[5];
 } cup2d_ps_config;
 
@@ -85,7 +87,7 @@ typedef struct {
 } cup2d_ps_stats;
 
 /* Fills the defaults: FP64, mean_constraint 0, NORM_INF, max_iter 1000,
- * eps 1e-100, breakdown_tol 1e-16, graph replay on. */
+ * eps 1e-100, breakdown_tol 1e-16, inner_tol 0, graph replay on. */
 void cup2d_ps_config_default(cup2d_ps_config *cfg);
 
 /* Creates a solver (reference: ExpAMRSolver ctor, ExpAMRSolver.cpp:25-99, which
@@ -168,7 +170,8 @@ const char *cup2d_ps_last_error(const cup2d_ps *h);
  * restarts and (mixed precision) FP64 refinement passes.  Vectors are the
  * instances' vectors concatenated in instance order (offsets from
  * cup2d_psb_offsets).  Config fields as for cup2d_ps_create; reserved_i[1]
- * sets the blocks per work item (0: tuned value of this process, else 8).
+ * sets the blocks per work item, 1..256 (0: tuned value of this process,
+ * else 8).
  * ------------------------------------------------------------------------ */
 typedef struct cup2d_psb cup2d_psb;
 
