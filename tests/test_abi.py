@@ -86,3 +86,34 @@ def test_factory_strings():
     sim = PS.SimulationData(poissonSolver="bogus")
     with pytest.raises(ValueError):
         PS.makePoissonSolver(sim)
+
+
+def test_batch_create_validates_config_before_device():
+    """Invalid configurations are rejected with EINVAL before any device is
+    touched; a valid one fails here with ENODEV (no GPU), never crashes."""
+    import subprocess
+    import sys
+    code = ("import sys, ctypes; sys.path.insert(0, %r)\n"
+            "from cup2d_amd import poisson as PS, _lib\n"
+            "lib = _lib.load()\n"
+            "def rc(**kw):\n"
+            "    c = PS.default_config()\n"
+            "    for k, v in kw.items():\n"
+            "        if k == 'ib': c.reserved_i[1] = v\n"
+            "        else: setattr(c, k, v)\n"
+            "    h = ctypes.c_void_p()\n"
+            "    return lib.cup2d_psb_create(ctypes.byref(c), ctypes.byref(h))\n"
+            "print(rc(max_iter=0), rc(inner_tol=-1.0), rc(inner_tol=1.0), rc(ib=257), rc(ib=-1),"
+            " rc(precision=7), rc(norm=3), rc(ib=256), rc())\n") % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    vals = [int(v) for v in r.stdout.split()]
+    assert vals[:7] == [1] * 7, r.stdout + r.stderr
+    assert all(v in (0, 5) for v in vals[7:]), vals  # OK on a GPU box, ENODEV here
+
+
+def test_batch_null_handle_is_einval():
+    lib = _lib.load()
+    assert lib.cup2d_psb_solve(None, None, None, 0.0, 1e-10, 10, None) == 1
+    assert lib.cup2d_psb_set_meshes(None, 1, None) == 1
+    assert lib.cup2d_psb_offsets(None, None) == 1
+    assert lib.cup2d_psb_destroy(None) == 0
