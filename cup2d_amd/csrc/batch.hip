@@ -47,7 +47,6 @@ namespace {
 constexpr int NQB = 7;        // update totals (NQ5) + sum h^2 x (initial mean row)
 constexpr int BGRID = 2048;   // workgroups of the item / tile kernels (grid-stride beyond)
 constexpr int MAX_INST = 4096;
-constexpr int MAX_PASSES = 16;  // FP64 refinement passes per instance (mixed)
 constexpr int MAX_ITEM_BLOCKS = 256;  // item_blocks keeps one block header per lane: 64 * WAVES
 
 struct BView {
@@ -120,11 +119,23 @@ __device__ void range_totals(const double *__restrict__ part, int NI, int i0, in
 constexpr int UB = 4;  // blocks per wavefront in flight
 
 // Calls body(item, meta) for the workgroup's items (grid-stride) whose
-// instance is running (every item when st is null); the next item's meta is
-// loaded while the current one is processed.  body may synchronize the
-// workgroup (the skip is uniform).
+// instance passes `filter` (every item when st is null); the next item's
+// meta is loaded while the current one is processed.  body may synchronize
+// the workgroup (the skip is uniform).
+// Which instances a kernel works on.
+enum Filter { F_ALL, F_RUNNING, F_LIVE, F_REPLACE };
+__device__ __forceinline__ bool wanted(const Scal &s, int f) {
+  switch (f) {
+    case F_RUNNING: return !s.idle;                 // iterating
+    case F_LIVE: return !s.done;                    // not finished
+    case F_REPLACE: return s.replace && !s.done;    // replacement pending
+    default: return true;
+  }
+}
+
 template <class F>
-__device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body) {
+__device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body,
+                                              int filter = F_RUNNING) {
   int item = blockIdx.x;
   if (item >= V.n_items) return;
   int4 mt = V.item[item];
@@ -132,7 +143,7 @@ __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F 
     const int next = item + (int)gridDim.x;
     const bool more = next < V.n_items;
     const int4 nmt = V.item[more ? next : item];
-    if (!st || !st[mt.x].done) body(item, mt);
+    if (!st || wanted(st[mt.x], filter)) body(item, mt);
     if (!more) break;
     item = next;
     mt = nmt;
@@ -179,14 +190,16 @@ __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict
 }
 
 // r = b - A x (FP64), stored as T to r and rhat; p = nu = 0; x_opt = x;
-// partials of the update totals and of sum h^2 x.  The mean row's residual
-// is finished by kb_eval.
+// partials of the update totals and of sum h^2 x, for the instances that
+// pass `filter` (the start of a solve, or a residual replacement).  The
+// mean row's residual is finished by kb_eval.
 template <class T>
 __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict__ b,
                                               const double *__restrict__ x,
                                               double *__restrict__ xopt, T *__restrict__ r,
                                               T *__restrict__ rhat, T *__restrict__ p,
-                                              T *__restrict__ nu, const Scal *st, double *p5) {
+                                              T *__restrict__ nu, const Scal *st, double *p5,
+                                              int filter) {
   const int c = lane_id();
   running_items(V, st, [&](int item, int4 mt) {
     const long long mr = V.mean_row[mt.x];
@@ -220,7 +233,7 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
     v[3] = v[1];
     v[5] = v[4];
     store_item_partials<NQB, 1u>(v, p5, V.n_items, item);
-  });
+  }, filter);
 }
 
 // nu = A z1; partials of (rhat, nu) and sum h^2 z1
@@ -293,7 +306,7 @@ __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict_
   running_items(V, nullptr, [&](int item, int4 mt) {
     double v[1] = {0.0};
     if (mt.z > 0) {
-      item_blocks<double>(V, x, (const double *)nullptr, mt.y, mt.z,
+      item_blocks<double>(V, x, x, mt.y, mt.z,
                           [&](const BlockRows<UB, double> &R, int u, int) {
                             bool plain;
                             const double yi = R.y(u, c, plain);
@@ -375,12 +388,14 @@ __device__ __forceinline__ void item_cells(const int4 &mt, long &i0, long &i1) {
   i1 = i0 + (mt.z > 0 ? (long)mt.z * 64 : 0);
 }
 
-// x = x_opt on the instances whose last iterate is not their best
+// x = x_opt on the instances (passing `filter`) whose last iterate is not
+// their best
 __global__ __launch_bounds__(WG) void kb_select(BView V, double *__restrict__ x,
-                                                const double *__restrict__ xopt, const Scal *st) {
+                                                const double *__restrict__ xopt, const Scal *st,
+                                                int filter) {
   for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
     const int4 mt = V.item[item];
-    if (st[mt.x].improved) continue;
+    if (st[mt.x].improved || !wanted(st[mt.x], filter)) continue;
     long i0, i1;
     item_cells(mt, i0, i1);
     for (long i = i0 + threadIdx.x; i < i1; i += WG) x[i] = xopt[i];
@@ -419,8 +434,8 @@ __global__ __launch_bounds__(WG) void kb_fwrite(BView V, const double *__restric
 // ----------------------------------------------------------------------------
 
 // Calls body(meta) for the wave's tiles (grid-stride over waves) whose
-// instance is running; the next tile's meta is loaded while the current one
-// is processed.
+// instance is iterating; the next tile's meta is loaded while the current
+// one is processed.
 template <class F>
 __device__ __forceinline__ void running_tiles(const BView &V, const Scal *st, F &&body) {
   const int stride = gridDim.x * WAVES;
@@ -431,7 +446,7 @@ __device__ __forceinline__ void running_tiles(const BView &V, const Scal *st, F 
     const int next = tile + stride;
     const bool more = next < V.n_tiles;
     const int4 ntm = V.tile[more ? next : tile];
-    if (!st[tm.x].done) body(tm);
+    if (!st[tm.x].idle) body(tm);
     if (!more) break;
     tile = next;
     tm = ntm;
@@ -591,7 +606,7 @@ template <class T>
 __global__ __launch_bounds__(WG) void kb_alpha(BView V, Scal *st, const double *p2,
                                                T *__restrict__ nu, const T *__restrict__ rhat) {
   const int k = blockIdx.x;
-  if (st[k].done) return;
+  if (st[k].idle) return;
   double o[2];
   range_totals<2, 0u>(p2, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
   if (threadIdx.x == 0) {
@@ -613,7 +628,7 @@ template <class T>
 __global__ __launch_bounds__(WG) void kb_omega(BView V, Scal *st, const double *p4,
                                                T *__restrict__ t, const T *__restrict__ sv) {
   const int k = blockIdx.x;
-  if (st[k].done) return;
+  if (st[k].idle) return;
   double o[3];
   range_totals<3, 0u>(p4, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
   if (threadIdx.x == 0) {
@@ -632,18 +647,20 @@ __global__ __launch_bounds__(WG) void kb_omega(BView V, Scal *st, const double *
   }
 }
 
-// Next state of every unfinished instance; after kb_init, its initial state
-// (the mean row of r0 is completed here: b = 0 on that row,
-// ExpAMRSolver.cpp:340-345, and (A x)_row = sum h^2 x).  `init_check`: an
-// initial residual already within tolerance ends the instance (the FP64
-// residual check of a refinement pass).  A non-finite initial residual ends
-// it unconverged.
+// Instance states from the update partials.  mode 0: end of an iteration
+// (iterating instances); mode 1: after kb_init at the start of a solve
+// (unfinished instances: their initial state); mode 2: after kb_init of a
+// residual replacement (instances with one pending: restart from it).  In
+// modes 1 and 2 the mean row of r is completed here: b = 0 on that row
+// (ExpAMRSolver.cpp:340-345) and (A x)_row = sum h^2 x.  `init_check`
+// (mode 1): an initial residual already within tolerance ends the instance.
+// A non-finite initial residual ends it unconverged.
 template <class T>
 __global__ __launch_bounds__(WG) void kb_eval(BView V, Scal *st, const double *p5,
-                                              T *__restrict__ r, T *__restrict__ rhat,
+                                              T *__restrict__ r, T *__restrict__ rhat, int mode,
                                               int init_check) {
   const int k = blockIdx.x;
-  if (st[k].done) return;
+  if (!wanted(st[k], mode == 0 ? F_RUNNING : (mode == 1 ? F_LIVE : F_REPLACE))) return;
   double o[NQB];
   range_totals<NQB, 1u>(p5, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
   if (threadIdx.x == 0) {
@@ -651,7 +668,7 @@ __global__ __launch_bounds__(WG) void kb_eval(BView V, Scal *st, const double *p
     double oo[NQ5];
     for (int q = 0; q < NQ5; q++) oo[q] = o[q];
     const long long mr = V.mean_row[k];
-    if (prev.iter < 0 && mr >= 0) {
+    if (mode != 0 && mr >= 0) {
       const T rm = (T)(0.0 - o[6]);
       r[mr] = rm;
       rhat[mr] = rm;
@@ -663,14 +680,12 @@ __global__ __launch_bounds__(WG) void kb_eval(BView V, Scal *st, const double *p
       oo[4] += rd;
       oo[5] += rd;
     }
-    Scal s = evaluate(prev, oo);
-    if (prev.iter < 0) {
-      if (s.err != s.err || s.err > 1.79e308) {
-        s.done = 1;
-      } else if (s.err == 0.0 || (init_check && (s.err <= s.tol_abs || s.tol_rel >= 1.0))) {
-        s.converged = 1;
-        s.done = 1;
-      }
+    Scal s = evaluate(prev, oo, mode == 2);
+    if (mode != 0 && (s.err != s.err || s.err > 1.79e308)) {
+      s.done = s.idle = 1;
+      s.converged = 0;
+    } else if (mode == 1 && (s.err == 0.0 || (init_check && (s.err <= s.tol_abs || s.tol_rel >= 1.0)))) {
+      s.converged = s.done = s.idle = 1;
     }
     st[k] = s;
   }
@@ -866,7 +881,7 @@ int build_units(cup2d_psb *h, int ib) {
   return CUP2D_PS_OK;
 }
 
-// One BiCGSTAB iteration of every unfinished instance: 8 kernels.
+// One BiCGSTAB iteration of every iterating instance: 8 kernels.
 template <class T>
 void launch_iteration_b(cup2d_psb *h) {
   const BView V = bview(h);
@@ -888,12 +903,28 @@ void launch_iteration_b(cup2d_psb *h) {
   kb_spmv_t<T><<<Gi, WG, 0, s>>>(V, z2, t, r, st, h->d_p4);
   kb_omega<T><<<K, WG, 0, s>>>(V, st, h->d_p4, t, r);
   kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, z1, z2, r, t, rhat, st, h->d_p5);
-  kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0);
+  kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0, 0);
 }
 
 int chunk_b(const cup2d_psb *h) { return h->cfg.chunk > 0 ? h->cfg.chunk : 8; }
 
+// Residual replacement step (mixed precision): the instances with one
+// pending take their best iterate, recompute r = b - A x in FP64 from it and
+// restart.
+void launch_replacement_b(cup2d_psb *h) {
+  const BView V = bview(h);
+  const int Gi = grid_items(h);
+  hipStream_t s = h->stream;
+  float *r = (float *)h->v_r, *rhat = (float *)h->v_rhat, *p = (float *)h->v_p,
+        *nu = (float *)h->v_nu;
+  kb_select<<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, h->d_st, F_REPLACE);
+  kb_init<float><<<Gi, WG, 0, s>>>(V, h->d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5,
+                                   F_REPLACE);
+  kb_eval<float><<<h->K, WG, 0, s>>>(V, h->d_st, h->d_p5, r, rhat, 2, 0);
+}
+
 void launch_chunk_b(cup2d_psb *h, int C) {
+  if (h->mixed) launch_replacement_b(h);
   for (int i = 0; i < C; i++) {
     if (h->mixed) launch_iteration_b<float>(h);
     else launch_iteration_b<double>(h);
@@ -946,7 +977,9 @@ int run_chunks_b(cup2d_psb *h, int max_iter, int max_chunks) {
     int done = 0;
     int rc = read_all_done(h, done);
     if (rc) return rc;
-    if (done || launched > max_iter + C) break;
+    // every instance stops at its own iteration cap; pauses for residual
+    // replacement skip iterations, hence the margin
+    if (done || launched > 2 * max_iter + 2 * C) break;
     if (max_chunks > 0 && chunks >= max_chunks) break;
   }
   return CUP2D_PS_OK;
@@ -957,7 +990,13 @@ Scal marker_b(const cup2d_psb *h, int k, double tol_abs, double tol_rel, int max
   Scal s{};
   s.n = 64.0 * double(h->boff[k + 1] - h->boff[k]);
   s.project = (h->mixed && h->h_mean_row[k] < 0) ? 1 : 0;
-  s.stall = h->mixed ? 150 : 0;
+  if (h->mixed) {
+    // FP32-stored residual: FP64 residual replacement and restart from the
+    // best iterate after each reduction by inner_tol (default 1e-2) of the
+    // recurrence residual, or a stall of 150 iterations
+    s.rep_factor = h->cfg.inner_tol > 0 ? h->cfg.inner_tol : 1e-2;
+    s.stall = 150;
+  }
   s.tol_abs = tol_abs;
   s.tol_rel = tol_rel;
   s.eps = h->cfg.eps;
@@ -979,8 +1018,9 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
   const int Gi = grid_items(h), K = h->K;
   T *r = (T *)h->v_r, *rhat = (T *)h->v_rhat, *p = (T *)h->v_p, *nu = (T *)h->v_nu;
   BCHK(h, hipMemcpyAsync(h->d_st, h->h_st.data(), K * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
-  kb_init<T><<<Gi, WG, 0, h->stream>>>(V, d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5);
-  kb_eval<T><<<K, WG, 0, h->stream>>>(V, h->d_st, h->d_p5, r, rhat, sizeof(T) == 4 ? 1 : 0);
+  kb_init<T><<<Gi, WG, 0, h->stream>>>(V, d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5,
+                                       F_LIVE);
+  kb_eval<T><<<K, WG, 0, h->stream>>>(V, h->d_st, h->d_p5, r, rhat, 1, sizeof(T) == 4 ? 1 : 0);
   kb_alldone<<<1, WG, 0, h->stream>>>(K, h->d_st, h->d_all_done);
   BCHK(h, hipGetLastError());
   int done = 0;
@@ -989,17 +1029,17 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
   if (!done && max_iter > 0) {
     if ((rc = run_chunks_b(h, max_iter, max_chunks))) return rc;
   }
-  kb_select<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_xopt, h->d_st);
+  kb_select<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_xopt, h->d_st, F_ALL);
   BCHK(h, hipGetLastError());
   BCHK(h, hipMemcpyAsync(h->h_st.data(), h->d_st, K * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
   BCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
 }
 
-// FP64: one pass.  Mixed: per-instance FP64 refinement -- each pass starts
-// from the FP64 residual of the instance's current x, an instance whose FP64
-// residual meets its target (or that cannot progress) leaves the batch, the
-// others iterate again towards the full remaining reduction.
+// One BiCGSTAB run per instance from r0 = b - A x0 (FP64).  Mixed precision:
+// FP32-stored Krylov vectors with FP64 residual replacement and restart
+// (launch_replacement_b at the start of every chunk), so the stopping test is
+// met on an FP64 residual.
 int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
             double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
   if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "solve before cup2d_psb_set_meshes");
@@ -1007,54 +1047,23 @@ int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, 
   const int K = h->K, max_iter = h->cfg.max_iter;
   BCHK(h, hipEventRecord(h->ev0, h->stream));
   BCHK(h, hipMemcpyAsync(h->d_x, d_x0, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-  std::vector<int> live(K, 1), iters(K, 0), restarts(K, 0), passes(K, 0), conv(K, 0);
+  if (d_b != h->d_b)  // the replacement step of the captured graph reads the handle's copy
+    BCHK(h, hipMemcpyAsync(h->d_b, d_b, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  std::vector<int> iters(K, 0), restarts(K, 0), reps(K, 0), conv(K, 0);
   std::vector<double> err0(K, -1.0), errf(K, 0.0);
-  // mixed: a pass ends at inner_tol of its initial residual (0: no limit)
-  const double inner = h->mixed ? h->cfg.inner_tol : 0.0;
-  for (int k = 0; k < K; k++)
-    h->h_st[k] = marker_b(h, k, tol_abs, std::max(tol_rel, inner), max_restarts, max_iter);
-  if (!h->mixed) {
-    int rc = run_pass<double>(h, d_b, max_iter, -1);
-    if (rc) return rc;
-    for (int k = 0; k < K; k++) {
-      const Scal &s = h->h_st[k];
-      iters[k] = std::max(s.iter, 0);
-      restarts[k] = s.restarts;
-      conv[k] = s.converged;
-      err0[k] = s.err0;
-      errf[k] = s.iter > 0 ? s.err_opt : s.err0;
-    }
-  } else {
-    for (;;) {
-      int rc = run_pass<float>(h, d_b, max_iter, -1);
-      if (rc) return rc;
-      int any = 0;
-      for (int k = 0; k < K; k++) {
-        if (!live[k]) continue;
-        Scal &s = h->h_st[k];
-        if (err0[k] < 0) err0[k] = s.err0;
-        if (s.iter <= 0) {
-          // decided by the pass's FP64 residual check
-          live[k] = 0;
-          conv[k] = s.converged;
-          errf[k] = s.err0;
-          continue;
-        }
-        iters[k] += s.iter;
-        restarts[k] += s.restarts;
-        passes[k]++;
-        // next pass: check the FP64 residual of the new x against the
-        // caller's target; iterate again only within the pass and
-        // iteration budgets
-        const double target = std::max(tol_abs, tol_rel * err0[k]);
-        const int budget = (passes[k] >= MAX_PASSES) ? 0 : std::max(0, max_iter - iters[k]);
-        s = marker_b(h, k, target, inner, max_restarts, budget);
-        any = 1;
-      }
-      if (!any) break;
-      for (int k = 0; k < K; k++)
-        if (!live[k]) h->h_st[k].done = 1;
-    }
+  for (int k = 0; k < K; k++) h->h_st[k] = marker_b(h, k, tol_abs, tol_rel, max_restarts, max_iter);
+  int rc = h->mixed ? run_pass<float>(h, h->d_b, max_iter, -1) : run_pass<double>(h, h->d_b, max_iter, -1);
+  if (rc) return rc;
+  for (int k = 0; k < K; k++) {
+    const Scal &s = h->h_st[k];
+    iters[k] = std::max(s.iter, 0);
+    restarts[k] = s.restarts;
+    reps[k] = s.n_rep;
+    conv[k] = s.converged;
+    err0[k] = s.err0;
+    // mixed: converged means the last FP64 residual met the target; it is
+    // the returned iterate's
+    errf[k] = (h->mixed && s.converged) ? s.err_rep : (s.iter > 0 ? s.err_opt : s.err0);
   }
   const BView V = bview(h);
   const int Gi = grid_items(h);
@@ -1072,7 +1081,7 @@ int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, 
       stats[k].iterations = iters[k];
       stats[k].restarts = restarts[k];
       stats[k].converged = conv[k];
-      stats[k].refinements = passes[k];
+      stats[k].refinements = reps[k];
       stats[k].err_init = err0[k];
       stats[k].err_final = errf[k];
       stats[k].ms = ms;
@@ -1302,6 +1311,8 @@ int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double
   if (!d_b) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
   if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "autotune before cup2d_psb_set_meshes");
   BCHK(h, hipSetDevice(h->cfg.device));
+  if (d_b != h->d_b)  // the captured graph reads the handle's copy of b
+    BCHK(h, hipMemcpyAsync(h->d_b, d_b, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
   static const int cand[] = {4, 8, 16, 32, 64};
   int best = h->ib;
   double best_ms = 1e300;
@@ -1309,7 +1320,7 @@ int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double
     int rc = build_units(h, ib);
     if (rc) return rc;
     double ms = 0;
-    if ((rc = time_units(h, d_b, ms))) return rc;
+    if ((rc = time_units(h, h->d_b, ms))) return rc;
     if (ms < best_ms) {
       best_ms = ms;
       best = ib;

@@ -52,6 +52,18 @@ struct Scal {
   int project;                           // remove the 1-component of r every iteration
   int iter_ref;                          // iteration of err_ref
   int stall;                             // end the solve after this many iterations without a 2x gain (0: never)
+  // residual replacement (FP32-stored Krylov vectors): once the recurrence
+  // residual has fallen by rep_factor, stalls, or claims convergence, the
+  // solve pauses; the next replacement step takes its best iterate,
+  // recomputes r = b - A x in FP64 and restarts BiCGSTAB from it.
+  // Convergence is accepted only on such a residual.  Off when rep_factor
+  // is 0.
+  double rep_factor;                     // replace when err < rep_factor * err_rep
+  double err_rep;                        // FP64 residual norm at the last replacement
+  int iter_rep;                          // iteration of the last replacement
+  int replace;                           // a replacement is pending
+  int n_rep;                             // replacements so far
+  int idle;                              // done, or waiting for its replacement
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -132,8 +144,8 @@ __device__ void load_totals(const double *__restrict__ part, int G, double (&out
 // Loads of the stencil row of cell c of block b (lane = cell): its own value
 // and the values across the block faces it touches (ix = 0 or 7: the
 // horizontal neighbour block, iy = 0 or 7: the vertical one; 0 across a wall
-// or coarse-fine face).  All loads are independent, so a wave has them in
-// flight together.
+// or coarse-fine face).  All loads are independent and unconditional, so a
+// wave has them in flight together.
 template <class TV>
 __device__ __forceinline__ void stencil_load(const TV *__restrict__ x, int b, int4 nb, int c,
                                              double &xc, double &xh, double &xv) {
@@ -142,9 +154,15 @@ __device__ __forceinline__ void stencil_load(const TV *__restrict__ x, int b, in
   const int vb = (iy == 0) ? nb.z : ((iy == 7) ? nb.w : -1);
   const int hc = (ix == 0) ? iy * 8 + 7 : iy * 8;
   const int vc = (iy == 0) ? 56 + ix : ix;
-  xc = (double)x[(long)b * 64 + c];
-  xh = (hb >= 0) ? (double)x[(long)hb * 64 + hc] : 0.0;
-  xv = (vb >= 0) ? (double)x[(long)vb * 64 + vc] : 0.0;
+  // unconditional loads (a lane with no neighbour block reads a cell of its
+  // own block, which the wave loads anyway), selected after: no branch
+  // between the loads, so none waits for another
+  const TV tc = x[(long)b * 64 + c];
+  const TV th = x[(long)(hb >= 0 ? hb : b) * 64 + hc];
+  const TV tv = x[(long)(vb >= 0 ? vb : b) * 64 + vc];
+  xc = (double)tc;
+  xh = (hb >= 0) ? (double)th : 0.0;
+  xv = (vb >= 0) ? (double)tv : 0.0;
 }
 
 // 5-point stencil row of cell c (FP64): sum over the non-wall faces of
@@ -175,9 +193,10 @@ __device__ __forceinline__ double explicit_row(const OpView &A, const TV *__rest
 #pragma unroll
     for (int w = 0; w < 8; w++) {
       const bool ok = w0 + w < A.ell_w;
-      const long k = (long)(ok ? w0 + w : 0) * A.n_exc + e;
+      const long k = (long)(ok ? w0 + w : 0) * A.n_exc + e;  // loads stay unconditional
       col[w] = A.ecol[k];
-      val[w] = ok ? A.eval[k] : 0.0;
+      const double v = A.eval[k];
+      val[w] = ok ? v : 0.0;
     }
     double xv[8];
 #pragma unroll
@@ -191,7 +210,8 @@ __device__ __forceinline__ double explicit_row(const OpView &A, const TV *__rest
 
 // The wave's U blocks b[u] (lane = cell): headers, then every load of the
 // U blocks at once -- the stencil loads, h^2 of the block and aux[i] (a
-// second vector the caller combines with the result; may be null).
+// second vector the caller combines with the result; callers without one
+// pass x).  No load sits behind a branch.
 template <int U, class TV>
 struct BlockRows {
   long i[U];
@@ -217,7 +237,7 @@ struct BlockRows {
     for (int u = 0; u < U; u++) {
       i[u] = (long)b[u] * 64 + c;
       stencil_load<TV>(x, b[u], nb[u], c, xc[u], xh[u], xv[u]);
-      a[u] = aux ? (double)aux[i[u]] : 0.0;
+      a[u] = (double)aux[i[u]];
       hh[u] = A.h2[b[u]];
     }
   }
@@ -308,14 +328,17 @@ constexpr int NQ5 = 6;  // totals of the x/r update: |r|_inf, |r|^2, (rhat,r), |
 
 // State at the top of iteration i from the state of iteration i-1 and the
 // totals o of iteration i-1's update (BiCGSTAB.cu:471-509 and :572-600).
-// prev.iter < 0 marks the state after r0 = b - A x0 (BiCGSTAB.cu:427-465).
+// prev.iter < 0 marks the state after r0 = b - A x0 (BiCGSTAB.cu:427-465);
+// `replaced` marks the state after a residual replacement (r = b - A x of
+// the best iterate, recomputed in FP64, rhat = r, p = nu = 0): BiCGSTAB
+// restarts from it, with the iteration and restart counts kept.
 // With `project` (FP32-stored residuals) the component of r along the left
 // null vector 1 of the Neumann operator, mu = sum r / n, is removed: it is
 // rounding noise no iterate can cancel, and left in place it stalls the
 // iteration at ~1e-7 and then makes it diverge.  The totals are corrected
 // for it here and the kernels subtract mu from r.  Deterministic:
 // recomputing it from the same inputs gives the same state.
-__device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
+__device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5], bool replaced = false) {
   Scal s = prev;
   double rr = o[1], rhr = o[2], hh = o[3], amax = o[0];
   s.mu = 0.0;
@@ -327,14 +350,27 @@ __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
     amax += fabs(mu);  // bound of max |r_i - mu|
   }
   const double err = prev.norm ? sqrt(rr) : amax;
-  if (prev.iter < 0) {
-    s.err = s.err0 = s.err_opt = s.err_ref = err;
-    s.iter = s.iter_ref = 0;
+  const bool rep_mode = prev.rep_factor > 0.0;
+  if (prev.iter < 0 || replaced) {
+    if (prev.iter < 0) {
+      s.err0 = err;
+      s.iter = 0;
+      s.restarts = s.n_rep = 0;
+      s.converged = 0;
+    } else {
+      s.n_rep = prev.n_rep + 1;
+    }
+    s.err = s.err_opt = s.err_ref = s.err_rep = err;
+    s.iter_ref = s.iter_rep = s.iter;
+    s.replace = 0;
     s.alpha = s.omega = s.rho_prev = 1.0;
     s.rho = rhr;
     s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
-    s.restarts = s.converged = s.improved = s.restart = 0;
-    s.done = (err == 0.0 || s.max_iter <= 0) ? 1 : 0;
+    s.improved = replaced ? 1 : 0;  // x is the selected best iterate
+    s.restart = 0;
+    s.done = (err == 0.0 || s.iter >= s.max_iter) ? 1 : 0;
+    if (replaced && (err <= s.tol_abs || err / s.err0 <= s.tol_rel)) s.converged = s.done = 1;
+    s.idle = s.done;
     return s;
   }
   if (prev.done) {
@@ -349,18 +385,26 @@ __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
     s.improved = 1;
     s.err_opt = err;
     if (err <= s.tol_abs || err / s.err0 <= s.tol_rel) {
-      s.converged = 1;
-      s.done = 1;
+      if (rep_mode) {
+        s.replace = 1;  // the recurrence says converged: confirm on the FP64 residual
+      } else {
+        s.converged = 1;
+        s.done = 1;
+      }
     }
   }
   if (!s.done && s.iter >= s.max_iter) s.done = 1;
-  // stagnation (mixed precision: the FP32 recurrence reached its floor; the
-  // caller restarts from the FP64 residual)
+  if (rep_mode && err < prev.rep_factor * s.err_rep) s.replace = 1;
+  // stagnation: mixed precision replaces the residual (the FP32 recurrence
+  // reached its floor), otherwise the solve ends
   if (err < 0.5 * s.err_ref) {
     s.err_ref = err;
     s.iter_ref = s.iter;
   }
-  if (!s.done && s.stall > 0 && s.iter - s.iter_ref > s.stall) s.done = 1;
+  if (!s.done && s.stall > 0 && s.iter - s.iter_ref > s.stall) {
+    if (rep_mode) s.replace = 1;
+    else s.done = 1;
+  }
   s.rho_prev = prev.rho;
   s.rho = rhr;
   s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
@@ -368,7 +412,7 @@ __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
   // storage (omega ~ 0) or is not finite is handled the same way
   const bool breakdown = rhr * rhr < s.bd_tol * rr * hh || !(fabs(s.beta) < 1e30);
   s.restart = 0;
-  if (!s.done && breakdown && s.max_restarts > 0) {
+  if (!s.done && !s.replace && breakdown && s.max_restarts > 0) {
     s.restarts += 1;
     if (s.restarts >= s.max_restarts) {
       s.done = 1;
@@ -379,6 +423,8 @@ __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5]) {
       s.beta = (s.rho / (s.rho_prev + s.eps)) * (s.alpha / (s.omega + s.eps));
     }
   }
+  if (s.done) s.replace = 0;
+  s.idle = s.done || s.replace;  // a pending replacement pauses the iteration
   s.red[5] = rr;
   s.red[6] = rhr;
   s.red[7] = hh;

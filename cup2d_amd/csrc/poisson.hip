@@ -616,7 +616,7 @@ __global__ __launch_bounds__(WG) void k_spmv(OpView A, int Gb, const T *__restri
         bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
       }
       BlockRows<SPMV_U, T> R;
-      R.load(A, x, (const T *)nullptr, bl, c);
+      R.load(A, x, x, bl, c);
 #pragma unroll
       for (int u = 0; u < SPMV_U; u++) {
         if (!ok[u]) continue;

@@ -68,10 +68,11 @@ typedef struct {
   double eps;             /* denominator guard; reference GPU 1e-21 (BiCGSTAB.cu:427),
                              reference CPU 1e-100 (AMRSolver.cpp:167) */
   double breakdown_tol;   /* restart when rho^2 < breakdown_tol |r|^2 |rhat|^2 (BiCGSTAB.cu:483) */
-  double inner_tol;       /* mixed precision: an FP32 inner pass ends when its recurrence residual
-                             has dropped to inner_tol times the pass's initial (FP64) residual,
-                             and the next pass restarts from the FP64 residual; 0 = no limit
-                             (a pass runs until it stops improving) */
+  double inner_tol;       /* mixed precision: the FP32 recurrence residual may fall to inner_tol
+                             times the last FP64 residual before the residual is recomputed in
+                             FP64 and BiCGSTAB restarts from the best iterate.  Single solve:
+                             0 = no limit (a pass runs until it stops improving); batch: 0 =
+                             1e-2 */
   double reserved_d[5];
 } cup2d_ps_config;
 
