@@ -36,6 +36,48 @@ METRIC = "Poisson solves/sec at 1e-10 rel residual, FP64 vs mixed; SpMV HBM GB/s
 HBM_PEAK_GBS = 8000.0
 
 
+KERNELS = ["p update + block preconditioner", "nu = A z1 (+ dots)", "s update + block preconditioner",
+           "t = A z2 (+ dots)", "x, r update (+ dots)"]
+
+
+def kernel_bytes(which, n, n_blocks, fp64=True):
+    """Algorithmic HBM bytes of one launch of iteration kernel `which`: the
+    vectors it must read and write once (s = 8 B FP64 / 4 B FP32 Krylov
+    vectors, x and x_opt always 8 B), plus 32 B of operator metadata per
+    block for the operator kernels (explicit rows add < 1 %)."""
+    s = 8 if fp64 else 4
+    per_cell = {0: 5 * s + (0 if fp64 else 4),   # r, p, nu in; p, z1 out (+ projected r)
+                1: 3 * s, 2: 4 * s, 3: 3 * s,    # z, aux in; y out / r, nu in; r, z2 out
+                4: 24 + 6 * s}[which]            # x in/out, x_opt; z1, z2, r, t, rhat in; r out
+    meta = 32 * n_blocks if which in (1, 3) else 0
+    return per_cell * n + meta
+
+
+def pmc_traffic(mode):
+    """HBM bytes per launch of the kernel of `--kernel-only MODE`, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/
+    pmc_traffic.txt), each scaled by its plain-copy calibration factor;
+    None when the summary has no line for it."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    rows, ff, wf = {}, None, None
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] == "fetch_factor":
+            ff = float(f[1])
+        elif f[0] == "write_factor":
+            wf = float(f[1])
+        elif len(f) >= 4:
+            rows[f[0]] = (float(f[2]), float(f[3]))
+    if mode not in rows or ff is None or wf is None:
+        return None
+    fetch, write = rows[mode]
+    return (fetch * ff + write * wf) * 1024.0
+
+
 def spmv_bytes_csr(n, nnz):
     """Algorithmic bytes of one FP64 CSR SpMV with int32 indices:
     nnz*(8+4) + (n+1)*4 + n*8 (x) + n*8 (y)."""
@@ -63,6 +105,28 @@ def cpu_baseline(mesh, b, budget_s=12.0):
             "sample": f"{solves} full solves of the bench workload (pipelined BiCGSTAB + block "
                       f"Cholesky, AMRSolver.cpp:149-490, to 1e-10 rel 2-norm, {iters[0]} iterations "
                       f"each) in {dt:.1f} s on {threads} OpenMP threads"}
+
+
+def batch_spmv_handle(PS, meshes):
+    """FP64 batch handle with 32 blocks per work item (the tuned size of the
+    FP64 batch) for the operator kernel alone."""
+    cfg = PS.default_config(norm=PS.NORM_L2)
+    cfg.reserved_i[1] = 32
+    return PS.BatchSolver(cfg).set_meshes(meshes)
+
+
+def batch_spmv_roofline(PS, meshes, reps):
+    bs = batch_spmv_handle(PS, meshes)
+    sz = bs.sizes()
+    d_x = bs.device_array(np.random.default_rng(0).normal(size=bs.n))
+    d_y = bs.device_array()
+    ms = bs.spmv_time(d_x, d_y, reps=reps)
+    byt = spmv_bytes_csr(sz["n"], sz["nnz"])
+    ach = byt / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "kb_spmv (batched stencil SpMV, FP64, 13.9M rows)",
+            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": pmc_traffic("batch_spmv"), "algorithmic_bytes_csr_equivalent": byt,
+            "ms_per_launch": ms}
 
 
 def bench_batch(PS, W, steps, use_graph):
@@ -100,15 +164,7 @@ def bench_batch(PS, W, steps, use_graph):
                      "refinements_max": int(max(s["refinements"] for s in st)),
                      "all_converged": bool(all(s["converged"] for s in st))}
         if prec == PS.FP64:
-            sz = bs.sizes()
-            d_y = bs.device_array()
-            ms = bs.spmv_time(d_x, d_y, reps=20)
-            byt = spmv_bytes_csr(sz["n"], sz["nnz"])
-            ach = byt / (ms * 1e-3) / 1e9
-            roof = {"bound": "hbm", "kernel": "kb_spmv (batched stencil SpMV, FP64, 13.9M rows)",
-                    "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": byt,
-                    "ms_per_launch": ms}
+            roof = batch_spmv_roofline(PS, meshes, reps=20)
         del bs, d_b, d_x
     # the same 64 solves one after another (single-solve FP64 solver)
     s = PS.Solver(norm=PS.NORM_L2, max_iter=4000, use_graph=use_graph)
@@ -138,10 +194,19 @@ def kernel_only(PS, W, which, reps):
         src = torch.ones(n, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(reps):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads src, writes dst
         torch.cuda.synchronize()
         print(json.dumps({"kernel": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
                           "launches": reps}))
+        return
+    if which.startswith("iter"):
+        mesh = W.config2_mesh()
+        s = PS.Solver(norm=PS.NORM_L2)
+        s.set_mesh(mesh)
+        k = int(which[4:])
+        ms = s.kernel_time(k, reps=reps)
+        print(json.dumps({"kernel": KERNELS[k], "ms_per_launch": ms, "launches": reps + 2,
+                          "algorithmic_bytes": kernel_bytes(k, mesh.n, mesh.n_blocks)}))
         return
     if which == "spmv":
         mesh = W.config2_mesh()
@@ -153,7 +218,7 @@ def kernel_only(PS, W, which, reps):
         ms = s.spmv_time(d_x, d_y, kind=0, reps=reps)
     else:
         meshes = W.config4_meshes()
-        s = PS.BatchSolver(PS.default_config(norm=PS.NORM_L2)).set_meshes(meshes)
+        s = batch_spmv_handle(PS, meshes)
         sz = s.sizes()
         d_x = s.device_array(np.random.default_rng(0).normal(size=s.n))
         d_y = s.device_array()
@@ -175,11 +240,13 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "copy"], default=None,
+    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "copy", "iter0", "iter1",
+                                              "iter2", "iter3", "iter4"], default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
                          "(spmv: operator on the bench mesh; batch_spmv: batched operator on the "
-                         "64-instance batch; copy: a plain 512 MiB device copy, the traffic "
-                         "counters' calibration)")
+                         "64-instance batch; iterK: iteration kernel K of the bench solve; copy: "
+                         "an elementwise 512 MiB read + 512 MiB write, the traffic counters' "
+                         "calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
@@ -232,7 +299,16 @@ def main():
     sz = s.sizes()
     ms_per_step = main["ms_per_step"]
 
-    # SpMV roofline, live: stencil kernel (the solver's operator application)
+    # every iteration kernel timed alone (HIP events on the solver's stream);
+    # the roofline object is the dominant one
+    kern = []
+    for k in range(5):
+        ms_k = s.kernel_time(k, reps=100)
+        byt = kernel_bytes(k, sz["n"], mesh.n_blocks, main_prec == PS.FP64)
+        kern.append({"kernel": KERNELS[k], "ms_per_launch": ms_k, "algorithmic_bytes": byt,
+                     "GB_per_s": byt / (ms_k * 1e-3) / 1e9})
+    dom = max(range(5), key=lambda k: kern[k]["ms_per_launch"])
+    # SpMV alone on the bench mesh: stencil kernel and plain CSR kernel
     d_y = PS.DeviceArray(s, mesh.n)
     ms_spmv = s.spmv_time(d_x, d_y, kind=0, reps=args.spmv_reps)
     ms_csr = s.spmv_time(d_x, d_y, kind=1, reps=args.spmv_reps)
@@ -271,14 +347,22 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv (stencil SpMV, FP64)",
-            "achieved": achieved,
+            "kernel": kern[dom]["kernel"] + (" (FP64)" if main_prec == PS.FP64 else " (mixed)"),
+            "achieved": kern[dom]["GB_per_s"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes": bytes_spmv,
-            "ms_per_launch": ms_spmv,
+            "frac": kern[dom]["GB_per_s"] / HBM_PEAK_GBS,
+            "traffic": pmc_traffic(f"iter{dom}") if main_prec == PS.FP64 else None,
+            "algorithmic_bytes": kern[dom]["algorithmic_bytes"],
+            "ms_per_launch": kern[dom]["ms_per_launch"],
+            "profile": "profiles/roofline_kernels.md, profiles/pmc_traffic.txt",
+        },
+        "iteration_kernels": kern,
+        "roofline_spmv": {
+            "bound": "hbm", "kernel": "k_spmv (stencil SpMV, FP64, bench mesh)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("spmv"),
+            "algorithmic_bytes_csr_equivalent": bytes_spmv, "ms_per_launch": ms_spmv,
             "csr_kernel_ms_per_launch": ms_csr,
         },
     }

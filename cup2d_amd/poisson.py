@@ -193,6 +193,13 @@ class Solver:
                                                       ctypes.byref(ms)))
         return ms.value
 
+    def kernel_time(self, which, reps=100):
+        """ms per launch of iteration kernel `which` (0 p update + prec,
+        1 nu = A z1, 2 s update + prec, 3 t = A z2, 4 x/r update)."""
+        ms = ctypes.c_double()
+        _check(self.h, _lib.load().cup2d_ps_kernel_time(self.h, int(which), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def precond(self, r):
         dr = DeviceArray.from_host(self, r)
         dz = DeviceArray(self, dr.n)

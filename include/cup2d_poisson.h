@@ -121,9 +121,19 @@ int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double to
 int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind);
 
 /* Times `reps` back-to-back launches of one SpMV kind on the handle's stream
- * with HIP events; writes the mean ms per launch. */
+ * with HIP events (after one untimed warm-up launch); writes the mean ms
+ * per launch. */
 int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, int reps,
                        double *ms_per_launch);
+
+/* Times one of the solver's five iteration kernels alone: `reps` launches
+ * back to back on the handle's stream (HIP events), each redoing the work
+ * of one iteration from the same state (set up from x = 0 and a fixed b; the
+ * handle's work vectors are overwritten, the next solve starts afresh).
+ * which: 0 p update + preconditioner, 1 nu = A z1, 2 s update +
+ * preconditioner, 3 t = A z2, 4 x and r update.  No counterpart in the
+ * reference (DeviceProfiler.cuh:26-107 times the reference's phases). */
+int cup2d_ps_kernel_time(cup2d_ps *h, int which, int reps, double *ms_per_launch);
 
 /* Block preconditioner z = P_inv r on every block (reference: cublasDgemm at
  * BiCGSTAB.cu:520, :545). */

@@ -219,3 +219,21 @@ def test_poisson_solver_interface(oracle):
     assert tr["relinf"] <= 1.5e-10
     # input untouched
     assert np.array_equal(sim.tmp.data.reshape(-1), b)
+
+
+@pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
+def test_kernel_time_then_solve(oracle, precision):
+    """Timing the iteration kernels alone leaves the handle usable: the next
+    solve starts afresh and is correct."""
+    m = W.config1_mesh()
+    s = PS.Solver(precision=precision, norm=PS.NORM_L2, max_iter=4000).set_mesh(m)
+    for k in range(5):
+        ms = s.kernel_time(k, reps=5)
+        assert 0.0 < ms < 100.0
+    b = W.vortex_rhs(m, 3)
+    x, st = s.solve(b, rel_tol=1e-10)
+    assert st["converged"]
+    tr = oracle.true_residual(oracle.assemble(m), b, x)
+    assert tr["rel2"] <= 1.05e-10
+    with pytest.raises(PS._lib.SolverError):
+        s.kernel_time(5)
