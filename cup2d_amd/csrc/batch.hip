@@ -116,7 +116,11 @@ __device__ void range_totals(const double *__restrict__ part, int NI, int i0, in
 // zeros where it has nothing to add.
 // ----------------------------------------------------------------------------
 
-constexpr int UB = 4;  // blocks per wavefront in flight
+// blocks per wavefront in flight in the operator kernels: FP32 vectors carry
+// half the bytes per load, so twice the blocks
+constexpr int UB64 = 4, UB32 = 4;
+template <class TV>
+constexpr int ub_of() { return sizeof(TV) == 4 ? UB32 : UB64; }
 
 // Calls body(item, meta) for the workgroup's items (grid-stride) whose
 // instance passes `filter` (every item when st is null); the next item's
@@ -150,14 +154,15 @@ __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F 
   }
 }
 
-// Runs f(R, u, bl) for the item's blocks, UB per wave at a time, after
-// issuing every load of those blocks (stencil, h^2 and aux[i]).  The headers
-// of all the wave's blocks of the item are loaded up front, one per lane
-// (lane s*UB + u holds block (s*WAVES + wave)*UB + u; items have at most
-// 64*WAVES blocks), and read back with readlane.
-template <class TV, class TA, class F>
+// Runs f(R, u, bl) for the item's blocks, UB = ub_of<TV>() per wave at a
+// time, after issuing every load of those blocks (stencil, h^2 and aux[i]).
+// The headers of all the wave's blocks of the item are loaded up front, one
+// per lane (lane s*UB + u holds block (s*WAVES + wave)*UB + u; items have at
+// most 64*WAVES blocks), and read back with readlane.
+template <class TV, class F>
 __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict__ x,
-                                            const TA *__restrict__ aux, int b0, int nbk, F &&f) {
+                                            const TV *__restrict__ aux, int b0, int nbk, F &&f) {
+  constexpr int UB = ub_of<TV>();
   const int c = lane_id(), w = wave_id();
   const int jl = ((c / UB) * WAVES + w) * UB + (c % UB);
   int4 hn = make_int4(-1, -1, -1, -1);
@@ -215,15 +220,15 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
       v[4] += rd;
     };
     if (mt.z > 0) {
-      item_blocks<double>(V, x, b, mt.y, mt.z, [&](const BlockRows<UB, double> &R, int u, int) {
+      item_blocks<double>(V, x, b, mt.y, mt.z, [&](const auto &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
-        const long i = R.i[u];
+        const long i = R.i(u, c);
         p[i] = T(0);
         nu[i] = T(0);
-        xopt[i] = R.xc[u];
-        v[6] += R.hh[u] * R.xc[u];
-        if (plain) store_r(i, y, R.a[u]);
+        xopt[i] = R.xc(u);
+        v[6] += R.hh[u] * R.xc(u);
+        if (plain) store_r(i, y, R.a(u));
       });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
@@ -252,11 +257,11 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
       v[0] += rh * (double)yt;
     };
     if (mt.z > 0) {
-      item_blocks<T>(V, z1, rhat, mt.y, mt.z, [&](const BlockRows<UB, T> &R, int u, int) {
+      item_blocks<T>(V, z1, rhat, mt.y, mt.z, [&](const auto &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
-        v[1] += R.hh[u] * R.xc[u];
-        if (plain) store_y(R.i[u], y, R.a[u]);
+        v[1] += R.hh[u] * R.xc(u);
+        if (plain) store_y(R.i(u, c), y, R.a(u));
       });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
@@ -284,11 +289,11 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
       v[1] += yd * yd;
     };
     if (mt.z > 0) {
-      item_blocks<T>(V, z2, sv, mt.y, mt.z, [&](const BlockRows<UB, T> &R, int u, int) {
+      item_blocks<T>(V, z2, sv, mt.y, mt.z, [&](const auto &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
-        v[2] += R.hh[u] * R.xc[u];
-        if (plain) store_y(R.i[u], y, R.a[u]);
+        v[2] += R.hh[u] * R.xc(u);
+        if (plain) store_y(R.i(u, c), y, R.a(u));
       });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
@@ -307,11 +312,11 @@ __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict_
     double v[1] = {0.0};
     if (mt.z > 0) {
       item_blocks<double>(V, x, x, mt.y, mt.z,
-                          [&](const BlockRows<UB, double> &R, int u, int) {
+                          [&](const auto &R, int u, int) {
                             bool plain;
                             const double yi = R.y(u, c, plain);
-                            v[0] += R.hh[u] * R.xc[u];
-                            if (plain) y[R.i[u]] = yi;
+                            v[0] += R.hh[u] * R.xc(u);
+                            if (plain) y[R.i(u, c)] = yi;
                           });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x;

@@ -141,44 +141,49 @@ __device__ void load_totals(const double *__restrict__ part, int G, double (&out
   }
 }
 
-// Loads of the stencil row of cell c of block b (lane = cell): its own value
-// and the values across the block faces it touches (ix = 0 or 7: the
-// horizontal neighbour block, iy = 0 or 7: the vertical one; 0 across a wall
-// or coarse-fine face).  All loads are independent and unconditional, so a
-// wave has them in flight together.
+// Loads of the stencil row of cell c of block b (lane = cell), kept in the
+// storage type: its own value and the values across the block faces it
+// touches (ix = 0 or 7: the horizontal neighbour block, iy = 0 or 7: the
+// vertical one).  All loads are independent and unconditional -- a lane
+// with no neighbour block across a face reads a cell of its own block, which
+// the wave loads anyway, and stencil_eval ignores it -- so a wave has them in
+// flight together and none waits behind a branch.
 template <class TV>
 __device__ __forceinline__ void stencil_load(const TV *__restrict__ x, int b, int4 nb, int c,
-                                             double &xc, double &xh, double &xv) {
+                                             TV &tc, TV &th, TV &tv) {
   const int ix = c & 7, iy = c >> 3;
   const int hb = (ix == 0) ? nb.x : ((ix == 7) ? nb.y : -1);
   const int vb = (iy == 0) ? nb.z : ((iy == 7) ? nb.w : -1);
   const int hc = (ix == 0) ? iy * 8 + 7 : iy * 8;
   const int vc = (iy == 0) ? 56 + ix : ix;
-  // unconditional loads (a lane with no neighbour block reads a cell of its
-  // own block, which the wave loads anyway), selected after: no branch
-  // between the loads, so none waits for another
-  const TV tc = x[(long)b * 64 + c];
-  const TV th = x[(long)(hb >= 0 ? hb : b) * 64 + hc];
-  const TV tv = x[(long)(vb >= 0 ? vb : b) * 64 + vc];
-  xc = (double)tc;
-  xh = (hb >= 0) ? (double)th : 0.0;
-  xv = (vb >= 0) ? (double)tv : 0.0;
+  tc = x[(long)b * 64 + c];
+  th = x[(long)(hb >= 0 ? hb : b) * 64 + hc];
+  tv = x[(long)(vb >= 0 ? vb : b) * 64 + vc];
 }
 
-// 5-point stencil row of cell c (FP64): sum over the non-wall faces of
-// (x_nb - x_c), in the face order Ym, Xm, Xp, Yp.  Neighbours inside the
-// block come from the other lanes of the wave.  Meaningless for explicit
-// rows (their coarse-fine faces are skipped), which the caller overwrites.
-__device__ __forceinline__ double stencil_eval(int4 nb, int c, double xc, double xh, double xv) {
+// 5-point stencil row of cell c (FP64 arithmetic): sum over the non-wall
+// faces of (x_nb - x_c), in the face order Ym, Xm, Xp, Yp.  Neighbours
+// inside the block come from the other lanes of the wave (shuffled in the
+// storage type), those across a face from th / tv (walls and coarse-fine
+// faces are skipped).  Meaningless for explicit rows, which the caller
+// overwrites.
+template <class TV>
+__device__ __forceinline__ double stencil_eval(int4 nb, int c, TV tc, TV th, TV tv) {
   const int ix = c & 7, iy = c >> 3;
-  const double xl = __shfl(xc, (c - 1) & 63), xr = __shfl(xc, (c + 1) & 63);
-  const double xd = __shfl(xc, (c - 8) & 63), xu = __shfl(xc, (c + 8) & 63);
+  const double xl = (double)__shfl(tc, (c - 1) & 63), xr = (double)__shfl(tc, (c + 1) & 63);
+  const double xd = (double)__shfl(tc, (c - 8) & 63), xu = (double)__shfl(tc, (c + 8) & 63);
+  const double xc = (double)tc, xh = (double)th, xv = (double)tv;
+  // branch-free: a face contributes its neighbour when it lies inside the
+  // block or across a same-level face (adding +0.0 otherwise leaves the sum
+  // unchanged, so the order and the result are those of the plain loop)
+  const bool od = iy > 0 || nb.z >= 0, ol = ix > 0 || nb.x >= 0;
+  const bool orr = ix < 7 || nb.y >= 0, ou = iy < 7 || nb.w >= 0;
   double y = 0.0;
-  int deg = 0;
-  if (iy > 0) { y += xd; deg++; } else if (nb.z >= 0) { y += xv; deg++; }
-  if (ix > 0) { y += xl; deg++; } else if (nb.x >= 0) { y += xh; deg++; }
-  if (ix < 7) { y += xr; deg++; } else if (nb.y >= 0) { y += xh; deg++; }
-  if (iy < 7) { y += xu; deg++; } else if (nb.w >= 0) { y += xv; deg++; }
+  y += od ? (iy > 0 ? xd : xv) : 0.0;
+  y += ol ? (ix > 0 ? xl : xh) : 0.0;
+  y += orr ? (ix < 7 ? xr : xh) : 0.0;
+  y += ou ? (iy < 7 ? xu : xv) : 0.0;
+  const int deg = int(od) + int(ol) + int(orr) + int(ou);
   return y - double(deg) * xc;
 }
 
@@ -210,18 +215,18 @@ __device__ __forceinline__ double explicit_row(const OpView &A, const TV *__rest
 
 // The wave's U blocks b[u] (lane = cell): headers, then every load of the
 // U blocks at once -- the stencil loads, h^2 of the block and aux[i] (a
-// second vector the caller combines with the result; callers without one
-// pass x).  No load sits behind a branch.
+// second vector of the same type the caller combines with the result;
+// callers without one pass x).  Values stay in the storage type until used.
 template <int U, class TV>
 struct BlockRows {
-  long i[U];
+  int bk[U];
   int4 nb[U];
   unsigned long long m[U];
-  double xc[U], xh[U], xv[U], a[U], hh[U];
+  TV tc[U], th[U], tv[U], ta[U];
+  double hh[U];
 
-  template <class TA>
   __device__ __forceinline__ void load(const OpView &A, const TV *__restrict__ x,
-                                       const TA *__restrict__ aux, const int (&b)[U], int c) {
+                                       const TV *__restrict__ aux, const int (&b)[U], int c) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       nb[u] = A.nbr[b[u]];
@@ -230,21 +235,23 @@ struct BlockRows {
     load_data(A, x, aux, b, c);
   }
   // the same with nb[] and m[] already set
-  template <class TA>
   __device__ __forceinline__ void load_data(const OpView &A, const TV *__restrict__ x,
-                                            const TA *__restrict__ aux, const int (&b)[U], int c) {
+                                            const TV *__restrict__ aux, const int (&b)[U], int c) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      i[u] = (long)b[u] * 64 + c;
-      stencil_load<TV>(x, b[u], nb[u], c, xc[u], xh[u], xv[u]);
-      a[u] = (double)aux[i[u]];
+      bk[u] = b[u];
+      stencil_load<TV>(x, b[u], nb[u], c, tc[u], th[u], tv[u]);
+      ta[u] = aux[(long)b[u] * 64 + c];
       hh[u] = A.h2[b[u]];
     }
   }
+  __device__ __forceinline__ long i(int u, int c) const { return (long)bk[u] * 64 + c; }
+  __device__ __forceinline__ double xc(int u) const { return (double)tc[u]; }
+  __device__ __forceinline__ double a(int u) const { return (double)ta[u]; }
   // stencil value of block u; `plain` is false on explicit rows
   __device__ __forceinline__ double y(int u, int c, bool &plain) const {
     plain = !((m[u] >> c) & 1ull);
-    return stencil_eval(nb[u], c, xc[u], xh[u], xv[u]);
+    return stencil_eval<TV>(nb[u], c, tc[u], th[u], tv[u]);
   }
 };
 

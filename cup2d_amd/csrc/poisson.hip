@@ -116,12 +116,12 @@ __global__ __launch_bounds__(WG) void k_init(OpView A, int Gb, const double *__r
         if (!ok[u]) continue;
         bool plain;
         const double y = R.y(u, c, plain);
-        const long i = R.i[u];
+        const long i = R.i(u, c);
         p[i] = TR(0);
         nu[i] = TR(0);
-        x[i] = R.xc[u];
-        xopt[i] = R.xc[u];
-        if (plain) store_r(i, y, R.a[u]);
+        x[i] = R.xc(u);
+        xopt[i] = R.xc(u);
+        if (plain) store_r(i, y, R.a(u));
       }
     }
   } else {
@@ -420,8 +420,8 @@ __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__res
         if (!ok[u]) continue;
         bool plain;
         const double y = R.y(u, c, plain);
-        if (plain) store_y(R.i[u], y, R.a[u]);
-        v[1] += R.hh[u] * R.xc[u];
+        if (plain) store_y(R.i(u, c), y, R.a(u));
+        v[1] += R.hh[u] * R.xc(u);
       }
     }
   } else {
@@ -472,8 +472,8 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__rest
         if (!ok[u]) continue;
         bool plain;
         const double y = R.y(u, c, plain);
-        if (plain) store_y(R.i[u], y, R.a[u]);
-        v[2] += R.hh[u] * R.xc[u];
+        if (plain) store_y(R.i(u, c), y, R.a(u));
+        v[2] += R.hh[u] * R.xc(u);
       }
     }
   } else {
@@ -622,8 +622,8 @@ __global__ __launch_bounds__(WG) void k_spmv(OpView A, int Gb, const T *__restri
         if (!ok[u]) continue;
         bool plain;
         const double yi = R.y(u, c, plain);
-        if (plain) y[R.i[u]] = (T)yi;
-        v[0] += R.hh[u] * R.xc[u];
+        if (plain) y[R.i(u, c)] = (T)yi;
+        v[0] += R.hh[u] * R.xc(u);
       }
     }
   } else {
