@@ -324,19 +324,56 @@ int ell_width(const Operator &op) {
   return w;
 }
 
-void ell_append(const Operator &op, int W, long long row_off, std::vector<int> &erow,
-                std::vector<int> &ecol_rm, std::vector<double> &eval_rm) {
+int EllRows::code(double v) {
+  uint64_t bits;
+  std::memcpy(&bits, &v, sizeof bits);
+  auto it = std::lower_bound(codes_.begin(), codes_.end(), std::make_pair(bits, uint16_t(0)),
+                             [](const auto &a, const auto &b) { return a.first < b.first; });
+  if (it != codes_.end() && it->first == bits) return it->second;
+  if (values.size() > 0xffff) return -1;
+  const uint16_t c = (uint16_t)values.size();
+  values.push_back(v);
+  codes_.insert(it, {bits, c});
+  return c;
+}
+
+bool EllRows::append(const Operator &op, long long row_off, std::string &err) {
+  const long long n_exc = op.n_exc();
+  std::vector<long long> order(n_exc);  // explicit rows of op in mesh order: block, then cell
+  std::vector<int> rowof(n_exc);
   for (long long b = 0; b < op.nb; b++)
     for (int e = op.exc_bptr[b]; e < op.exc_bptr[b + 1]; e++) {
-      const int row = (int)(b * BLEN + op.exc_cell[e] + row_off);
-      erow.push_back(row);
-      const int k0 = op.exc_rptr[e], k1 = op.exc_rptr[e + 1];
-      for (int w = 0; w < W; w++) {
-        const bool ok = k0 + w < k1;
-        ecol_rm.push_back(ok ? (int)(op.exc_col[k0 + w] + row_off) : row);
-        eval_rm.push_back(ok ? op.exc_val[k0 + w] : 0.0);
-      }
+      order[e] = e;
+      rowof[e] = (int)(b * BLEN + op.exc_cell[e] + row_off);
     }
+  auto length = [&](long long e) { return op.exc_rptr[e + 1] - op.exc_rptr[e]; };
+  std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) { return length(a) < length(b); });
+  for (long long e : order) {
+    const int row = rowof[e], k0 = op.exc_rptr[e], k1 = op.exc_rptr[e + 1];
+    if (k1 - k0 > W) {
+      err = "explicit row wider than the ELL width";
+      return false;
+    }
+    erow.push_back(row);
+    len.push_back((uint8_t)(k1 - k0));
+    for (int w = 0; w < W; w++) {
+      const bool ok = k0 + w < k1;
+      ecol_rm.push_back(ok ? (int)(op.exc_col[k0 + w] + row_off) : row);
+      const int c = ok ? code(op.exc_val[k0 + w]) : 0;
+      if (c < 0) {
+        err = "more than 65536 distinct coefficients in the explicit rows";
+        return false;
+      }
+      ecode_rm.push_back((uint16_t)c);
+    }
+  }
+  return true;
+}
+
+int EllRows::width(long long e0, long long e1) const {
+  int w = 0;
+  for (long long e = e0; e < e1; e++) w = std::max(w, (int)len[e]);
+  return w;
 }
 
 std::vector<double> make_pinv() {

@@ -62,13 +62,30 @@ struct Operator {
 bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err);
 
 // Explicit rows in ELL form.  ell_width: most entries of an explicit row
-// (0 when there are none).  ell_append: appends op's explicit rows with
+// (0 when there are none).  EllRows::append adds op's explicit rows with
 // global row / column offset row_off, W entries per row (row-major), padded
-// with (own row, 0.0).  ell_transpose: row-major [n][W] to the kernels'
-// column-major [W][n].
+// with (own row, 0.0), shortest rows first (a stable sort, so rows of one
+// length keep their mesh order): a run of rows then shares one width and
+// the kernels stop at it.  A coefficient is stored as a 16-bit code into
+// `values`, the distinct FP64 coefficients by bit pattern (code 0 is 0.0);
+// the interface stencils have a few dozen.  append fails when op has a row
+// wider than W or the codes run out.  ell_transpose: row-major [n][W] to
+// the kernels' column-major [W][n].
 int ell_width(const Operator &op);
-void ell_append(const Operator &op, int W, long long row_off, std::vector<int> &erow,
-                std::vector<int> &ecol_rm, std::vector<double> &eval_rm);
+struct EllRows {
+  int W = 0;
+  std::vector<int> erow;           // [n] global row
+  std::vector<int> ecol_rm;        // [n][W] global column
+  std::vector<uint16_t> ecode_rm;  // [n][W] coefficient code
+  std::vector<uint8_t> len;        // [n] entries of the row
+  std::vector<double> values{0.0};
+  bool append(const Operator &op, long long row_off, std::string &err);
+  // widest row in [e0, e1)
+  int width(long long e0, long long e1) const;
+ private:
+  std::vector<std::pair<uint64_t, uint16_t>> codes_{{0, 0}};  // sorted by bit pattern
+  int code(double v);  // -1 when the table is full
+};
 template <class T>
 std::vector<T> ell_transpose(const std::vector<T> &rm, int W) {
   const size_t n = W > 0 ? rm.size() / W : 0;

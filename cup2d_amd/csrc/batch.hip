@@ -232,7 +232,7 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
       });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
-      store_r(row, explicit_row<double>(V.A, x, e), b[row]);
+      store_r(row, explicit_row<double>(V.A, x, e, mt.w), b[row]);
     }
     v[2] = v[1];
     v[3] = v[1];
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
       });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
-      store_y(row, explicit_row<T>(V.A, z1, e), (double)rhat[row]);
+      store_y(row, explicit_row<T>(V.A, z1, e, mt.w), (double)rhat[row]);
     }
     store_item_partials<2, 0u>(v, p2, V.n_items, item);
   });
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
       });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
-      store_y(row, explicit_row<T>(V.A, z2, e), (double)sv[row]);
+      store_y(row, explicit_row<T>(V.A, z2, e, mt.w), (double)sv[row]);
     }
     store_item_partials<3, 0u>(v, p4, V.n_items, item);
   });
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict_
                           });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x;
-      y[V.A.erow[e]] = explicit_row<double>(V.A, x, e);
+      y[V.A.erow[e]] = explicit_row<double>(V.A, x, e, mt.w);
     }
     store_item_partials<1, 0u>(v, pm, V.n_items, item);
   });
@@ -733,13 +733,15 @@ struct cup2d_psb {
   std::vector<long long> boff;      // [K+1] block offset of each instance
   std::vector<long long> eoff;      // [K+1] explicit-row offset of each instance
   std::vector<long long> h_mean_row;
-  std::vector<int> item_inst, item_b0, item_nb, item_begin;
+  std::vector<int> item_inst, item_b0, item_nb, item_w, item_begin;
   std::vector<int> tile_inst, tile_b0, tile_nb;
   // device: operator
   int4 *d_nbr = nullptr;
   unsigned long long *d_mask = nullptr;
   int *d_erow = nullptr, *d_ecol = nullptr;
-  double *d_eval = nullptr, *d_h2 = nullptr, *d_P = nullptr;
+  uint16_t *d_ecode = nullptr;
+  double *d_etab = nullptr, *d_h2 = nullptr, *d_P = nullptr;
+  std::vector<uint8_t> elen;        // [n_exc] entries of each explicit row
   int ell_w = 0;
   bf16x8 *d_pfrag = nullptr;
   long long *d_mean_row = nullptr;
@@ -817,7 +819,9 @@ BView bview(const cup2d_psb *h) {
   V.A.ell_w = h->ell_w;
   V.A.erow = h->d_erow;
   V.A.ecol = h->d_ecol;
-  V.A.eval = h->d_eval;
+  V.A.ecode = h->d_ecode;
+  V.A.etab = h->d_etab;
+  V.A.ewid = nullptr;                 // explicit items carry their width
   V.A.h2 = h->d_h2;
   V.item = h->d_item;
   V.item_begin = h->d_item_begin;
@@ -846,6 +850,7 @@ int build_units(cup2d_psb *h, int ib) {
   h->item_inst.clear();
   h->item_b0.clear();
   h->item_nb.clear();
+  h->item_w.clear();
   h->item_begin.assign(1, 0);
   h->tile_inst.clear();
   h->tile_b0.clear();
@@ -856,11 +861,16 @@ int build_units(cup2d_psb *h, int ib) {
       h->item_inst.push_back(k);
       h->item_b0.push_back((int)b);
       h->item_nb.push_back((int)std::min<long long>(ib, b1 - b));
+      h->item_w.push_back(0);
     }
     for (long long e = h->eoff[k]; e < h->eoff[k + 1]; e += WG) {
+      const long long e1 = std::min<long long>(e + WG, h->eoff[k + 1]);
       h->item_inst.push_back(k);
       h->item_b0.push_back((int)e);
-      h->item_nb.push_back(-(int)std::min<long long>(WG, h->eoff[k + 1] - e));
+      h->item_nb.push_back(-(int)(e1 - e));
+      int w = 0;
+      for (long long j = e; j < e1; j++) w = std::max(w, (int)h->elen[j]);
+      h->item_w.push_back(w);
     }
     h->item_begin.push_back((int)h->item_inst.size());
     for (long long b = b0; b < b1; b += TILE) {
@@ -876,7 +886,8 @@ int build_units(cup2d_psb *h, int ib) {
   for (size_t i = 0; i < tile4.size(); i++)
     tile4[i] = make_int4(h->tile_inst[i], h->tile_b0[i], h->tile_nb[i], 0);
   std::vector<int4> item4(NI);
-  for (size_t i = 0; i < NI; i++) item4[i] = make_int4(h->item_inst[i], h->item_b0[i], h->item_nb[i], 0);
+  for (size_t i = 0; i < NI; i++)
+    item4[i] = make_int4(h->item_inst[i], h->item_b0[i], h->item_nb[i], h->item_w[i]);
   if ((rc = bupload(h, pool, &h->d_item, item4)) ||
       (rc = bupload(h, pool, &h->d_item_begin, h->item_begin)) ||
       (rc = bupload(h, pool, &h->d_tile, tile4)) ||
@@ -1212,8 +1223,9 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
   std::vector<int4> nbr;
   std::vector<unsigned long long> mask;
   std::vector<double> h2;
-  std::vector<int> erow, ecol_rm;
-  std::vector<double> eval_rm;
+  EllRows ell;
+  ell.W = W;
+  std::string eerr;
   nbr.reserve(h->nb);
   mask.reserve(h->nb);
   h2.reserve(h->nb);
@@ -1234,20 +1246,23 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
       mask.push_back(op.exc_mask[b]);
       h2.push_back(op.h2[b]);
     }
-    ell_append(op, W, ro, erow, ecol_rm, eval_rm);
-    h->eoff[k + 1] = (long long)erow.size();
+    if (!ell.append(op, ro, eerr))
+      return bfail(h, CUP2D_PS_EINVAL, "mesh " + std::to_string(k) + ": " + eerr);
+    h->eoff[k + 1] = (long long)ell.erow.size();
     if (op.mean_row >= 0) h->h_mean_row[k] = op.mean_row + ro;
   }
   h->ell_w = W;
+  h->elen = std::move(ell.len);
   ops.clear();
   auto &pool = h->allocs;
   int rc;
   const std::vector<double> P = make_pinv();
   const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = bupload(h, pool, &h->d_nbr, nbr)) || (rc = bupload(h, pool, &h->d_mask, mask)) ||
-      (rc = bupload(h, pool, &h->d_erow, erow)) ||
-      (rc = bupload(h, pool, &h->d_ecol, ell_transpose(ecol_rm, W))) ||
-      (rc = bupload(h, pool, &h->d_eval, ell_transpose(eval_rm, W))) ||
+      (rc = bupload(h, pool, &h->d_erow, ell.erow)) ||
+      (rc = bupload(h, pool, &h->d_ecol, ell_transpose(ell.ecol_rm, W))) ||
+      (rc = bupload(h, pool, &h->d_ecode, ell_transpose(ell.ecode_rm, W))) ||
+      (rc = bupload(h, pool, &h->d_etab, ell.values)) ||
       (rc = bupload(h, pool, &h->d_h2, h2)) ||
       (rc = bupload(h, pool, &h->d_P, P)) ||
       (rc = bupload(h, pool, &h->d_mean_row, h->h_mean_row)))

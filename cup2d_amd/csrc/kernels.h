@@ -24,8 +24,10 @@ struct OpView {
   const unsigned long long *mask;        // [nb] explicit rows of each block
   int n_exc, ell_w;
   const int *erow;                       // [n_exc] global row of each explicit row
-  const int *ecol;                       // [ell_w * n_exc]
-  const double *eval;                    // [ell_w * n_exc]
+  const int *ecol;                       // [ell_w * n_exc] column (padding: own row)
+  const unsigned short *ecode;           // [ell_w * n_exc] coefficient, as an index into etab
+  const double *etab;                    // distinct coefficients (etab[0] = 0.0, the padding)
+  const unsigned char *ewid;             // [ceil(n_exc / 64)] widest row of each run of 64
   const double *h2;                      // [nb]
 };
 
@@ -187,30 +189,40 @@ __device__ __forceinline__ double stencil_eval(int4 nb, int c, TV tc, TV th, TV 
   return y - double(deg) * xc;
 }
 
-// Explicit row e (one thread): sum_w val * x[col] in FP64, entries in
-// ascending column order; loads issued eight entries at a time.
+// Explicit row e (one thread): sum_w val * x[col] in FP64 over its first
+// `width` entries (wave-uniform, at least the row's length; entries past the
+// row's length are (own row, 0.0)), in ascending column order; loads issued
+// eight entries at a time.
 template <class TV>
-__device__ __forceinline__ double explicit_row(const OpView &A, const TV *__restrict__ x, int e) {
+__device__ __forceinline__ double explicit_row(const OpView &A, const TV *__restrict__ x, int e,
+                                               int width) {
   double acc = 0.0;
-  for (int w0 = 0; w0 < A.ell_w; w0 += 8) {
+  for (int w0 = 0; w0 < width; w0 += 8) {
     int col[8];
-    double val[8];
+    unsigned code[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) {
-      const bool ok = w0 + w < A.ell_w;
-      const long k = (long)(ok ? w0 + w : 0) * A.n_exc + e;  // loads stay unconditional
+      const long k = (long)(w0 + w < width ? w0 + w : w0) * A.n_exc + e;  // loads stay unconditional
       col[w] = A.ecol[k];
-      const double v = A.eval[k];
-      val[w] = ok ? v : 0.0;
+      code[w] = A.ecode[k];
     }
-    double xv[8];
+    double xv[8], val[8];
 #pragma unroll
-    for (int w = 0; w < 8; w++) xv[w] = (double)x[col[w]];
+    for (int w = 0; w < 8; w++) {
+      xv[w] = (double)x[col[w]];
+      val[w] = A.etab[code[w]];
+    }
 #pragma unroll
     for (int w = 0; w < 8; w++)
-      if (w0 + w < A.ell_w) acc += val[w] * xv[w];
+      if (w0 + w < width) acc += val[w] * xv[w];
   }
   return acc;
+}
+
+// Width of explicit row e's run of 64 rows (e..e+63 aligned at 64 are one
+// wave's rows in the single-mesh kernels, so the value is wave-uniform).
+__device__ __forceinline__ int ell_run_width(const OpView &A, int e) {
+  return __builtin_amdgcn_readfirstlane((int)A.ewid[e >> 6]);
 }
 
 // The wave's U blocks b[u] (lane = cell): headers, then every load of the

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(WG) void k_init(OpView A, int Gb, const double *__r
     const int ge = gridDim.x - Gb;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
       const int row = A.erow[e];
-      store_r(row, explicit_row<double>(A, x0, e), b[row]);
+      store_r(row, explicit_row<double>(A, x0, e, ell_run_width(A, e)), b[row]);
     }
   }
   v[2] = v[1];
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__res
     if (done) return;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
       const int row = A.erow[e];
-      store_y(row, explicit_row<T>(A, z1, e), (double)rhat[row]);
+      store_y(row, explicit_row<T>(A, z1, e, ell_run_width(A, e)), (double)rhat[row]);
     }
   }
   if (done) return;
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__rest
     if (done) return;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
       const int row = A.erow[e];
-      store_y(row, explicit_row<T>(A, z2, e), (double)sv[row]);
+      store_y(row, explicit_row<T>(A, z2, e, ell_run_width(A, e)), (double)sv[row]);
     }
   }
   if (done) return;
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(WG) void k_spmv(OpView A, int Gb, const T *__restri
   } else {
     const int ge = gridDim.x - Gb;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG)
-      y[A.erow[e]] = (T)explicit_row<T>(A, x, e);
+      y[A.erow[e]] = (T)explicit_row<T>(A, x, e, ell_run_width(A, e));
   }
   if (A.mean_row >= 0) store_partials<1, 0u>(v, pm);
 }
@@ -747,7 +747,9 @@ struct cup2d_ps {
   int4 *d_nbr = nullptr;
   unsigned long long *d_mask = nullptr;
   int *d_erow = nullptr, *d_ecol = nullptr;
-  double *d_eval = nullptr, *d_h2 = nullptr;
+  uint16_t *d_ecode = nullptr;
+  uint8_t *d_ewid = nullptr;
+  double *d_etab = nullptr, *d_h2 = nullptr;
   int ell_w = 0;
   int *d_csr_rp = nullptr, *d_csr_col = nullptr;
   double *d_csr_val = nullptr;
@@ -827,7 +829,9 @@ OpView view(const cup2d_ps *h) {
   A.ell_w = h->ell_w;
   A.erow = h->d_erow;
   A.ecol = h->d_ecol;
-  A.eval = h->d_eval;
+  A.ecode = h->d_ecode;
+  A.etab = h->d_etab;
+  A.ewid = h->d_ewid;
   A.h2 = h->d_h2;
   return A;
 }
@@ -1159,15 +1163,21 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
   h->ell_w = ell_width(op);
   if (h->ell_w > ELL_MAX)
     return fail(h, CUP2D_PS_EINVAL, "explicit row with more than " + std::to_string(ELL_MAX) + " entries");
-  std::vector<int> erow, ecol_rm;
-  std::vector<double> eval_rm;
-  ell_append(op, h->ell_w, 0, erow, ecol_rm, eval_rm);
+  EllRows ell;
+  ell.W = h->ell_w;
+  std::string eerr;
+  if (!ell.append(op, 0, eerr)) return fail(h, CUP2D_PS_EINVAL, eerr);
+  const size_t n_exc = ell.erow.size();
+  std::vector<uint8_t> ewid((n_exc + 63) / 64);
+  for (size_t r = 0; r < ewid.size(); r++)
+    ewid[r] = (uint8_t)ell.width((long long)(64 * r), (long long)std::min(n_exc, 64 * r + 64));
   const std::vector<double> P = make_pinv();
   const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = upload(h, &h->d_nbr, nbr)) || (rc = upload(h, &h->d_mask, mask)) ||
-      (rc = upload(h, &h->d_erow, erow)) ||
-      (rc = upload(h, &h->d_ecol, ell_transpose(ecol_rm, h->ell_w))) ||
-      (rc = upload(h, &h->d_eval, ell_transpose(eval_rm, h->ell_w))) ||
+      (rc = upload(h, &h->d_erow, ell.erow)) ||
+      (rc = upload(h, &h->d_ecol, ell_transpose(ell.ecol_rm, h->ell_w))) ||
+      (rc = upload(h, &h->d_ecode, ell_transpose(ell.ecode_rm, h->ell_w))) ||
+      (rc = upload(h, &h->d_etab, ell.values)) || (rc = upload(h, &h->d_ewid, ewid)) ||
       (rc = upload(h, &h->d_h2, op.h2)) || (rc = upload(h, &h->d_csr_rp, op.rp)) ||
       (rc = upload(h, &h->d_csr_col, op.col)) || (rc = upload(h, &h->d_csr_val, op.val)) ||
       (rc = upload(h, &h->d_P, P)))
