@@ -71,6 +71,7 @@ SIGNATURES = {
     "cup2d_psb_item_blocks": (_I, [_P]),
     "cup2d_psb_spmv_device": (_I, [_P, _P, _P]),
     "cup2d_psb_spmv_time": (_I, [_P, _P, _P, _I, _P]),
+    "cup2d_psb_copy_time": (_I, [_P, _P, _P, _I, _P]),
     "cup2d_psb_sizes": (_I, [_P, _P, _P, _P]),
     "cup2d_psb_last_error": (ctypes.c_char_p, [_P]),
 }

@@ -89,22 +89,27 @@ __device__ void range_totals(const double *__restrict__ part, int NI, int i0, in
                              double (&out)[NQ]) {
   __shared__ double fin[NQ][WAVES];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) a[q] = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += WG) {
+    double p[NQ];  // the NQ loads of a partial are in flight together
+#pragma unroll
+    for (int q = 0; q < NQ; q++) p[q] = part[(long)q * NI + i];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) a[q] = ((MAXQ >> q) & 1) ? nmax(a[q], p[q]) : a[q] + p[q];
+  }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    double a = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += WG) {
-      const double p = part[(long)q * NI + i];
-      a = ((MAXQ >> q) & 1) ? nmax(a, p) : a + p;
-    }
-    a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
-    if (lane == 0) fin[q][w] = a;
+    const double r = ((MAXQ >> q) & 1) ? wave_max(a[q]) : wave_sum(a[q]);
+    if (lane == 0) fin[q][w] = r;
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    double a = fin[q][0];
-    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? nmax(a, fin[q][i]) : a + fin[q][i];
-    out[q] = a;
+    double r = fin[q][0];
+    for (int i = 1; i < WAVES; i++) r = ((MAXQ >> q) & 1) ? nmax(r, fin[q][i]) : r + fin[q][i];
+    out[q] = r;
   }
 }
 
@@ -122,10 +127,6 @@ constexpr int UB64 = 4, UB32 = 4;
 template <class TV>
 constexpr int ub_of() { return sizeof(TV) == 4 ? UB32 : UB64; }
 
-// Calls body(item, meta) for the workgroup's items (grid-stride) whose
-// instance passes `filter` (every item when st is null); the next item's
-// meta is loaded while the current one is processed.  body may synchronize
-// the workgroup (the skip is uniform).
 // Which instances a kernel works on.
 enum Filter { F_ALL, F_RUNNING, F_LIVE, F_REPLACE };
 __device__ __forceinline__ bool wanted(const Scal &s, int f) {
@@ -137,41 +138,73 @@ __device__ __forceinline__ bool wanted(const Scal &s, int f) {
   }
 }
 
-template <class F>
+// Block headers of a block item, one per lane: lane c of wave w holds block
+// ((c / UB) * WAVES + w) * UB + c % UB of the item (items have at most
+// 64 * WAVES blocks), the order in which item_blocks visits them; empty for
+// explicit items and past the item's end.
+struct Heads {
+  int4 n;
+  unsigned long long m;
+};
+template <int UB>
+__device__ __forceinline__ Heads load_heads(const BView &V, int4 mt) {
+  Heads h{make_int4(-1, -1, -1, -1), 0ull};
+  const int c = lane_id(), w = wave_id();
+  const int jl = ((c / UB) * WAVES + w) * UB + (c % UB);
+  if (jl < mt.z) {
+    h.n = V.A.nbr[mt.y + jl];
+    h.m = V.A.mask[mt.y + jl];
+  }
+  return h;
+}
+
+__device__ __forceinline__ int4 readlane4(int4 v, int l) {
+  return make_int4(__builtin_amdgcn_readlane(v.x, l), __builtin_amdgcn_readlane(v.y, l),
+                   __builtin_amdgcn_readlane(v.z, l), __builtin_amdgcn_readlane(v.w, l));
+}
+
+// Calls body(item, meta, heads) for the workgroup's items (grid-stride)
+// whose instance passes `filter` (every item when st is null).  The metas of
+// up to 64 of the workgroup's items are loaded at once (one per lane), and
+// with UB > 0 the next item's block headers (load_heads<UB>) are loaded
+// while the current item is processed, so an item's data loads wait on no
+// other load.  body may synchronize the workgroup (the skip is uniform).
+template <int UB = 0, class F>
 __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body,
                                               int filter = F_RUNNING) {
-  int item = blockIdx.x;
-  if (item >= V.n_items) return;
-  int4 mt = V.item[item];
-  for (;;) {
-    const int next = item + (int)gridDim.x;
-    const bool more = next < V.n_items;
-    const int4 nmt = V.item[more ? next : item];
-    if (!st || wanted(st[mt.x], filter)) body(item, mt);
-    if (!more) break;
-    item = next;
-    mt = nmt;
+  const int G = (int)gridDim.x, c = lane_id();
+  for (int base = blockIdx.x; base < V.n_items; base += 64 * G) {
+    const int cnt = min(64, (V.n_items - base + G - 1) / G);
+    const int mine = base + c * G;
+    const int4 ml = V.item[c < cnt ? mine : base];
+    auto want = [&](int4 mt) { return !st || wanted(st[mt.x], filter); };
+    int4 mt = readlane4(ml, 0);
+    Heads hd{make_int4(-1, -1, -1, -1), 0ull};
+    if constexpr (UB > 0)
+      if (want(mt)) hd = load_heads<UB>(V, mt);
+    for (int k = 0; k < cnt; k++) {
+      const int4 nmt = readlane4(ml, k + 1 < cnt ? k + 1 : k);
+      Heads nhd{make_int4(-1, -1, -1, -1), 0ull};
+      if constexpr (UB > 0)
+        if (k + 1 < cnt && want(nmt)) nhd = load_heads<UB>(V, nmt);
+      if (want(mt)) body(base + k * G, mt, hd);
+      mt = nmt;
+      hd = nhd;
+    }
   }
 }
 
 // Runs f(R, u, bl) for the item's blocks, UB = ub_of<TV>() per wave at a
-// time, after issuing every load of those blocks (stencil, h^2 and aux[i]).
-// The headers of all the wave's blocks of the item are loaded up front, one
-// per lane (lane s*UB + u holds block (s*WAVES + wave)*UB + u; items have at
-// most 64*WAVES blocks), and read back with readlane.
+// time, after issuing every load of those blocks (stencil, h^2 and aux[i]);
+// hd holds the item's block headers (load_heads<UB>), read back with
+// readlane.
 template <class TV, class F>
 __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict__ x,
-                                            const TV *__restrict__ aux, int b0, int nbk, F &&f) {
+                                            const TV *__restrict__ aux, int b0, int nbk,
+                                            const Heads &hd, F &&f) {
   constexpr int UB = ub_of<TV>();
   const int c = lane_id(), w = wave_id();
-  const int jl = ((c / UB) * WAVES + w) * UB + (c % UB);
-  int4 hn = make_int4(-1, -1, -1, -1);
-  unsigned long long hm = 0;
-  if (jl < nbk) {
-    hn = V.A.nbr[b0 + jl];
-    hm = V.A.mask[b0 + jl];
-  }
-  const unsigned hm_lo = (unsigned)hm, hm_hi = (unsigned)(hm >> 32);
+  const unsigned hm_lo = (unsigned)hd.m, hm_hi = (unsigned)(hd.m >> 32);
   for (int s = 0; (s * WAVES + w) * UB < nbk; s++) {
     const int j0 = (s * WAVES + w) * UB;
     int bl[UB];
@@ -182,8 +215,7 @@ __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict
       ok[u] = j0 + u < nbk;
       bl[u] = b0 + (ok[u] ? j0 + u : j0);
       const int src = s * UB + u;
-      R.nb[u] = make_int4(__builtin_amdgcn_readlane(hn.x, src), __builtin_amdgcn_readlane(hn.y, src),
-                          __builtin_amdgcn_readlane(hn.z, src), __builtin_amdgcn_readlane(hn.w, src));
+      R.nb[u] = readlane4(hd.n, src);
       R.m[u] = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hm_lo, src) |
                ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hm_hi, src) << 32);
     }
@@ -206,7 +238,7 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
                                               T *__restrict__ nu, const Scal *st, double *p5,
                                               int filter) {
   const int c = lane_id();
-  running_items(V, st, [&](int item, int4 mt) {
+  running_items<UB64>(V, st, [&](int item, int4 mt, const Heads &hd) {
     const long long mr = V.mean_row[mt.x];
     double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
     auto store_r = [&](long i, double y, double bi) {
@@ -220,7 +252,7 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
       v[4] += rd;
     };
     if (mt.z > 0) {
-      item_blocks<double>(V, x, b, mt.y, mt.z, [&](const auto &R, int u, int) {
+      item_blocks<double>(V, x, b, mt.y, mt.z, hd, [&](const auto &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
         const long i = R.i(u, c);
@@ -247,7 +279,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
                                                  T *__restrict__ nu, const T *__restrict__ rhat,
                                                  const Scal *st, double *p2) {
   const int c = lane_id();
-  running_items(V, st, [&](int item, int4 mt) {
+  running_items<ub_of<T>()>(V, st, [&](int item, int4 mt, const Heads &hd) {
     const long long mr = V.mean_row[mt.x];
     double v[2] = {0.0, 0.0};
     auto store_y = [&](long i, double y, double rh) {
@@ -257,7 +289,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
       v[0] += rh * (double)yt;
     };
     if (mt.z > 0) {
-      item_blocks<T>(V, z1, rhat, mt.y, mt.z, [&](const auto &R, int u, int) {
+      item_blocks<T>(V, z1, rhat, mt.y, mt.z, hd, [&](const auto &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
         v[1] += R.hh[u] * R.xc(u);
@@ -277,7 +309,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
                                                 T *__restrict__ t, const T *__restrict__ sv,
                                                 const Scal *st, double *p4) {
   const int c = lane_id();
-  running_items(V, st, [&](int item, int4 mt) {
+  running_items<ub_of<T>()>(V, st, [&](int item, int4 mt, const Heads &hd) {
     const long long mr = V.mean_row[mt.x];
     double v[3] = {0.0, 0.0, 0.0};
     auto store_y = [&](long i, double y, double si) {
@@ -289,7 +321,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
       v[1] += yd * yd;
     };
     if (mt.z > 0) {
-      item_blocks<T>(V, z2, sv, mt.y, mt.z, [&](const auto &R, int u, int) {
+      item_blocks<T>(V, z2, sv, mt.y, mt.z, hd, [&](const auto &R, int u, int) {
         bool plain;
         const double y = R.y(u, c, plain);
         v[2] += R.hh[u] * R.xc(u);
@@ -308,10 +340,10 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
 __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict__ x,
                                               double *__restrict__ y, double *pm) {
   const int c = lane_id();
-  running_items(V, nullptr, [&](int item, int4 mt) {
+  running_items<UB64>(V, nullptr, [&](int item, int4 mt, const Heads &hd) {
     double v[1] = {0.0};
     if (mt.z > 0) {
-      item_blocks<double>(V, x, x, mt.y, mt.z,
+      item_blocks<double>(V, x, x, mt.y, mt.z, hd,
                           [&](const auto &R, int u, int) {
                             bool plain;
                             const double yi = R.y(u, c, plain);
@@ -347,7 +379,7 @@ __global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ 
                                                   const T *__restrict__ t,
                                                   const T *__restrict__ rhat, const Scal *st,
                                                   double *p5) {
-  running_items(V, st, [&](int item, int4 mt) {
+  running_items(V, st, [&](int item, int4 mt, const Heads &) {
     const double alpha = st[mt.x].alpha, omega = st[mt.x].omega;
     const int improved = st[mt.x].improved;
     double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
@@ -419,6 +451,26 @@ __global__ __launch_bounds__(WG) void kb_fmean(BView V, const double *__restrict
       v[1] += hh;
     }
     store_item_partials<2, 0u>(v, pf, V.n_items, item);
+  }
+}
+
+// y = x over n doubles: the streaming ceiling of the vector kernels' access
+// pattern (8-byte loads, four per thread in flight), which calibrates the
+// bandwidth and traffic figures of the solver kernels.
+__global__ __launch_bounds__(WG) void kb_copy(const double *__restrict__ x, double *__restrict__ y,
+                                              long n) {
+  for (long base = (long)blockIdx.x * 4 * WG; base < n; base += (long)gridDim.x * 4 * WG) {
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const long i = base + q * WG + threadIdx.x;
+      v[q] = i < n ? x[i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const long i = base + q * WG + threadIdx.x;
+      if (i < n) y[i] = v[q];
+    }
   }
 }
 
@@ -611,18 +663,22 @@ template <class T>
 __global__ __launch_bounds__(WG) void kb_alpha(BView V, Scal *st, const double *p2,
                                                T *__restrict__ nu, const T *__restrict__ rhat) {
   const int k = blockIdx.x;
-  if (st[k].idle) return;
+  // the state and the mean row's values are read before the reduction
+  const int idle = st[k].idle;
+  const double rho = st[k].rho, eps = st[k].eps;
+  const long long mr = V.mean_row[k];
+  const double rh_mr = mr >= 0 ? (double)rhat[mr] : 0.0;
   double o[2];
   range_totals<2, 0u>(p2, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (idle) return;
   if (threadIdx.x == 0) {
     double rn = o[0];
-    const long long mr = V.mean_row[k];
     if (mr >= 0) {
       const T m = (T)o[1];
       nu[mr] = m;
-      rn += (double)rhat[mr] * (double)m;
+      rn += rh_mr * (double)m;
     }
-    st[k].alpha = st[k].rho / (rn + st[k].eps);
+    st[k].alpha = rho / (rn + eps);
     st[k].red[2] = rn;
   }
 }
@@ -633,20 +689,24 @@ template <class T>
 __global__ __launch_bounds__(WG) void kb_omega(BView V, Scal *st, const double *p4,
                                                T *__restrict__ t, const T *__restrict__ sv) {
   const int k = blockIdx.x;
-  if (st[k].idle) return;
+  // the state and the mean row's values are read before the reduction
+  const int idle = st[k].idle;
+  const double eps = st[k].eps;
+  const long long mr = V.mean_row[k];
+  const double s_mr = mr >= 0 ? (double)sv[mr] : 0.0;
   double o[3];
   range_totals<3, 0u>(p4, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (idle) return;
   if (threadIdx.x == 0) {
     double ts = o[0], tt = o[1];
-    const long long mr = V.mean_row[k];
     if (mr >= 0) {
       const T m = (T)o[2];
       t[mr] = m;
       const double md = (double)m;
-      ts += md * (double)sv[mr];
+      ts += md * s_mr;
       tt += md * md;
     }
-    st[k].omega = ts / (tt + st[k].eps);
+    st[k].omega = ts / (tt + eps);
     st[k].red[3] = ts;
     st[k].red[4] = tt;
   }
@@ -665,14 +725,20 @@ __global__ __launch_bounds__(WG) void kb_eval(BView V, Scal *st, const double *p
                                               T *__restrict__ r, T *__restrict__ rhat, int mode,
                                               int init_check) {
   const int k = blockIdx.x;
-  if (!wanted(st[k], mode == 0 ? F_RUNNING : (mode == 1 ? F_LIVE : F_REPLACE))) return;
+  // the state is staged in LDS (a word per thread) while the partials are
+  // reduced; the reduction's barrier publishes it
+  __shared__ Scal prev;
+  static_assert(sizeof(Scal) % 8 == 0 && sizeof(Scal) / 8 <= WG, "Scal staging");
+  if (threadIdx.x < sizeof(Scal) / 8)
+    reinterpret_cast<unsigned long long *>(&prev)[threadIdx.x] =
+        reinterpret_cast<const unsigned long long *>(&st[k])[threadIdx.x];
+  const long long mr = V.mean_row[k];
   double o[NQB];
   range_totals<NQB, 1u>(p5, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (!wanted(prev, mode == 0 ? F_RUNNING : (mode == 1 ? F_LIVE : F_REPLACE))) return;
   if (threadIdx.x == 0) {
-    const Scal prev = st[k];
     double oo[NQ5];
     for (int q = 0; q < NQ5; q++) oo[q] = o[q];
-    const long long mr = V.mean_row[k];
     if (mode != 0 && mr >= 0) {
       const T rm = (T)(0.0 - o[6]);
       r[mr] = rm;
@@ -1378,6 +1444,25 @@ int cup2d_psb_spmv_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
   kb_spmv<<<G, WG, 0, h->stream>>>(V, d_x, d_y, h->d_pm);  // warm-up, untimed
   BCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int k = 0; k < reps; k++) kb_spmv<<<G, WG, 0, h->stream>>>(V, d_x, d_y, h->d_pm);
+  BCHK(h, hipEventRecord(h->ev1, h->stream));
+  BCHK(h, hipEventSynchronize(h->ev1));
+  BCHK(h, hipGetLastError());
+  float ms = 0;
+  BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *ms_per_launch = ms / reps;
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_copy_time(cup2d_psb *h, const double *d_x, double *d_y, int reps, double *ms_per_launch) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!d_x || !d_y || !ms_per_launch || reps <= 0) return bfail(h, CUP2D_PS_EINVAL, "bad argument");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "copy before cup2d_psb_set_meshes");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  const long n = (long)h->n;
+  const int G = (int)std::min<long>(16384, (n + 4 * WG - 1) / (4 * WG));
+  kb_copy<<<G, WG, 0, h->stream>>>(d_x, d_y, n);  // warm-up, untimed
+  BCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int k = 0; k < reps; k++) kb_copy<<<G, WG, 0, h->stream>>>(d_x, d_y, n);
   BCHK(h, hipEventRecord(h->ev1, h->stream));
   BCHK(h, hipEventSynchronize(h->ev1));
   BCHK(h, hipGetLastError());

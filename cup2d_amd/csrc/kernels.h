@@ -124,22 +124,27 @@ template <int NQ, unsigned MAXQ>
 __device__ void load_totals(const double *__restrict__ part, int G, double (&out)[NQ]) {
   __shared__ double fin[NQ][WAVES];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) a[q] = 0.0;
+  for (int i = threadIdx.x; i < G; i += WG) {
+    double p[NQ];  // the NQ loads of a partial are in flight together
+#pragma unroll
+    for (int q = 0; q < NQ; q++) p[q] = part[q * G + i];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) a[q] = ((MAXQ >> q) & 1) ? nmax(a[q], p[q]) : a[q] + p[q];
+  }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    double a = 0.0;
-    for (int i = threadIdx.x; i < G; i += WG) {
-      const double p = part[q * G + i];
-      a = ((MAXQ >> q) & 1) ? nmax(a, p) : a + p;
-    }
-    a = ((MAXQ >> q) & 1) ? wave_max(a) : wave_sum(a);
-    if (lane == 0) fin[q][w] = a;
+    const double r = ((MAXQ >> q) & 1) ? wave_max(a[q]) : wave_sum(a[q]);
+    if (lane == 0) fin[q][w] = r;
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    double a = fin[q][0];
-    for (int i = 1; i < WAVES; i++) a = ((MAXQ >> q) & 1) ? nmax(a, fin[q][i]) : a + fin[q][i];
-    out[q] = a;
+    double r = fin[q][0];
+    for (int i = 1; i < WAVES; i++) r = ((MAXQ >> q) & 1) ? nmax(r, fin[q][i]) : r + fin[q][i];
+    out[q] = r;
   }
 }
 

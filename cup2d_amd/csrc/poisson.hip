@@ -50,11 +50,22 @@ constexpr int SPMV_U = 2;  // blocks per wavefront in flight in the operator ker
 
 // Evaluates the state of this iteration in every workgroup (identically);
 // workgroup 0 stores it to slot `par`.
+// The previous slot is copied to LDS (a word per thread) while the partials
+// are reduced, so the evaluation waits on no global load.
+__device__ __forceinline__ void stage_scal(const Scal &src, Scal &dst) {
+  static_assert(sizeof(Scal) % 8 == 0 && sizeof(Scal) / 8 <= WG, "Scal staging");
+  if (threadIdx.x < sizeof(Scal) / 8)
+    reinterpret_cast<unsigned long long *>(&dst)[threadIdx.x] =
+        reinterpret_cast<const unsigned long long *>(&src)[threadIdx.x];
+}
+
 __device__ __forceinline__ void top_of_iteration(Scal *st, int par, const double *p5, int G5,
                                                  Scal &cur) {
+  __shared__ Scal prev;
+  stage_scal(st[1 - par], prev);
   double o[NQ5];
-  load_totals<NQ5, 1u>(p5, G5, o);
-  if (threadIdx.x == 0) cur = evaluate(st[1 - par], o);
+  load_totals<NQ5, 1u>(p5, G5, o);  // its barrier publishes prev
+  if (threadIdx.x == 0) cur = evaluate(prev, o);
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) st[par] = cur;
 }
@@ -250,11 +261,13 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
   };
   if (g < ng) load_group(g);
   const double rh_mr = mean_row >= 0 ? rhat[mean_row] : 0.0;
+  const int done = st[par].done;  // state read before the reduction's barrier
+  const double rho = st[par].rho, eps = st[par].eps;
   double o[2];
   load_totals<2, 0u>(p2, G2, o);
-  if (st[par].done) return;
+  if (done) return;
   const double rn = o[0] + (mean_row >= 0 ? rh_mr * o[1] : 0.0);
-  const double alpha = st[par].rho / (rn + st[par].eps);
+  const double alpha = rho / (rn + eps);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[par].alpha = alpha;
     st[par].red[2] = rn;
@@ -348,11 +361,13 @@ __global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restr
   };
   if (tile < ntiles) load_tile(tile);
   const double rh_mr = mean_row >= 0 ? (double)rhat[mean_row] : 0.0;
+  const int done = st[par].done;  // state read before the reduction's barrier
+  const double rho = st[par].rho, eps = st[par].eps;
   double o[2];
   load_totals<2, 0u>(p2, G2, o);
-  if (st[par].done) return;
+  if (done) return;
   const double rn = o[0] + (mean_row >= 0 ? rh_mr * o[1] : 0.0);
-  const double alpha = st[par].rho / (rn + st[par].eps);
+  const double alpha = rho / (rn + eps);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[par].alpha = alpha;
     st[par].red[2] = rn;
@@ -385,8 +400,10 @@ __global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restr
 // host sees convergence of the chunk's last iteration.
 __global__ __launch_bounds__(WG) void k_eval(Scal *st, int par, const double *p5, int G5) {
   double o[NQ5];
+  __shared__ Scal prev;
+  stage_scal(st[1 - par], prev);
   load_totals<NQ5, 1u>(p5, G5, o);
-  if (threadIdx.x == 0) st[par] = evaluate(st[1 - par], o);
+  if (threadIdx.x == 0) st[par] = evaluate(prev, o);
 }
 
 // nu = A z1; partials of (rhat, nu) and sum h^2 z1   (BiCGSTAB.cu:523-527)
@@ -506,17 +523,18 @@ __global__ __launch_bounds__(WG) void k_xrupdate(int nb, double *__restrict__ x,
   const long i0 = i < n ? i : 0;
   double xo = x[i0], a1 = (double)z1[i0], a2 = (double)z2[i0], rr = (double)r[i0],
          ti = (double)t[i0], rh = (double)rhat[i0];
+  const int done = st[par].done;  // state read before the reduction's barrier
+  const double eps = st[par].eps, alpha = st[par].alpha;
+  const int improved = st[par].improved;
   double o[4];
   load_totals<4, 0u>(p4, G4, o);
-  if (st[par].done) return;
+  if (done) return;
   double ts = o[0], tt = o[1];
   if (mean_row >= 0) {
     ts += o[2] * o[3];  // t and s at the mean row
     tt += o[2] * o[2];
   }
-  const double omega = ts / (tt + st[par].eps);
-  const double alpha = st[par].alpha;
-  const int improved = st[par].improved;
+  const double omega = ts / (tt + eps);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[par].omega = omega;
     st[par].red[3] = ts;

@@ -329,6 +329,13 @@ class BatchSolver:
                                                          ctypes.byref(ms)))
         return ms.value
 
+    def copy_time(self, d_x: DeviceArray, d_y: DeviceArray, reps=50):
+        """ms per launch of the library's streaming copy d_y = d_x over n."""
+        ms = ctypes.c_double()
+        _check_b(self.h, _lib.load().cup2d_psb_copy_time(self.h, d_x.ptr, d_y.ptr, int(reps),
+                                                         ctypes.byref(ms)))
+        return ms.value
+
 
 # ---------------------------------------------------------------------------
 # Mirror of the reference's solver plugin interface

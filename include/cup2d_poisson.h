@@ -222,6 +222,13 @@ int cup2d_psb_item_blocks(const cup2d_psb *h);
 int cup2d_psb_spmv_device(cup2d_psb *h, const double *d_x, double *d_y);
 int cup2d_psb_spmv_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
                         double *ms_per_launch);
+/* d_y = d_x over the batch's unknowns with the library's streaming copy
+ * kernel, timed over `reps` launches (HIP events, ms per launch): the
+ * bandwidth ceiling of the solver's vector access pattern, and the kernel
+ * that calibrates HBM traffic counters.  No counterpart in the reference (a
+ * measurement utility). */
+int cup2d_psb_copy_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
+                        double *ms_per_launch);
 /* Totals over the instances: unknowns, nonzeros, explicit (coarse-fine) rows. */
 int cup2d_psb_sizes(const cup2d_psb *h, long long *n, long long *nnz, long long *n_explicit);
 
