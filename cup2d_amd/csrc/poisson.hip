@@ -207,11 +207,11 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
     }
   };
   if (g < ng) load_group(g);
+  store_P(Ps, pr);  // published by the barriers of the state evaluation
   top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega;
   const int restart = cur.restart;
-  store_P(Ps, pr);
   for (; g < ng; g += nw) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
@@ -264,6 +264,7 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
   const int done = st[par].done;  // state read before the reduction's barrier
   const double rho = st[par].rho, eps = st[par].eps;
   double o[2];
+  store_P(Ps, pr);  // published by the reduction's barrier
   load_totals<2, 0u>(p2, G2, o);
   if (done) return;
   const double rn = o[0] + (mean_row >= 0 ? rh_mr * o[1] : 0.0);
@@ -272,7 +273,6 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
     st[par].alpha = alpha;
     st[par].red[2] = rn;
   }
-  store_P(Ps, pr);
   for (; g < ng; g += nw) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
