@@ -205,6 +205,22 @@ def kernel_only(PS, W, which, reps):
                           "bytes_read": s.n * 8, "bytes_written": s.n * 8,
                           "GB_per_s": 16 * s.n / (ms * 1e-3) / 1e9}))
         return
+    if which in ("batch_fp64", "batch_mixed"):
+        # 64 iterations of the batched solve (every instance running), plain
+        # stream launches so that each kernel is traced: the batched iteration
+        # kernels at their real shape
+        meshes = W.config4_meshes()
+        b = np.concatenate([W.config_rhs(m, 400 + k) for k, m in enumerate(meshes)])
+        cfg = PS.default_config(precision=PS.FP64 if which == "batch_fp64" else PS.MIXED,
+                                max_iter=64, norm=PS.NORM_L2, use_graph=0)
+        cfg.reserved_i[1] = 32
+        bs = PS.BatchSolver(cfg).set_meshes(meshes)
+        d_b = bs.device_array(b)
+        d_x = bs.device_array(np.zeros_like(b))
+        st = bs.solve_device(d_b, d_x, rel_tol=1e-14)
+        print(json.dumps({"kernel": which, "iterations": max(s["iterations"] for s in st),
+                          "ms": st[0]["ms"]}))
+        return
     if which.startswith("iter"):
         mesh = W.config2_mesh()
         s = PS.Solver(norm=PS.NORM_L2)
@@ -246,12 +262,14 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "copy", "iter0", "iter1",
+    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "iter0", "iter1",
                                               "iter2", "iter3", "iter4"], default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
                          "(spmv: operator on the bench mesh; batch_spmv: batched operator on the "
-                         "64-instance batch; iterK: iteration kernel K of the bench solve; copy: "
-                         "an elementwise 512 MiB read + 512 MiB write, the traffic counters' "
+                         "64-instance batch; batch_fp64 / batch_mixed: 64 iterations of the "
+                         "batched solve, every kernel launched from the stream; iterK: iteration "
+                         "kernel K of the bench solve; copy: the library's streaming copy over "
+                         "13,881,344 doubles, the bandwidth ceiling and the traffic counters' "
                          "calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -304,6 +322,13 @@ def main():
     _, _, other = run_variant(other_prec, max(3, args.steps // 4), 1)
     sz = s.sizes()
     ms_per_step = main["ms_per_step"]
+    # the same solve through the host-buffer entry point (b in and x out over
+    # PCIe every solve): reported beside the device-resident value, not as it
+    nh = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        s.solve(b, rel_tol=1e-10, max_restarts=100)
+    host_rate = nh / (time.perf_counter() - t0)
 
     # every iteration kernel timed alone (HIP events on the solver's stream);
     # the roofline object is the dominant one
@@ -350,6 +375,7 @@ def main():
             "fp64": fp64,
             "mixed": mixed,
             "mixed_speedup_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+            "host_buffers_solves_per_s": host_rate,
         },
         "roofline": {
             "bound": "hbm",

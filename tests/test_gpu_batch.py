@@ -195,3 +195,16 @@ def test_batch_spmv_matches_oracle(oracle, batch_meshes, mc):
         ref = oracle.spmv(A, bs.split(x)[k], mean_mode=mc, h2v=oracle.h2(m))
         assert np.abs(bs.split(y)[k] - ref).max() <= 1e-14 * np.abs(ref).max()
     assert sz["nnz"] == nnz
+
+
+def test_batch_copy_time_copies(batch_meshes):
+    """The streaming copy used as the bandwidth ceiling moves every unknown."""
+    bs = make_batch(batch_meshes)
+    x = np.random.default_rng(3).normal(size=bs.n)
+    d_x = bs.device_array(x)
+    d_y = bs.device_array(np.zeros(bs.n))
+    ms = bs.copy_time(d_x, d_y, reps=3)
+    assert ms > 0.0
+    assert np.array_equal(d_y.numpy(), x)
+    with pytest.raises(PS._lib.SolverError):
+        bs.copy_time(d_x, d_y, reps=0)
