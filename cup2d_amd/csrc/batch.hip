@@ -1075,9 +1075,10 @@ Scal marker_b(const cup2d_psb *h, int k, double tol_abs, double tol_rel, int max
   if (h->mixed) {
     // FP32-stored residual: FP64 residual replacement and restart from the
     // best iterate after each reduction by inner_tol (default 1e-2) of the
-    // recurrence residual, or a stall of 150 iterations
+    // recurrence residual, or a stall of STALL_BASE iterations (doubled per
+    // replacement that gained less than 2x)
     s.rep_factor = h->cfg.inner_tol > 0 ? h->cfg.inner_tol : 1e-2;
-    s.stall = 150;
+    s.stall = STALL_BASE;
   }
   s.tol_abs = tol_abs;
   s.tol_rel = tol_rel;

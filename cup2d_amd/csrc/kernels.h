@@ -362,6 +362,9 @@ constexpr int NQ5 = 6;  // totals of the x/r update: |r|_inf, |r|^2, (rhat,r), |
 // iteration at ~1e-7 and then makes it diverge.  The totals are corrected
 // for it here and the kernels subtract mu from r.  Deterministic:
 // recomputing it from the same inputs gives the same state.
+constexpr int STALL_BASE = 150;          // mixed precision: stall window, iterations without a 2x gain
+constexpr int STALL_MAX = 150 << 5;      // after repeated unproductive replacements
+
 __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5], bool replaced = false) {
   Scal s = prev;
   double rr = o[1], rhr = o[2], hh = o[3], amax = o[0];
@@ -383,6 +386,11 @@ __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5], bool r
       s.converged = 0;
     } else {
       s.n_rep = prev.n_rep + 1;
+      // the pass since the last replacement gained less than 2x in FP64:
+      // it was cut short on a plateau BiCGSTAB would have left (hard
+      // meshes need long runs between breakdowns), so wait twice as long
+      // before the next stall replacement
+      if (prev.stall > 0 && err > 0.5 * prev.err_rep) s.stall = min(2 * prev.stall, STALL_MAX);
     }
     s.err = s.err_opt = s.err_ref = s.err_rep = err;
     s.iter_ref = s.iter_rep = s.iter;

@@ -950,14 +950,15 @@ int run_iterations(cup2d_ps *h, int max_iter) {
   return CUP2D_PS_OK;
 }
 
-Scal marker_state(const cup2d_ps *h, double tol_abs, double tol_rel, int max_restarts, int max_iter) {
+Scal marker_state(const cup2d_ps *h, double tol_abs, double tol_rel, int max_restarts, int max_iter,
+                  int stall = STALL_BASE) {
   Scal s{};
   s.n = (double)h->op.n;
   // the projection applies to the singular Neumann operator only: with the
   // mean constraint the replaced row makes A nonsingular and 1 is no longer
   // its left null vector
   s.project = (h->mixed && h->op.mean_row < 0) ? 1 : 0;
-  s.stall = h->mixed ? 150 : 0;
+  s.stall = h->mixed ? stall : 0;
   s.tol_abs = tol_abs;
   s.tol_rel = tol_rel;
   s.eps = h->cfg.eps;
@@ -1043,7 +1044,9 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
   int total_iters = 0, total_restarts = 0, outer = 0;
   double err0 = -1, err = 0;
   bool conv = false;
-  const int max_outer = 16;
+  const int max_outer = 64;
+  int stall = STALL_BASE;  // doubled after a pass that gained less than 2x in FP64
+  double err_prev = -1;
   for (;;) {
     if (h->op.mean_row >= 0) k_hsum<double><<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_h2, h->d_pm);
     k_init<float><<<G, WG, 0, h->stream>>>(A, h->Gb, d_b, h->d_x, h->d_x, h->d_xopt, h->f_r, h->f_rhat,
@@ -1058,11 +1061,15 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
       break;
     }
     if (outer >= max_outer || total_iters >= h->cfg.max_iter) break;
-    // iterate towards the remaining reduction, at most inner_tol of this
-    // pass's initial residual (0: no limit); the FP64 check above decides
-    // whether another pass is needed
-    const double rel = std::max(tol_rel > 0 ? tol_rel * err0 / err : 0.0, h->cfg.inner_tol);
-    h->h_s[0] = h->h_s[1] = marker_state(h, tol_abs, rel, max_restarts, h->cfg.max_iter - total_iters);
+    if (err_prev > 0 && err > 0.5 * err_prev) stall = std::min(2 * stall, STALL_MAX);
+    err_prev = err;
+    // iterate towards the remaining reduction, at most inner_tol (0: 1e-2)
+    // of this pass's initial residual: the FP32 recurrence is trusted that
+    // far; the FP64 check above decides whether another pass is needed
+    const double itol = h->cfg.inner_tol > 0 ? h->cfg.inner_tol : 1e-2;
+    const double rel = std::max(tol_rel > 0 ? tol_rel * err0 / err : 0.0, itol);
+    h->h_s[0] = h->h_s[1] =
+        marker_state(h, tol_abs, rel, max_restarts, h->cfg.max_iter - total_iters, stall);
     HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
     if ((rc = run_iterations(h, h->cfg.max_iter - total_iters))) return rc;
     k_select_best<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, h->d_s, 0);

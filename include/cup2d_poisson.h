@@ -70,9 +70,7 @@ typedef struct {
   double breakdown_tol;   /* restart when rho^2 < breakdown_tol |r|^2 |rhat|^2 (BiCGSTAB.cu:483) */
   double inner_tol;       /* mixed precision: the FP32 recurrence residual may fall to inner_tol
                              times the last FP64 residual before the residual is recomputed in
-                             FP64 and BiCGSTAB restarts from the best iterate.  Single solve:
-                             0 = no limit (a pass runs until it stops improving); batch: 0 =
-                             1e-2 */
+                             FP64 and BiCGSTAB restarts from the best iterate; 0 = 1e-2 */
   double reserved_d[5];
 } cup2d_ps_config;
 
