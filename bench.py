@@ -134,13 +134,13 @@ def batch_spmv_roofline(PS, meshes, reps):
             "stream_copy_GB_per_s": 16 * sz["n"] / (ms_copy * 1e-3) / 1e9}
 
 
-def bench_config3(PS, W, steps, use_graph):
+def bench_amr6(PS, W, steps, use_graph):
     """Mixed precision at scale: FP64 and mixed single solves on the 6-level
     AMR mesh around 4 seeded disks (~16k blocks, ~1M unknowns), to 1e-10
     relative 2-norm residual, and the relative L2 difference of the mixed
     pressure from the FP64 one (same mean removal)."""
-    mesh = W.config3_mesh()
-    b = W.config_rhs(mesh, 3)
+    mesh = W.four_disk_amr6_mesh()
+    b = W.workload_rhs(mesh, 3)
     out = {"workload": "6-level AMR grid refined around 4 seeded disks (r in [0.02, 0.08]), "
                        "seeded vortex RHS, to 1e-10 rel 2-norm",
            "n_blocks": mesh.n_blocks, "n": mesh.n}
@@ -177,8 +177,8 @@ def bench_batch(PS, W, steps, use_graph):
     """The 64-instance batch: batched FP64 and mixed solves/s, the same 64
     solves one after another with the single-solve FP64 solver, and the
     roofline of the batched operator kernel (13.9M unknowns, DRAM-resident)."""
-    meshes = W.config4_meshes()
-    rhs = [W.config_rhs(m, 400 + k) for k, m in enumerate(meshes)]
+    meshes = W.batch64_meshes()
+    rhs = [W.workload_rhs(m, 400 + k) for k, m in enumerate(meshes)]
     b = np.concatenate(rhs)
     out = {"workload": "64 AMR instances, N_b log-uniform in [512, 8192] (seed 4), 5 levels, one "
                        "random disk each, seeded vortex RHS; all solved to 1e-10 rel 2-norm",
@@ -236,7 +236,7 @@ def kernel_only(PS, W, which, reps):
         # the library's streaming copy over the batch's 13,881,344 unknowns
         # (8-byte loads, the access pattern of the solver's vector kernels):
         # the bandwidth ceiling and the traffic-counter calibration
-        s = batch_spmv_handle(PS, W.config4_meshes())
+        s = batch_spmv_handle(PS, W.batch64_meshes())
         d_x = s.device_array(np.ones(s.n))
         d_y = s.device_array()
         ms = s.copy_time(d_x, d_y, reps=reps)
@@ -248,8 +248,8 @@ def kernel_only(PS, W, which, reps):
         # 64 iterations of the batched solve (every instance running), plain
         # stream launches so that each kernel is traced: the batched iteration
         # kernels at their real shape
-        meshes = W.config4_meshes()
-        b = np.concatenate([W.config_rhs(m, 400 + k) for k, m in enumerate(meshes)])
+        meshes = W.batch64_meshes()
+        b = np.concatenate([W.workload_rhs(m, 400 + k) for k, m in enumerate(meshes)])
         cfg = PS.default_config(precision=PS.FP64 if which == "batch_fp64" else PS.MIXED,
                                 max_iter=64, norm=PS.NORM_L2, use_graph=0)
         cfg.reserved_i[1] = 32
@@ -261,7 +261,7 @@ def kernel_only(PS, W, which, reps):
                           "ms": st[0]["ms"]}))
         return
     if which.startswith("iter"):
-        mesh = W.config2_mesh()
+        mesh = W.graded_disk_mesh()
         s = PS.Solver(norm=PS.NORM_L2)
         s.set_mesh(mesh)
         k = int(which[4:])
@@ -270,7 +270,7 @@ def kernel_only(PS, W, which, reps):
                           "algorithmic_bytes": kernel_bytes(k, mesh.n, mesh.n_blocks)}))
         return
     if which == "spmv":
-        mesh = W.config2_mesh()
+        mesh = W.graded_disk_mesh()
         s = PS.Solver(norm=PS.NORM_L2)
         s.set_mesh(mesh)
         sz = s.sizes()
@@ -278,7 +278,7 @@ def kernel_only(PS, W, which, reps):
         d_y = PS.DeviceArray(s, mesh.n)
         ms = s.spmv_time(d_x, d_y, kind=0, reps=reps)
     else:
-        meshes = W.config4_meshes()
+        meshes = W.batch64_meshes()
         s = batch_spmv_handle(PS, meshes)
         sz = s.sizes()
         d_x = s.device_array(np.random.default_rng(0).normal(size=s.n))
@@ -323,8 +323,8 @@ def main():
         kernel_only(PS, W, args.kernel_only, args.steps)
         return
 
-    mesh = W.config2_mesh()
-    b = W.config_rhs(mesh, 2, with_sources=True)
+    mesh = W.graded_disk_mesh()
+    b = W.workload_rhs(mesh, 2, with_sources=True)
 
     def run_variant(precision, steps, warmup):
         s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=4000, use_graph=args.graph,
@@ -438,7 +438,7 @@ def main():
         },
     }
     if not args.no_batch:
-        out["config"]["config3"] = bench_config3(PS, W, args.batch_steps, args.graph)
+        out["config"]["amr6_1M"] = bench_amr6(PS, W, args.batch_steps, args.graph)
         batch, roof_dram = bench_batch(PS, W, args.batch_steps, args.graph)
         out["config"]["batch"] = batch
         out["roofline_dram"] = roof_dram

@@ -69,7 +69,7 @@ def disk_sources(seed, cx, cy, r, count=8, sigma=0.02):
 # Meshes of the measured configurations.
 # ---------------------------------------------------------------------------
 
-def config1_mesh():
+def uniform_16x8_mesh():
     """Uniform 16x8 blocks of 8x8 cells, domain [0,2]x[0,1] (128x64 cells)."""
     return meshmod.uniform(16, 8, extent=2.0)
 
@@ -78,10 +78,10 @@ def config1_mesh():
 # of the disk centre) yields 320 blocks, far from its stated ~1.5k blocks /
 # ~100k cells; the radii are therefore scaled by 5 (0.75 and 1.5 from the
 # centre), which gives 1448 blocks / 92.7k cells.
-CONFIG2_RADIUS_SCALE = 5.0
+GRADED_RADIUS_SCALE = 5.0
 
 
-def config2_mesh(scale=CONFIG2_RADIUS_SCALE):
+def graded_disk_mesh(scale=GRADED_RADIUS_SCALE):
     cx, cy, r = 0.5, 0.5, 0.1
     bands = [0.3 * scale - r, 0.15 * scale - r]
     return meshmod.graded(16, 8, 3, 2.0, [(cx, cy, r)], bands)
@@ -120,12 +120,12 @@ def multi_disk_mesh(seed, ndisks, rmin, rmax, levels, target, tol=0.02):
     return _bisect_mesh(make, target, tol)
 
 
-def config3_mesh(seed=3, target=16384):
+def four_disk_amr6_mesh(seed=3, target=16384):
     """6 levels refined around 4 seeded disks (r in [0.02,0.08]), ~16k blocks."""
     return multi_disk_mesh(seed, 4, 0.02, 0.08, 6, target)
 
 
-def config4_meshes(seed=4, count=64, nmin=512, nmax=8192, tol=0.05):
+def batch64_meshes(seed=4, count=64, nmin=512, nmax=8192, tol=0.05):
     """`count` independent meshes, N_b log-uniform in [nmin, nmax] (seeded),
     each refined around its own random disk to within tol of its N_b."""
     rng = np.random.default_rng(seed)
@@ -136,7 +136,7 @@ def config4_meshes(seed=4, count=64, nmin=512, nmax=8192, tol=0.05):
     return out
 
 
-def config_rhs(m, seed, with_sources=False):
+def workload_rhs(m, seed, with_sources=False):
     src = None
     if with_sources:
         src = disk_sources(seed, 0.5, 0.5, 0.1)

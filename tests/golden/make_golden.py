@@ -76,7 +76,7 @@ def main():
 
     # uniform 16x8 grid (128 blocks): CSR digest, rhs, solutions without and
     # with the mean-constraint row
-    m = W.config1_mesh()
+    m = W.uniform_16x8_mesh()
     b = W.vortex_rhs(m, 1)
     A, x, _ = solve_case(m, b)
     A1, x1, _ = solve_case(m, b, mean_constraint=1)

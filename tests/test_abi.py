@@ -54,11 +54,11 @@ def test_create_without_gpu_fails_cleanly():
 @pytest.mark.parametrize("mc", [0, 1])
 def test_product_assembly_equals_oracle(oracle, which, mc):
     if which == "c1":
-        m = W.config1_mesh()
+        m = W.uniform_16x8_mesh()
     elif which == "c2":
-        m = W.config2_mesh()
+        m = W.graded_disk_mesh()
     elif which == "c3small":
-        m = W.config3_mesh(target=2500)
+        m = W.four_disk_amr6_mesh(target=2500)
     elif which == "periodic":
         m = M.graded(8, 4, 4, 2.0, [(0.1, 0.1, 0.05), (1.9, 0.9, 0.04)], [0.3, 0.15, 0.08], True, True)
     else:

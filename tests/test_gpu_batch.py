@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def batch_meshes():
-    ms = [W.config1_mesh(), M.uniform(4, 2, 2.0)]
-    ms += W.config4_meshes(seed=7, count=3, nmin=64, nmax=600)
+    ms = [W.uniform_16x8_mesh(), M.uniform(4, 2, 2.0)]
+    ms += W.batch64_meshes(seed=7, count=3, nmin=64, nmax=600)
     return ms
 
 

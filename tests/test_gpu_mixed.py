@@ -32,8 +32,8 @@ def mixed_solvers():
     def get(name, mc=0, **cfg):
         key = (name, mc, tuple(sorted(cfg.items())))
         if key not in cache:
-            m = {"config1": W.config1_mesh, "config2": W.config2_mesh,
-                 "amr6_small": lambda: W.config3_mesh(target=3000),
+            m = {"uniform16x8": W.uniform_16x8_mesh, "graded": W.graded_disk_mesh,
+                 "amr6_small": lambda: W.four_disk_amr6_mesh(target=3000),
                  "tiny": lambda: M.uniform(4, 2, 2.0)}[name]()
             s = PS.Solver(precision=PS.MIXED, mean_constraint=mc, max_iter=4000, **cfg)
             s.set_mesh(m)
@@ -42,7 +42,7 @@ def mixed_solvers():
     return get
 
 
-@pytest.mark.parametrize("name", ["tiny", "config2"])
+@pytest.mark.parametrize("name", ["tiny", "graded"])
 def test_bf16_mfma_precond_layout(oracle, mixed_solvers, name):
     m, s = mixed_solvers(name)
     P = oracle.pinv()
@@ -56,7 +56,7 @@ def test_bf16_mfma_precond_layout(oracle, mixed_solvers, name):
     assert np.abs(z - ref).max() <= 2e-2 * scale             # and P itself is P_inv to bf16
 
 
-@pytest.mark.parametrize("name", ["config1", "config2", "amr6_small"])
+@pytest.mark.parametrize("name", [("uniform16x8"), "graded", "amr6_small"])
 @pytest.mark.parametrize("norm", [PS.NORM_L2, PS.NORM_INF])
 def test_mixed_solve_reaches_fp64_residual(oracle, mixed_solvers, name, norm):
     m, s = mixed_solvers(name, norm=norm)
@@ -74,7 +74,7 @@ def test_mixed_solve_reaches_fp64_residual(oracle, mixed_solvers, name, norm):
 
 
 def test_mixed_solve_matches_oracle_1e8(oracle, mixed_solvers):
-    m, s = mixed_solvers("config2", norm=PS.NORM_L2)
+    m, s = mixed_solvers("graded", norm=PS.NORM_L2)
     A = oracle.assemble(m)
     h2 = oracle.h2(m)
     b = W.vortex_rhs(m, 4)
@@ -86,7 +86,7 @@ def test_mixed_solve_matches_oracle_1e8(oracle, mixed_solvers):
 
 
 def test_mixed_mean_constraint(oracle, mixed_solvers):
-    m, s = mixed_solvers("config2", mc=1, norm=PS.NORM_L2)
+    m, s = mixed_solvers("graded", mc=1, norm=PS.NORM_L2)
     A = oracle.assemble(m, 1)
     h2 = oracle.h2(m)
     b = W.vortex_rhs(m, 6)
@@ -100,14 +100,14 @@ def test_mixed_mean_constraint(oracle, mixed_solvers):
 
 
 def test_mixed_deterministic(mixed_solvers):
-    m, s = mixed_solvers("config2")
+    m, s = mixed_solvers("graded")
     b = W.vortex_rhs(m, 8)
     x1, st1 = s.solve(b, rel_tol=1e-10)
     x2, st2 = s.solve(b, rel_tol=1e-10)
     assert np.array_equal(x1, x2) and st1 == {**st2, "ms": st1["ms"]}
 
 
-def test_config3_full_size_mixed_matches_fp64(oracle):
+def test_four_disk_amr6_full_size_mixed_matches_fp64(oracle):
     """The configuration quoted for mixed precision at scale: the 6-level AMR
     mesh around 4 disks (~16k blocks, ~1M unknowns), whose BiCGSTAB runs are
     long and full of breakdown restarts.  Solved to 1e-11, the mixed solve
@@ -118,10 +118,10 @@ def test_config3_full_size_mixed_matches_fp64(oracle):
     1e-8 relative L2 (the oracle's own solve at this size is out of a test's
     time budget; the FP64 GPU solver is pinned to it on every smaller
     mesh)."""
-    m = W.config3_mesh()
+    m = W.four_disk_amr6_mesh()
     assert 15000 <= m.n_blocks <= 17500
     A = oracle.assemble(m)
-    b = W.config_rhs(m, 3)
+    b = W.workload_rhs(m, 3)
     xs = {}
     for prec in (PS.FP64, PS.MIXED):
         s = PS.Solver(precision=prec, norm=PS.NORM_L2, max_iter=8000).set_mesh(m)
@@ -135,12 +135,12 @@ def test_config3_full_size_mixed_matches_fp64(oracle):
 
 
 @pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
-def test_config3_full_size_batched(oracle, precision):
+def test_four_disk_amr6_full_size_batched(oracle, precision):
     """The same mesh as a one-instance batch: the batched solver's device-side
     residual replacement converges on it too (FP64 residual to 1e-10)."""
-    m = W.config3_mesh()
+    m = W.four_disk_amr6_mesh()
     A = oracle.assemble(m)
-    b = W.config_rhs(m, 3)
+    b = W.workload_rhs(m, 3)
     c = PS.default_config(precision=precision, norm=PS.NORM_L2, max_iter=8000)
     bs = PS.BatchSolver(c).set_meshes([m])
     x, st = bs.solve(b, rel_tol=1e-10)

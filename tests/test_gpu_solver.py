@@ -25,9 +25,9 @@ def _tiny_cf():
 
 MESHES = {
     "tiny_cf": _tiny_cf,
-    "config1": W.config1_mesh,
-    "config2": W.config2_mesh,
-    "amr6_small": lambda: W.config3_mesh(target=3000),
+    "uniform16x8": W.uniform_16x8_mesh,
+    "graded": W.graded_disk_mesh,
+    "amr6_small": lambda: W.four_disk_amr6_mesh(target=3000),
     "periodic": lambda: M.graded(8, 4, 4, 2.0, [(0.1, 0.1, 0.05), (1.9, 0.9, 0.04)],
                                  [0.3, 0.15, 0.08], True, True),
 }
@@ -63,7 +63,7 @@ def test_spmv_matches_oracle(oracle, solvers, name, mc):
         assert np.abs(y - ref).max() <= 1e-14 * scale, (name, kind)
 
 
-@pytest.mark.parametrize("name", ["tiny_cf", "config2"])
+@pytest.mark.parametrize("name", ["tiny_cf", "graded"])
 def test_precond_matches_oracle(oracle, solvers, name):
     m, s = solvers(name)
     P = oracle.pinv()
@@ -88,7 +88,7 @@ def _solve_case(oracle, solvers, name, mc, norm, seed=1, rel_tol=1e-10):
     return m, A, h2, bb, x, st, xr, sr
 
 
-@pytest.mark.parametrize("name", ["tiny_cf", "config1", "config2", "amr6_small", "periodic"])
+@pytest.mark.parametrize("name", ["tiny_cf", ("uniform16x8"), "graded", "amr6_small", "periodic"])
 @pytest.mark.parametrize("norm", [PS.NORM_INF, PS.NORM_L2])
 def test_solve_reaches_residual(oracle, solvers, name, norm):
     """Stopping at 1e-10 relative residual: the independently recomputed true
@@ -104,7 +104,7 @@ def test_solve_reaches_residual(oracle, solvers, name, norm):
     assert err <= 1e-6, err
 
 
-@pytest.mark.parametrize("name", ["tiny_cf", "config1", "config2", "amr6_small", "periodic"])
+@pytest.mark.parametrize("name", ["tiny_cf", ("uniform16x8"), "graded", "amr6_small", "periodic"])
 def test_solve_matches_oracle_1e8(oracle, solvers, name):
     """Solved to 1e-12 relative residual, the pressure agrees with the FP64
     oracle to 1e-8 relative L2 after the same mean removal."""
@@ -116,7 +116,7 @@ def test_solve_matches_oracle_1e8(oracle, solvers, name):
     assert err <= 1e-8, err
 
 
-@pytest.mark.parametrize("name", ["tiny_cf", "config2"])
+@pytest.mark.parametrize("name", ["tiny_cf", "graded"])
 def test_solve_mean_constraint_matches_oracle(oracle, solvers, name):
     m, A, h2, bb, x, st, xr, sr = _solve_case(oracle, solvers, name, 1, PS.NORM_L2, rel_tol=1e-12)
     assert st["converged"], st
@@ -128,7 +128,7 @@ def test_solve_mean_constraint_matches_oracle(oracle, solvers, name):
 def test_iteration_trajectory_close_to_oracle(oracle, solvers):
     """Same algorithm, same inputs: iteration and restart counts of the GPU
     solve track the oracle's reference-algorithm run."""
-    m, s = solvers("config1", 0, norm=PS.NORM_INF)
+    m, s = solvers(("uniform16x8"), 0, norm=PS.NORM_INF)
     A = oracle.assemble(m)
     b = W.vortex_rhs(m, 1)
     _, st = s.solve(b, rel_tol=1e-8, max_restarts=100)
@@ -139,7 +139,7 @@ def test_iteration_trajectory_close_to_oracle(oracle, solvers):
 
 def test_reference_eps_stagnates_like_oracle(oracle):
     """eps = 1e-21 (BiCGSTAB.cu:427) stalls on the h^2-scaled RHS in both."""
-    m = W.config1_mesh()
+    m = W.uniform_16x8_mesh()
     s = PS.Solver(eps=1e-21, norm=PS.NORM_L2)
     s.set_mesh(m)
     b = W.vortex_rhs(m, 1)
@@ -151,7 +151,7 @@ def test_reference_eps_stagnates_like_oracle(oracle):
 
 
 def test_zero_rhs_returns_mean_free_guess(solvers):
-    m, s = solvers("config1")
+    m, s = solvers(("uniform16x8"))
     x0 = np.random.default_rng(2).normal(size=m.n)
     b = np.zeros(m.n)
     # x0 is not a solution of A x = 0 unless constant: use a constant guess
@@ -162,7 +162,7 @@ def test_zero_rhs_returns_mean_free_guess(solvers):
 
 
 def test_iteration_cap_returns_best_iterate(oracle, solvers):
-    m, s0 = solvers("config2")
+    m, s0 = solvers("graded")
     s = PS.Solver(max_iter=7)
     s.set_mesh(m)
     b = W.vortex_rhs(m, 4)
@@ -174,7 +174,7 @@ def test_iteration_cap_returns_best_iterate(oracle, solvers):
 
 
 def test_deterministic_and_graph_equivalent(solvers):
-    m, s = solvers("config2")
+    m, s = solvers("graded")
     b = W.vortex_rhs(m, 7)
     x1, st1 = s.solve(b, rel_tol=1e-10)
     x2, st2 = s.solve(b, rel_tol=1e-10)
@@ -190,7 +190,7 @@ def test_deterministic_and_graph_equivalent(solvers):
 
 
 def test_device_solve_equals_host_solve(solvers):
-    m, s = solvers("config2")
+    m, s = solvers("graded")
     b = W.vortex_rhs(m, 8)
     xh, sth = s.solve(b, rel_tol=1e-10)
     db = PS.DeviceArray.from_host(s, b)
@@ -204,7 +204,7 @@ def test_device_solve_equals_host_solve(solvers):
 def test_poisson_solver_interface(oracle):
     """makePoissonSolver / solve(input, output) as PressureSingle calls it
     (PressureSingle.cpp:836-851): x0 = 0, output receives the mean-free Δp."""
-    m = W.config2_mesh()
+    m = W.graded_disk_mesh()
     sim = PS.SimulationData(poissonSolver="cuda_iterative", PoissonTol=0.0, PoissonTolRel=1e-10, step=20)
     sim.tmp = PS.ScalarGrid(m)
     sim.pres = PS.ScalarGrid(m)
@@ -225,7 +225,7 @@ def test_poisson_solver_interface(oracle):
 def test_kernel_time_then_solve(oracle, precision):
     """Timing the iteration kernels alone leaves the handle usable: the next
     solve starts afresh and is correct."""
-    m = W.config1_mesh()
+    m = W.uniform_16x8_mesh()
     s = PS.Solver(precision=precision, norm=PS.NORM_L2, max_iter=4000).set_mesh(m)
     for k in range(5):
         ms = s.kernel_time(k, reps=5)

@@ -35,20 +35,20 @@ def test_cover_and_balance(seed):
 
 
 def test_uniform_counts():
-    m = W.config1_mesh()
+    m = W.uniform_16x8_mesh()
     assert m.n_blocks == 128 and m.n == 8192
     assert np.allclose(m.h(), 1.0 / 64)
 
 
 def test_config_sizes():
-    m2 = W.config2_mesh()
+    m2 = W.graded_disk_mesh()
     assert 1300 <= m2.n_blocks <= 1700
     assert set(m2.level.tolist()) == {0, 1, 2} or set(m2.level.tolist()) == {1, 2}
 
 
 def test_hilbert_locality():
     """Consecutive blocks are spatial neighbours far more often than not."""
-    m = W.config2_mesh()
+    m = W.graded_disk_mesh()
     h = m.h() * 8
     cx = (m.bx + 0.5) * h
     cy = (m.by + 0.5) * h

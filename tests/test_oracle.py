@@ -216,9 +216,9 @@ def test_bicgstab_mean_row_vs_dense(oracle):
 
 
 def test_pipelined_matches_plain(oracle):
-    m = W.config1_mesh()
+    m = W.uniform_16x8_mesh()
     A = oracle.assemble(m)
-    b = W.config_rhs(m, 1)
+    b = W.workload_rhs(m, 1)
     h2 = oracle.h2(m)
     x1, s1 = oracle.bicgstab(A, b, rel_tol=1e-11, norm="l2")
     x2, s2 = oracle.pipelined_bicgstab(A, m, b, rel_tol=1e-11)
@@ -232,9 +232,9 @@ def test_pipelined_matches_plain(oracle):
 def test_reference_eps_stagnates(oracle):
     """With the reference's eps = 1e-21 (BiCGSTAB.cu:427) the h^2-scaled system
     stagnates far above 1e-10; with eps = 1e-100 it converges."""
-    m = W.config1_mesh()
+    m = W.uniform_16x8_mesh()
     A = oracle.assemble(m)
-    b = W.config_rhs(m, 1)
+    b = W.workload_rhs(m, 1)
     _, s21 = oracle.bicgstab(A, b, rel_tol=1e-10, norm="l2", eps=1e-21)
     _, s100 = oracle.bicgstab(A, b, rel_tol=1e-10, norm="l2", eps=1e-100)
     assert not s21["converged"]
