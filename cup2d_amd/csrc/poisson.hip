@@ -222,14 +222,21 @@ __global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict
       }
       vs[w][j * 64 + c] = ok[j] ? pi : 0.0;
     }
+    long io[NB];
+    bool oo[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      io[j] = ii[j];
+      oo[j] = ok[j];
+    }
+    if (g + nw < ng) load_group(g + nw);  // in flight during the product
     wave_sync();
     double z[NB];
     prec_group<NB>(Ps, vs[w], c, z);
     wave_sync();
 #pragma unroll
     for (int j = 0; j < NB; j++)
-      if (ok[j]) z1[ii[j]] = z[j];
-    if (g + nw < ng) load_group(g + nw);
+      if (oo[j]) z1[io[j]] = z[j];
   }
 }
 
@@ -284,14 +291,21 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
       }
       vs[w][j * 64 + c] = ok[j] ? si : 0.0;
     }
+    long io[NB];
+    bool oo[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      io[j] = ii[j];
+      oo[j] = ok[j];
+    }
+    if (g + nw < ng) load_group(g + nw);  // in flight during the product
     wave_sync();
     double z[NB];
     prec_group<NB>(Ps, vs[w], c, z);
     wave_sync();
 #pragma unroll
     for (int j = 0; j < NB; j++)
-      if (ok[j]) z2[ii[j]] = z[j];
-    if (g + nw < ng) load_group(g + nw);
+      if (oo[j]) z2[io[j]] = z[j];
   }
 }
 
@@ -331,10 +345,12 @@ __global__ __launch_bounds__(WG) void k_pupdate_mf(int nb, const bf16x8 *__restr
   for (; tile < ntiles; tile += nw) {
     float pin[16];
     mf_pupdate(in, r, rhat, p, base, q, ok, beta, omega, mu, restart, pin);
+    const bool oo = ok;
+    const long bo = base;
+    if (tile + nw < ntiles) load_tile(tile + nw);  // in flight during the product
     float z[16];
     prec_mfma(P, pin, z);
-    if (ok) st16_out(z1, base, q, z);
-    if (tile + nw < ntiles) load_tile(tile + nw);
+    if (oo) st16_out(z1, bo, q, z);
   }
 }
 
@@ -385,10 +401,12 @@ __global__ __launch_bounds__(WG) void k_supdate_mf(int nb, const bf16x8 *__restr
 #pragma unroll
     for (int k = 0; k < 16; k++) sv[k] = ok ? (float)((double)rv[k] - alpha * (double)nv[k]) : 0.f;
     if (ok) st16_in(r, base, q, sv);
+    const bool oo = ok;
+    const long bo = base;
+    if (tile + nw < ntiles) load_tile(tile + nw);  // in flight during the product
     float z[16];
     prec_mfma(P, sv, z);
-    if (ok) st16_out(z2, base, q, z);
-    if (tile + nw < ntiles) load_tile(tile + nw);
+    if (oo) st16_out(z2, bo, q, z);
   }
 }
 
@@ -406,6 +424,52 @@ __global__ __launch_bounds__(WG) void k_eval(Scal *st, int par, const double *p5
   if (threadIdx.x == 0) st[par] = evaluate(prev, o);
 }
 
+// Runs f(R, u) for the blocks of the wave in the block part of an operator
+// kernel (SPMV_U blocks per wave at a time, grid-stride over Gb
+// workgroups); the next group's loads are issued before the current group
+// is processed.  Returns false, having only issued the first loads, when
+// `stop` holds.
+template <class TV, class F>
+__device__ __forceinline__ bool block_groups(const OpView &A, int Gb, const TV *__restrict__ x,
+                                             const TV *__restrict__ aux, bool stop, F &&f) {
+  const int c = lane_id(), stride = Gb * WAVES;
+  int g = blockIdx.x * WAVES + wave_id();
+  auto index = [&](int gg, int (&bl)[SPMV_U], bool (&ok)[SPMV_U]) {
+#pragma unroll
+    for (int u = 0; u < SPMV_U; u++) {
+      ok[u] = gg * SPMV_U + u < A.nb;
+      bl[u] = ok[u] ? gg * SPMV_U + u : gg * SPMV_U;
+    }
+  };
+  if (g * SPMV_U >= A.nb) return !stop;
+  int bl[SPMV_U];
+  bool ok[SPMV_U];
+  index(g, bl, ok);
+  BlockRows<SPMV_U, TV> R;
+  R.load(A, x, aux, bl, c);
+  if (stop) return false;
+  for (;;) {
+    const int gn = g + stride;
+    const bool more = gn * SPMV_U < A.nb;
+    int bln[SPMV_U];
+    bool okn[SPMV_U];
+    BlockRows<SPMV_U, TV> Rn;
+    if (more) {
+      index(gn, bln, okn);
+      Rn.load(A, x, aux, bln, c);
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_U; u++)
+      if (ok[u]) f(R, u);
+    if (!more) break;
+    R = Rn;
+#pragma unroll
+    for (int u = 0; u < SPMV_U; u++) ok[u] = okn[u];
+    g = gn;
+  }
+  return true;
+}
+
 // nu = A z1; partials of (rhat, nu) and sum h^2 z1   (BiCGSTAB.cu:523-527)
 template <class T>
 __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__restrict__ z1,
@@ -421,26 +485,13 @@ __global__ __launch_bounds__(WG) void k_spmv_nu(OpView A, int Gb, const T *__res
   };
   if ((int)blockIdx.x < Gb) {
     const int c = lane_id();
-    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
-      int bl[SPMV_U];
-      bool ok[SPMV_U];
-#pragma unroll
-      for (int u = 0; u < SPMV_U; u++) {
-        ok[u] = g * SPMV_U + u < A.nb;
-        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
-      }
-      BlockRows<SPMV_U, T> R;
-      R.load(A, z1, rhat, bl, c);
-      if (done) return;
-#pragma unroll
-      for (int u = 0; u < SPMV_U; u++) {
-        if (!ok[u]) continue;
-        bool plain;
-        const double y = R.y(u, c, plain);
-        if (plain) store_y(R.i(u, c), y, R.a(u));
-        v[1] += R.hh[u] * R.xc(u);
-      }
-    }
+    if (!block_groups<T>(A, Gb, z1, rhat, done, [&](const BlockRows<SPMV_U, T> &R, int u) {
+          bool plain;
+          const double y = R.y(u, c, plain);
+          if (plain) store_y(R.i(u, c), y, R.a(u));
+          v[1] += R.hh[u] * R.xc(u);
+        }))
+      return;
   } else {
     const int ge = gridDim.x - Gb;
     if (done) return;
@@ -473,26 +524,13 @@ __global__ __launch_bounds__(WG) void k_spmv_t(OpView A, int Gb, const T *__rest
   };
   if ((int)blockIdx.x < Gb) {
     const int c = lane_id();
-    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
-      int bl[SPMV_U];
-      bool ok[SPMV_U];
-#pragma unroll
-      for (int u = 0; u < SPMV_U; u++) {
-        ok[u] = g * SPMV_U + u < A.nb;
-        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
-      }
-      BlockRows<SPMV_U, T> R;
-      R.load(A, z2, sv, bl, c);
-      if (done) return;
-#pragma unroll
-      for (int u = 0; u < SPMV_U; u++) {
-        if (!ok[u]) continue;
-        bool plain;
-        const double y = R.y(u, c, plain);
-        if (plain) store_y(R.i(u, c), y, R.a(u));
-        v[2] += R.hh[u] * R.xc(u);
-      }
-    }
+    if (!block_groups<T>(A, Gb, z2, sv, done, [&](const BlockRows<SPMV_U, T> &R, int u) {
+          bool plain;
+          const double y = R.y(u, c, plain);
+          if (plain) store_y(R.i(u, c), y, R.a(u));
+          v[2] += R.hh[u] * R.xc(u);
+        }))
+      return;
   } else {
     const int ge = gridDim.x - Gb;
     if (done) return;
@@ -625,25 +663,12 @@ __global__ __launch_bounds__(WG) void k_spmv(OpView A, int Gb, const T *__restri
   double v[1] = {0.0};
   if ((int)blockIdx.x < Gb) {
     const int c = lane_id();
-    for (int g = blockIdx.x * WAVES + wave_id(); g * SPMV_U < A.nb; g += Gb * WAVES) {
-      int bl[SPMV_U];
-      bool ok[SPMV_U];
-#pragma unroll
-      for (int u = 0; u < SPMV_U; u++) {
-        ok[u] = g * SPMV_U + u < A.nb;
-        bl[u] = ok[u] ? g * SPMV_U + u : g * SPMV_U;
-      }
-      BlockRows<SPMV_U, T> R;
-      R.load(A, x, x, bl, c);
-#pragma unroll
-      for (int u = 0; u < SPMV_U; u++) {
-        if (!ok[u]) continue;
-        bool plain;
-        const double yi = R.y(u, c, plain);
-        if (plain) y[R.i(u, c)] = (T)yi;
-        v[0] += R.hh[u] * R.xc(u);
-      }
-    }
+    block_groups<T>(A, Gb, x, x, false, [&](const BlockRows<SPMV_U, T> &R, int u) {
+      bool plain;
+      const double yi = R.y(u, c, plain);
+      if (plain) y[R.i(u, c)] = (T)yi;
+      v[0] += R.hh[u] * R.xc(u);
+    });
   } else {
     const int ge = gridDim.x - Gb;
     for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG)
