@@ -21,6 +21,10 @@ pytestmark = pytest.mark.gpu
 def batch_meshes():
     ms = [W.uniform_16x8_mesh(), M.uniform(4, 2, 2.0)]
     ms += W.batch64_meshes(seed=7, count=3, nmin=64, nmax=600)
+    # periodic in both directions, 4 levels: wrap-around same-level and
+    # coarse-fine neighbours inside a batch
+    ms.append(M.graded(8, 4, 4, 2.0, [(0.1, 0.1, 0.05), (1.9, 0.9, 0.04)], [0.3, 0.15, 0.08],
+                       True, True))
     return ms
 
 
