@@ -920,7 +920,7 @@ void launch_iteration(cup2d_ps *h, int par) {
 
 // Iterations per chunk: even, so that every chunk starts on state slot 0.
 int chunk_of(const cup2d_ps *h) {
-  int c = h->cfg.chunk > 0 ? h->cfg.chunk : 16;
+  int c = h->cfg.chunk > 0 ? h->cfg.chunk : 32;
   return c + (c & 1);
 }
 
