@@ -104,7 +104,8 @@ void ghost(RowAcc &row, long long close, long long far_, long long c, int t, Ste
 
 }  // namespace
 
-bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err) {
+bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err,
+                    bool with_csr) {
   const long long nb = m.n_blocks;
   if (nb <= 0 || m.bpdx <= 0 || m.bpdy <= 0 || m.levels <= 0 || m.levels > 24 || m.extent <= 0 ||
       !m.level || !m.bx || !m.by) {
@@ -195,9 +196,12 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
   }
 
   // ---- rows -----------------------------------------------------------------
-  op.rp.assign(op.n + 1, 0);
-  op.col.reserve(op.n * 5 + op.n / 4);
-  op.val.reserve(op.n * 5 + op.n / 4);
+  if (with_csr) {
+    op.rp.assign(op.n + 1, 0);
+    op.col.reserve(op.n * 5 + op.n / 4);
+    op.val.reserve(op.n * 5 + op.n / 4);
+  }
+  long long nnz = 0;
   op.exc_mask.assign(nb, 0);
   op.exc_bptr.assign(nb + 1, 0);
   op.exc_rptr.push_back(0);
@@ -210,6 +214,12 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
         const bool edge[4] = {ix == 0, ix == BS - 1, iy == 0, iy == BS - 1};
         bool cf = false;
         for (int f = 0; f < 4; f++) cf |= edge[f] && (fb[f].kind == COARSER || fb[f].kind == FINER);
+        if (!with_csr && !cf) {  // a plain row only counts: self + its non-wall faces
+          int cnt = 1;
+          for (int f = 0; f < 4; f++) cnt += (!edge[f] || fb[f].kind == SAME) ? 1 : 0;
+          nnz += cnt;
+          continue;
+        }
         RowAcc row;
         if (!edge[0] && !edge[1] && !edge[2] && !edge[3]) {
           // interior cell: ascending columns (ExpAMRSolver.cpp:270-277)
@@ -272,15 +282,18 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
             }
           }
         }
+        nnz += row.n;
         // sort columns ascending (the reference keeps rows in a std::map)
         int order[48];
         for (int k = 0; k < row.n; k++) order[k] = k;
         std::sort(order, order + row.n, [&](int a, int c) { return row.c[a] < row.c[c]; });
-        for (int k = 0; k < row.n; k++) {
-          op.col.push_back((int)row.c[order[k]]);
-          op.val.push_back(row.v[order[k]]);
+        if (with_csr) {
+          for (int k = 0; k < row.n; k++) {
+            op.col.push_back((int)row.c[order[k]]);
+            op.val.push_back(row.v[order[k]]);
+          }
+          op.rp[self + 1] = (int)op.col.size();
         }
-        op.rp[self + 1] = (int)op.col.size();
         if (cf) {
           op.exc_mask[b] |= uint64_t(1) << (iy * BS + ix);
           op.exc_cell.push_back(iy * BS + ix);
@@ -293,6 +306,7 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
       }
     op.exc_bptr[b + 1] = (int)op.exc_cell.size();
   }
+  op.nnz_rows = nnz;
 
   // ---- h^2 and the mean-constraint row ---------------------------------------
   op.h2.resize(nb);

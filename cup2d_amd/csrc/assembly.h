@@ -51,7 +51,8 @@ struct Operator {
   long long mean_row = -1;
   long long corner_block = -1;   // block with index (0,0), whose cell (0,0) is the mean row
 
-  long long nnz() const { return (long long)col.size(); }
+  long long nnz_rows = 0;        // nonzeros of the whole matrix (also without the CSR copy)
+  long long nnz() const { return nnz_rows; }
   long long n_exc() const { return (long long)exc_cell.size(); }
 };
 
@@ -59,7 +60,9 @@ struct Operator {
 // the operator cannot be built on (out-of-range or duplicate blocks, a face
 // whose neighbour is neither same-level, one level coarser nor one level
 // finer).
-bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err);
+// with_csr = false skips the plain CSR copy (rp, col, val stay empty).
+bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::string &err,
+                    bool with_csr = true);
 
 // Explicit rows in ELL form.  ell_width: most entries of an explicit row
 // (0 when there are none).  EllRows::append adds op's explicit rows with

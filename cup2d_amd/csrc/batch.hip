@@ -33,6 +33,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -1261,16 +1262,30 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
   bfree_pool(h->unit_allocs);
   bfree_pool(h->allocs);
   h->ready = false;
+  // the instances' operators are independent: built on host threads, without
+  // the plain CSR copy (the batch never reads it)
   std::vector<cup2d::Operator> ops(n_inst);
+  std::vector<std::string> errs(n_inst);
+  std::vector<char> ok(n_inst, 0);
+  {
+    const int nt = (int)std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
+    auto work = [&](int t) {
+      for (int k = t; k < n_inst; k += nt) {
+        const cup2d_mesh_desc &m = meshes[k];
+        MeshDesc md{m.bpdx, m.bpdy, m.levels, m.periodic_x, m.periodic_y, m.extent, m.n_blocks,
+                    m.level, m.bx, m.by};
+        ok[k] = build_operator(md, h->cfg.mean_constraint, ops[k], errs[k], false) ? 1 : 0;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < std::min(nt, n_inst); t++) pool.emplace_back(work, t);
+    work(0);
+    for (std::thread &th : pool) th.join();
+  }
   std::vector<long long> boff(n_inst + 1, 0);
   long long nnz = 0;
   for (int k = 0; k < n_inst; k++) {
-    const cup2d_mesh_desc &m = meshes[k];
-    MeshDesc md{m.bpdx, m.bpdy, m.levels, m.periodic_x, m.periodic_y, m.extent, m.n_blocks,
-                m.level, m.bx, m.by};
-    std::string err;
-    if (!build_operator(md, h->cfg.mean_constraint, ops[k], err))
-      return bfail(h, CUP2D_PS_EINVAL, "mesh " + std::to_string(k) + ": " + err);
+    if (!ok[k]) return bfail(h, CUP2D_PS_EINVAL, "mesh " + std::to_string(k) + ": " + errs[k]);
     boff[k + 1] = boff[k] + ops[k].nb;
     nnz += ops[k].nnz();
   }
