@@ -527,10 +527,10 @@ __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__rest
     const int k = tm.x, b0 = tm.y, tnb = tm.z;
     const double beta = st[k].beta, omega = st[k].omega;
     const int restart = st[k].restart;
-    for (int g = 0; g < tnb; g += 4) {
-      long ii[4];
-      bool ok[4];
-      double rv[4], pv[4] = {}, nv[4] = {};
+    long ii[4];
+    bool ok[4];
+    double rv[4], pv[4] = {}, nv[4] = {};
+    auto load = [&](int g) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         ok[j] = g + j < tnb;
@@ -543,6 +543,9 @@ __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__rest
           pv[j] = p[ii[j]];
           nv[j] = nu[ii[j]];
         }
+    };
+    load(0);
+    for (int g = 0; g < tnb; g += 4) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const double pi = restart ? rv[j] : rv[j] + beta * (pv[j] - omega * nv[j]);
@@ -552,13 +555,21 @@ __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__rest
         }
         vs[w][j * 64 + c] = ok[j] ? pi : 0.0;
       }
+      long io[4];
+      bool oo[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        io[j] = ii[j];
+        oo[j] = ok[j];
+      }
+      if (g + 4 < tnb) load(g + 4);  // the next group's loads, in flight during the product
       wave_sync();
       double z[4];
       prec_group<4>(Ps, vs[w], c, z);
       wave_sync();
 #pragma unroll
       for (int j = 0; j < 4; j++)
-        if (ok[j]) z1[ii[j]] = z[j];
+        if (oo[j]) z1[io[j]] = z[j];
     }
   });
 }
@@ -575,10 +586,10 @@ __global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__rest
   running_tiles(V, st, [&](int4 tm) {
     const int k = tm.x, b0 = tm.y, tnb = tm.z;
     const double alpha = st[k].alpha;
-    for (int g = 0; g < tnb; g += 4) {
-      long ii[4];
-      bool ok[4];
-      double rv[4], nv[4];
+    long ii[4];
+    bool ok[4];
+    double rv[4], nv[4];
+    auto load = [&](int g) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         ok[j] = g + j < tnb;
@@ -586,19 +597,30 @@ __global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__rest
         rv[j] = r[ii[j]];
         nv[j] = nu[ii[j]];
       }
+    };
+    load(0);
+    for (int g = 0; g < tnb; g += 4) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const double si = rv[j] - alpha * nv[j];
         if (ok[j]) r[ii[j]] = si;
         vs[w][j * 64 + c] = ok[j] ? si : 0.0;
       }
+      long io[4];
+      bool oo[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        io[j] = ii[j];
+        oo[j] = ok[j];
+      }
+      if (g + 4 < tnb) load(g + 4);  // the next group's loads, in flight during the product
       wave_sync();
       double z[4];
       prec_group<4>(Ps, vs[w], c, z);
       wave_sync();
 #pragma unroll
       for (int j = 0; j < 4; j++)
-        if (ok[j]) z2[ii[j]] = z[j];
+        if (oo[j]) z2[io[j]] = z[j];
     }
   });
 }
