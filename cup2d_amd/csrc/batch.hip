@@ -861,6 +861,20 @@ int bfail(cup2d_psb *h, int code, const std::string &msg) {
   return code;
 }
 
+// Runs an entry point's body so that no C++ exception crosses the C ABI: a
+// failed host allocation becomes CUP2D_PS_ENOMEM, any other exception EINVAL,
+// with its text as the last error.
+template <class F>
+int bguarded(cup2d_psb *h, F &&body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return bfail(h, CUP2D_PS_ENOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return bfail(h, CUP2D_PS_EINVAL, e.what());
+  }
+}
+
 #define BCHK(h, expr)                                                                 \
   do {                                                                                \
     hipError_t _e = (expr);                                                           \
@@ -1146,8 +1160,8 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
 // FP32-stored Krylov vectors with FP64 residual replacement and restart
 // (launch_replacement_b at the start of every chunk), so the stopping test is
 // met on an FP64 residual.
-int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
-            double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+int solve_b_body(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out,
+                 double tol_abs, double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
   if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "solve before cup2d_psb_set_meshes");
   BCHK(h, hipSetDevice(h->cfg.device));
   const int K = h->K, max_iter = h->cfg.max_iter;
@@ -1214,6 +1228,13 @@ int time_units(cup2d_psb *h, const double *d_b, double &ms_per_iter) {
   return CUP2D_PS_OK;
 }
 
+int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
+            double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+  return bguarded(h, [&] {
+    return solve_b_body(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, stats);
+  });
+}
+
 // Tuned item sizes of this process, by (precision, instances, blocks).
 std::map<std::tuple<int, int, long long>, int> g_tuned;
 
@@ -1229,7 +1250,7 @@ bool config_ok(const cup2d_ps_config &c) {
 
 extern "C" {
 
-int cup2d_psb_create(const cup2d_ps_config *cfg, cup2d_psb **out) {
+static int create_body(const cup2d_ps_config *cfg, cup2d_psb **out) {
   if (!out) return bfail(nullptr, CUP2D_PS_EINVAL, "out is NULL");
   *out = nullptr;
   cup2d_ps_config c;
@@ -1274,7 +1295,7 @@ int cup2d_psb_destroy(cup2d_psb *h) {
   return CUP2D_PS_OK;
 }
 
-int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
+static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
   if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
   if (!meshes || n_inst <= 0 || n_inst > MAX_INST)
     return bfail(h, CUP2D_PS_EINVAL, "need 1.." + std::to_string(MAX_INST) + " meshes");
@@ -1296,12 +1317,24 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
         const cup2d_mesh_desc &m = meshes[k];
         MeshDesc md{m.bpdx, m.bpdy, m.levels, m.periodic_x, m.periodic_y, m.extent, m.n_blocks,
                     m.level, m.bx, m.by};
-        ok[k] = build_operator(md, h->cfg.mean_constraint, ops[k], errs[k], false) ? 1 : 0;
+        try {
+          ok[k] = build_operator(md, h->cfg.mean_constraint, ops[k], errs[k], false) ? 1 : 0;
+        } catch (const std::exception &e) {  // e.g. out of host memory
+          errs[k] = e.what();
+          ok[k] = 0;
+        }
       }
     };
     std::vector<std::thread> pool;
-    for (int t = 1; t < std::min(nt, n_inst); t++) pool.emplace_back(work, t);
-    work(0);
+    std::vector<int> mine(1, 0);  // strides this thread runs: its own, and any not started
+    for (int t = 1; t < std::min(nt, n_inst); t++) {
+      try {
+        pool.emplace_back(work, t);
+      } catch (...) {  // no exception may cross the C ABI: run that stride here
+        mine.push_back(t);
+      }
+    }
+    for (int t : mine) work(t);
     for (std::thread &th : pool) th.join();
   }
   std::vector<long long> boff(n_inst + 1, 0);
@@ -1430,7 +1463,7 @@ int cup2d_psb_solve(cup2d_psb *h, const double *b, double *x, double tol_abs, do
   return CUP2D_PS_OK;
 }
 
-int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter) {
+static int autotune_body(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter) {
   if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
   if (!d_b) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
   if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "autotune before cup2d_psb_set_meshes");
@@ -1523,6 +1556,18 @@ int cup2d_psb_item_blocks(const cup2d_psb *h) { return h ? h->ib : 0; }
 
 const char *cup2d_psb_last_error(const cup2d_psb *h) {
   return h ? h->err.c_str() : g_berr.c_str();
+}
+
+int cup2d_psb_create(const cup2d_ps_config *cfg, cup2d_psb **out) {
+  return bguarded(nullptr, [&] { return create_body(cfg, out); });
+}
+
+int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
+  return bguarded(h, [&] { return set_meshes_body(h, n_inst, meshes); });
+}
+
+int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter) {
+  return bguarded(h, [&] { return autotune_body(h, d_b, item_blocks, ms_per_iter); });
 }
 
 }  // extern "C"

@@ -827,6 +827,20 @@ int fail(cup2d_ps *h, int code, const std::string &msg) {
   return code;
 }
 
+// Runs an entry point's body so that no C++ exception crosses the C ABI: a
+// failed host allocation becomes CUP2D_PS_ENOMEM, any other exception EINVAL,
+// with its text as the last error.
+template <class F>
+auto guarded(cup2d_ps *h, F &&body) -> decltype(body()) {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return fail(h, CUP2D_PS_ENOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(h, CUP2D_PS_EINVAL, e.what());
+  }
+}
+
 #define HIPCHK(h, expr)                                                                      \
   do {                                                                                       \
     hipError_t _e = (expr);                                                                  \
@@ -1128,8 +1142,10 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
                double tol_rel, int max_restarts, cup2d_ps_stats *st) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "solve before cup2d_ps_set_mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  return h->mixed ? solve_mixed(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st)
-                  : solve_fp64(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st);
+  return guarded(h, [&] {
+    return h->mixed ? solve_mixed(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st)
+                    : solve_fp64(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st);
+  });
 }
 
 }  // namespace
@@ -1150,7 +1166,7 @@ void cup2d_ps_config_default(cup2d_ps_config *cfg) {
   cfg->inner_tol = 0.0;
 }
 
-int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
+static int create_body(const cup2d_ps_config *cfg, cup2d_ps **out) {
   if (!out) return fail(nullptr, CUP2D_PS_EINVAL, "out is NULL");
   *out = nullptr;
   cup2d_ps_config c;
@@ -1176,8 +1192,10 @@ int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
   if (hipSetDevice(c.device) != hipSuccess) return fail(nullptr, CUP2D_PS_EHIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-      hipHostMalloc((void **)&h->h_s, 2 * sizeof(Scal), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc((void **)&h->h_s, 2 * sizeof(Scal), hipHostMallocDefault) != hipSuccess) {
+    cup2d_ps_destroy(h.release());  // releases whichever of them were made
     return fail(nullptr, CUP2D_PS_EHIP, "stream/event/pinned allocation failed");
+  }
   *out = h.release();
   return CUP2D_PS_OK;
 }
@@ -1195,7 +1213,7 @@ int cup2d_ps_destroy(cup2d_ps *h) {
   return CUP2D_PS_OK;
 }
 
-int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
+static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   if (!h || !m) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1329,7 +1347,7 @@ int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, in
   return CUP2D_PS_OK;
 }
 
-int cup2d_ps_kernel_time(cup2d_ps *h, int which, int reps, double *ms_per_launch) {
+static int kernel_time_body(cup2d_ps *h, int which, int reps, double *ms_per_launch) {
   if (!h || !ms_per_launch || reps <= 0 || which < 0 || which > 4)
     return fail(h, CUP2D_PS_EINVAL, "bad argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
@@ -1432,7 +1450,7 @@ int cup2d_ps_precond_bf16_device(cup2d_ps *h, const float *d_r, float *d_z) {
   return CUP2D_PS_OK;
 }
 
-int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]) {
+static int residual_body(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]) {
   if (!h || !d_b || !d_x || !out) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
@@ -1479,8 +1497,8 @@ int cup2d_ps_get_csr(const cup2d_ps *h, long long *row_ptr, int *col, double *va
   return CUP2D_PS_OK;
 }
 
-long long cup2d_op_csr(const cup2d_mesh_desc *m, int mean_constraint, long long *row_ptr, int *col,
-                       double *val, long long cap, long long *mean_row) {
+static long long op_csr_body(const cup2d_mesh_desc *m, int mean_constraint, long long *row_ptr,
+                             int *col, double *val, long long cap, long long *mean_row) {
   if (!m) {
     g_create_err = "NULL mesh";
     return -1;
@@ -1529,6 +1547,32 @@ int cup2d_ps_d2h(cup2d_ps *h, void *dst, const void *src, size_t bytes) {
 
 const char *cup2d_ps_last_error(const cup2d_ps *h) {
   return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
+  return guarded(nullptr, [&] { return create_body(cfg, out); });
+}
+
+int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
+  return guarded(h, [&] { return set_mesh_body(h, m); });
+}
+
+int cup2d_ps_kernel_time(cup2d_ps *h, int which, int reps, double *ms_per_launch) {
+  return guarded(h, [&] { return kernel_time_body(h, which, reps, ms_per_launch); });
+}
+
+int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]) {
+  return guarded(h, [&] { return residual_body(h, d_b, d_x, out); });
+}
+
+long long cup2d_op_csr(const cup2d_mesh_desc *m, int mean_constraint, long long *row_ptr, int *col,
+                       double *val, long long cap, long long *mean_row) {
+  try {  // errors are -1 here: a positive value is the entry count
+    return op_csr_body(m, mean_constraint, row_ptr, col, val, cap, mean_row);
+  } catch (const std::exception &e) {
+    fail(nullptr, CUP2D_PS_EINVAL, e.what());
+    return -1;
+  }
 }
 
 }  // extern "C"

@@ -31,7 +31,7 @@ typedef enum {
   CUP2D_PS_EINVAL = 1,   /* invalid argument / mesh */
   CUP2D_PS_ENOMESH = 2,  /* solve before cup2d_ps_set_mesh */
   CUP2D_PS_EHIP = 3,     /* HIP runtime error (message in cup2d_ps_last_error) */
-  CUP2D_PS_ENOMEM = 4,   /* device allocation failed */
+  CUP2D_PS_ENOMEM = 4,   /* device or host allocation failed */
   CUP2D_PS_ENODEV = 5    /* no usable gfx950 device */
 } cup2d_ps_status;
 
