@@ -172,7 +172,30 @@ def bench_amr6(PS, W, steps, use_graph):
     out["mixed_speedup_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     out["mixed_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
                                         np.linalg.norm(xs["fp64"]))
+    out["assembly"] = bench_assembly(PS, mesh)
     return out
+
+
+def bench_assembly(PS, mesh, reps=5):
+    """A mesh change on the host (assembly + upload) against the device-side
+    assembly, each the median of `reps` after one warm-up, and whether the two
+    operators give bit-identical y = A x."""
+    s = PS.Solver(device=0)
+    res = {}
+    for key, dev in (("host_set_mesh_ms", False), ("device_set_mesh_ms", True)):
+        s.set_mesh(mesh, on_device=dev)
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            s.set_mesh(mesh, on_device=dev)
+            t.append(time.perf_counter() - t0)
+        res[key] = float(np.median(t)) * 1e3
+    x = np.random.default_rng(0).normal(size=mesh.n)
+    y_dev = s.spmv(x)
+    s.set_mesh(mesh)
+    res["identical_spmv"] = bool(np.array_equal(y_dev, s.spmv(x)))
+    res["speedup"] = res["host_set_mesh_ms"] / res["device_set_mesh_ms"]
+    return res
 
 
 def bench_batch(PS, W, steps, use_graph):

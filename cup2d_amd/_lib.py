@@ -45,6 +45,7 @@ SIGNATURES = {
     "cup2d_ps_create": (_I, [_P, _P]),
     "cup2d_ps_destroy": (_I, [_P]),
     "cup2d_ps_set_mesh": (_I, [_P, _P]),
+    "cup2d_ps_set_mesh_on_device": (_I, [_P, _P]),
     "cup2d_ps_solve": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_ps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_ps_spmv_device": (_I, [_P, _P, _P, _I]),

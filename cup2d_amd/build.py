@@ -1,7 +1,7 @@
 """Builds the native libraries of the package in-tree:
 
 * libcup2d_mesh.so    -- quadtree generator (host C++, g++)
-* libcup2d_poisson.so -- the solver (HIP for gfx950 + host C++, hipcc)
+* libcup2d_poisson.so -- the solver and the device assembly (HIP for gfx950 + host C++, hipcc)
 
 Run as ``python -m cup2d_amd.build``; __graft_entry__.build() calls build().
 """
@@ -48,7 +48,7 @@ def build_poisson():
         obj = os.path.join(bdir, src + ".o")
         _run(["g++", *CXX_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
-    for src in ("poisson.hip", "batch.hip"):
+    for src in ("poisson.hip", "batch.hip", "assemble_device.hip"):
         obj = os.path.join(bdir, src + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)

@@ -1,0 +1,667 @@
+// assemble_device.hip -- the AMR Poisson operator assembled on the GPU.
+//
+// Restates the host assembly of assembly.cpp (reference
+// source/Poisson/ExpAMRSolver.cpp:100-313) as a short pipeline of kernels:
+//   1. ka_insert / ka_dups: the blocks go into an open-addressing hash table
+//      keyed by (level, i, j), the lookup that getBlockInfoAll(level, Z)
+//      gives the reference, with the host's range and duplicate checks;
+//   2. ka_faces: one thread per block classifies its four faces (same level,
+//      wall, one level coarser, one level finer; ExpAMRSolver.cpp:221-249,
+//      :141-177) and writes the neighbour table, the explicit-cell mask, h^2
+//      and its count of plain-row nonzeros;
+//   3. an exclusive scan of the explicit rows per block gives each row its
+//      place in mesh order (block, then cell);
+//   4. ka_rows: one wavefront per block, lane = cell; the lanes of explicit
+//      cells build their row (ghost interpolation ExpAMRSolver.cpp:100-128,
+//      D1/D2 ExpAMRSolver.h:268-300) with the host's sequence of additions
+//      and sort it by column;
+//   5. a stable radix sort orders the rows by length; a sort + unique of the
+//      coefficients' bit patterns gives the coefficient table;
+//   6. ka_ell writes the column-major ELL arrays, each coefficient's code (a
+//      binary search in the table) and the width of every run of 64 rows.
+// Floating-point contraction is off in this file: every coefficient is the
+// host's sequence of IEEE products, quotients and sums, so the values are
+// bit-identical.
+
+#include "assemble_device.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/cup2d_poisson.h"
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace cup2d {
+namespace {
+
+constexpr int AWG = 256;                    // threads per workgroup
+constexpr unsigned long long EMPTY = ~0ull;  // free hash slot
+constexpr int RCAP = ELL_MAX;               // entries a row can hold; one more is an error
+
+struct Blocks {
+  const int *lvl, *bx, *by;
+  long long nb;
+  int levels, bpdx, bpdy, px, py;
+};
+
+struct Hash {
+  unsigned long long *keys;  // pos_key, EMPTY when free
+  int *vals;                 // smallest block with that key
+  unsigned long long mask;   // slots - 1 (a power of two minus one)
+};
+
+// Failure reports and counters, reduced with atomics (all integers, so the
+// result does not depend on the order of the updates).
+struct Errs {
+  int lvl, idx, dup;  // smallest block with a level / index out of range, or a duplicate
+  int face;           // smallest 4 b + f whose neighbour is neither same, coarser nor finer
+  int wide;           // smallest explicit row wider than RCAP
+  int corner;         // last block with index (0, 0), -1 when none
+  unsigned long long nnz;
+};
+
+__device__ __forceinline__ unsigned long long pos_key(int l, long long i, long long j) {
+  return ((unsigned long long)l << 58) | ((unsigned long long)i << 29) | (unsigned long long)j;
+}
+
+// murmur3's 64-bit finaliser
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z ^= z >> 33;
+  z *= 0xff51afd7ed558ccdull;
+  z ^= z >> 33;
+  z *= 0xc4ceb9fe1a85ec53ull;
+  z ^= z >> 33;
+  return z;
+}
+
+__device__ long long lookup(const Hash &H, int levels, int l, long long i, long long j) {
+  if (l < 0 || l >= levels) return -1;
+  const unsigned long long key = pos_key(l, i, j);
+  unsigned long long s = mix64(key) & H.mask;
+  for (unsigned long long p = 0; p <= H.mask; p++, s = (s + 1) & H.mask) {
+    const unsigned long long k = H.keys[s];
+    if (k == key) return H.vals[s];
+    if (k == EMPTY) return -1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ long long gidx(long long b, int ix, int iy) { return b * BLEN + iy * BS + ix; }
+
+__global__ __launch_bounds__(AWG) void ka_init(Hash H, Errs *e) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * AWG;
+  for (unsigned long long s = blockIdx.x * (unsigned long long)AWG + threadIdx.x; s <= H.mask; s += stride) {
+    H.keys[s] = EMPTY;
+    H.vals[s] = INT_MAX;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    e->lvl = e->idx = e->dup = e->face = e->wide = INT_MAX;
+    e->corner = -1;
+    e->nnz = 0;
+  }
+}
+
+// Block status: 0 valid, 1 level out of range, 2 index out of range (the
+// order of build_operator's checks), 3 repeats an earlier block.
+enum : int { ST_OK = 0, ST_LEVEL = 1, ST_INDEX = 2, ST_DUP = 3 };
+
+// Failures are recorded per block with plain stores and reduced by
+// ka_status, whose atomics are straight-line code: error atomics on several
+// divergent paths of one kernel (the hash insertion's atomicMin among them)
+// were merged by the compiler into one instruction whose address register
+// was overwritten on one of the paths.
+__global__ __launch_bounds__(AWG) void ka_insert(Blocks B, Hash H, int *status) {
+  const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
+  if (b >= B.nb) return;
+  const int l = B.lvl[b], i = B.bx[b], j = B.by[b];
+  const bool lvl_ok = l >= 0 && l < B.levels;
+  const int ls = lvl_ok ? l : 0;
+  const long long nx = (long long)B.bpdx << ls, ny = (long long)B.bpdy << ls;
+  const bool idx_ok = i >= 0 && i < nx && j >= 0 && j < ny;
+  const int st = !lvl_ok ? ST_LEVEL : !idx_ok ? ST_INDEX : ST_OK;
+  status[b] = st;
+  if (st != ST_OK) return;
+  const unsigned long long key = pos_key(l, i, j);
+  unsigned long long s = mix64(key) & H.mask;
+  for (unsigned long long p = 0; p <= H.mask; p++, s = (s + 1) & H.mask) {
+    const unsigned long long prev = atomicCAS(&H.keys[s], EMPTY, key);
+    if (prev == EMPTY || prev == key) {
+      atomicMin(&H.vals[s], (int)b);  // the first occurrence owns the position
+      return;
+    }
+  }
+}
+
+// A block that does not own its position repeats an earlier block; the
+// smallest such block is the one the host's in-order insertion rejects.
+__global__ __launch_bounds__(AWG) void ka_dups(Blocks B, Hash H, int *status) {
+  const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
+  if (b >= B.nb || status[b] != ST_OK) return;
+  if (lookup(H, B.levels, B.lvl[b], B.bx[b], B.by[b]) != b) status[b] = ST_DUP;
+}
+
+__device__ __forceinline__ int wave_min_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Smallest block of each failure kind.
+__global__ __launch_bounds__(AWG) void ka_status(long long nb, const int *__restrict__ status, Errs *e) {
+  const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
+  const int st = b < nb ? status[b] : ST_OK;
+  const int lv = wave_min_int(st == ST_LEVEL ? (int)b : INT_MAX);
+  const int ix = wave_min_int(st == ST_INDEX ? (int)b : INT_MAX);
+  const int du = wave_min_int(st == ST_DUP ? (int)b : INT_MAX);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&e->lvl, lv);
+    atomicMin(&e->idx, ix);
+    atomicMin(&e->dup, du);
+  }
+}
+
+// Faces of block b: nbr (same-level id, -1 wall, -2 coarse-fine), fid (the
+// coarser block, or the two finer blocks low/high along the face), the
+// explicit cells, h^2, and the nonzeros of the plain rows.
+__global__ __launch_bounds__(AWG) void ka_faces(Blocks B, Hash H, double h0, int4 *nbr, int2 *fid,
+                                                unsigned long long *mask, int *cnt, double *h2, Errs *e) {
+  const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
+  long long plain = 0;
+  int bad_face = INT_MAX, corner = -1;
+  if (b < B.nb) {
+    const int l = B.lvl[b];
+    const long long i = B.bx[b], j = B.by[b];
+    const long long nx = (long long)B.bpdx << l, ny = (long long)B.bpdy << l;
+    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+    int v[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+      long long I = i + di[f], J = j + dj[f];
+      const bool outx = I < 0 || I >= nx, outy = J < 0 || J >= ny;
+      int2 fd = make_int2(-1, -1);
+      v[f] = -1;
+      if (!((outx && !B.px) || (outy && !B.py))) {
+        I = (I + nx) % nx;
+        J = (J + ny) % ny;
+        long long s;
+        if ((s = lookup(H, B.levels, l, I, J)) >= 0) {
+          v[f] = (int)s;
+          fd.x = (int)s;
+        } else if ((s = lookup(H, B.levels, l - 1, I >> 1, J >> 1)) >= 0) {
+          v[f] = -2;
+          fd.x = (int)s;
+        } else {
+          long long c[2];
+          for (int hh = 0; hh < 2; hh++) {
+            const long long ci = f < 2 ? 2 * I + (f == 0 ? 1 : 0) : 2 * I + hh;
+            const long long cj = f < 2 ? 2 * J + hh : 2 * J + (f == 2 ? 1 : 0);
+            c[hh] = lookup(H, B.levels, l + 1, ci, cj);
+          }
+          if (c[0] < 0 || c[1] < 0) {
+            bad_face = min(bad_face, (int)(4 * b + f));
+          } else {
+            v[f] = -2;
+            fd = make_int2((int)c[0], (int)c[1]);
+          }
+        }
+      }
+      fid[4 * b + f] = fd;
+    }
+    nbr[b] = make_int4(v[0], v[1], v[2], v[3]);
+    const unsigned long long C0 = 0x0101010101010101ull, R0 = 0xffull;
+    unsigned long long m = 0;
+    if (v[0] == -2) m |= C0;
+    if (v[1] == -2) m |= C0 << 7;
+    if (v[2] == -2) m |= R0;
+    if (v[3] == -2) m |= R0 << 56;
+    mask[b] = m;
+    cnt[b] = __popcll(m);
+    if (b == 0) cnt[B.nb] = 0;
+    // a plain row: the cell and each face that is inside the block or same-level
+    for (int c = 0; c < BLEN; c++) {
+      if ((m >> c) & 1) continue;
+      const int ix = c & 7, iy = c >> 3;
+      plain += 1 + (ix > 0 || v[0] >= 0) + (ix < BS - 1 || v[1] >= 0) + (iy > 0 || v[2] >= 0) +
+               (iy < BS - 1 || v[3] >= 0);
+    }
+    const double h = h0 / double(1LL << l);
+    h2[b] = h * h;
+    if (i == 0 && j == 0) corner = (int)b;
+  }
+  // one lane per wavefront reports, in straight-line code
+  for (int o = 32; o > 0; o >>= 1) {
+    plain += __shfl_xor(plain, o);
+    bad_face = min(bad_face, __shfl_xor(bad_face, o));
+    corner = max(corner, __shfl_xor(corner, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&e->nnz, (unsigned long long)plain);
+    atomicMin(&e->face, bad_face);
+    atomicMax(&e->corner, corner);
+  }
+}
+
+// Coefficients of one explicit row, per column in order of first appearance.
+struct RowB {
+  int c[RCAP];
+  double v[RCAP];
+  int n = 0;
+  bool over = false;
+  __device__ void add(long long col, double val) {
+    for (int k = 0; k < n; k++)
+      if (c[k] == col) {
+        v[k] += val;
+        return;
+      }
+    if (n == RCAP) {
+      over = true;
+      return;
+    }
+    c[n] = (int)col;
+    v[n] = val;
+    n++;
+  }
+};
+
+// One coarse-fine ghost: sI (2/3 close - 1/5 far + 8/15 (c + sT D1(c) + D2(c))),
+// with one-sided differences at the tangential indices 3, 7 (backward) and
+// 0, 4 (forward).  The additions are those of assembly.cpp's ghost(), in the
+// same order.
+template <class Step>
+__device__ void ghost(RowB &row, long long close, long long far_, long long c, int t, Step step,
+                      double sI, double sT) {
+  row.add(close, sI * 2. / 3.);
+  row.add(far_, -sI * 1. / 5.);
+  const double w = sI * 8. / 15.;
+  row.add(c, w);
+  const bool bwd = (t == BS - 1) || (t == BS / 2 - 1);
+  const bool fwd = (t == 0) || (t == BS / 2);
+  if (bwd) {
+    row.add(step(-2), sT * w * (1. / 8.));
+    row.add(step(-1), sT * w * (-1. / 2.));
+    row.add(c, sT * w * (3. / 8.));
+    row.add(step(-2), w * (1. / 32.));
+    row.add(step(-1), w * (-1. / 16.));
+    row.add(c, w * (1. / 32.));
+  } else if (fwd) {
+    row.add(step(2), sT * w * (-1. / 8.));
+    row.add(step(1), sT * w * (1. / 2.));
+    row.add(c, sT * w * (-3. / 8.));
+    row.add(step(2), w * (1. / 32.));
+    row.add(step(1), w * (-1. / 16.));
+    row.add(c, w * (1. / 32.));
+  } else {
+    row.add(step(-1), sT * w * (-1. / 8.));
+    row.add(step(1), sT * w * (1. / 8.));
+    row.add(c, sT * w * 0.);
+    row.add(step(-1), w * (1. / 32.));
+    row.add(step(1), w * (1. / 32.));
+    row.add(c, w * (-1. / 16.));
+  }
+}
+
+// Row of cell `lane` of block b, stored at explicit row er.
+__device__ void build_row(const Blocks &B, const int4 *__restrict__ nbr,
+                                       const int2 *__restrict__ fid, long long b, int lane, int er,
+                                       unsigned *rlen, int *rrow, int *rcol, double *rval, int &wide,
+                                       unsigned long long &entries) {
+  const int ix = lane & 7, iy = lane >> 3;
+  const long long self = gidx(b, ix, iy);
+  const int4 n4 = nbr[b];
+  const int nv[4] = {n4.x, n4.y, n4.z, n4.w};
+  const bool edge[4] = {ix == 0, ix == BS - 1, iy == 0, iy == BS - 1};
+  RowB row;
+  for (int f = 0; f < 4; f++) {
+    if (!edge[f]) {
+      const long long inner = f == 0 ? gidx(b, ix - 1, iy) : f == 1 ? gidx(b, ix + 1, iy)
+                            : f == 2 ? gidx(b, ix, iy - 1) : gidx(b, ix, iy + 1);
+      row.add(inner, 1);
+      row.add(self, -1);
+      continue;
+    }
+    if (nv[f] == -1) continue;  // wall
+    const bool xf = f < 2;
+    if (nv[f] >= 0) {
+      const long long s = nv[f];
+      const long long c = f == 0 ? gidx(s, BS - 1, iy) : f == 1 ? gidx(s, 0, iy)
+                        : f == 2 ? gidx(s, ix, BS - 1) : gidx(s, ix, 0);
+      row.add(c, 1);
+      row.add(self, -1);
+      continue;
+    }
+    const int2 fd = fid[4 * b + f];
+    if (fd.y < 0) {  // coarser neighbour
+      int ixc, iyc;
+      long long inward;
+      if (xf) {
+        ixc = f == 0 ? BS - 1 : 0;
+        iyc = iy / 2 + ((B.by[b] & 1) ? BS / 2 : 0);
+        inward = gidx(b, f == 0 ? ix + 1 : ix - 1, iy);
+      } else {
+        iyc = f == 2 ? BS - 1 : 0;
+        ixc = ix / 2 + ((B.bx[b] & 1) ? BS / 2 : 0);
+        inward = gidx(b, ix, f == 2 ? iy + 1 : iy - 1);
+      }
+      const double sT = ((xf ? iy : ix) % 2 == 0) ? -1. : 1.;
+      const long long cb = fd.x;
+      auto step = [&](int d) { return xf ? gidx(cb, ixc, iyc + d) : gidx(cb, ixc + d, iyc); };
+      ghost(row, self, inward, gidx(cb, ixc, iyc), xf ? iyc : ixc, step, 1., sT);
+      row.add(self, -1);
+    } else {  // two finer neighbours
+      const long long fbk = ((xf ? iy : ix) >= BS / 2) ? fd.y : fd.x;
+      const int t0 = ((xf ? iy : ix) % (BS / 2)) * 2;
+      auto step = [&](int d) { return xf ? gidx(b, ix, iy + d) : gidx(b, ix + d, iy); };
+      for (int hh = 0; hh < 2; hh++) {
+        const int t = t0 + hh;
+        long long close, far_;
+        switch (f) {
+          case 0: close = gidx(fbk, BS - 1, t); far_ = gidx(fbk, BS - 2, t); break;
+          case 1: close = gidx(fbk, 0, t); far_ = gidx(fbk, 1, t); break;
+          case 2: close = gidx(fbk, t, BS - 1); far_ = gidx(fbk, t, BS - 2); break;
+          default: close = gidx(fbk, t, 0); far_ = gidx(fbk, t, 1); break;
+        }
+        row.add(close, 1);
+        ghost(row, close, far_, self, xf ? iy : ix, step, -1., hh == 0 ? -1. : 1.);
+      }
+    }
+  }
+  rrow[er] = (int)self;
+  if (row.over) {
+    wide = er;
+    rlen[er] = RCAP + 1;
+    return;
+  }
+  // ascending columns (columns of a row are distinct)
+  for (int k = 1; k < row.n; k++)
+    for (int q = k; q > 0 && row.c[q - 1] > row.c[q]; q--) {
+      const int tc = row.c[q];
+      row.c[q] = row.c[q - 1];
+      row.c[q - 1] = tc;
+      const double tv = row.v[q];
+      row.v[q] = row.v[q - 1];
+      row.v[q - 1] = tv;
+    }
+  rlen[er] = (unsigned)row.n;
+  for (int k = 0; k < row.n; k++) {
+    rcol[(long long)er * RCAP + k] = row.c[k];
+    rval[(long long)er * RCAP + k] = row.v[k];
+  }
+  entries = (unsigned long long)row.n;
+}
+
+__global__ __launch_bounds__(AWG) void ka_rows(Blocks B, const int4 *__restrict__ nbr,
+                                               const int2 *__restrict__ fid,
+                                               const unsigned long long *__restrict__ mask,
+                                               const int *__restrict__ bptr, unsigned *rlen, int *rrow,
+                                               int *rcol, double *rval, Errs *e) {
+  const long long b = (blockIdx.x * (long long)AWG + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (b >= B.nb) return;  // uniform over the wavefront
+  const unsigned long long m = mask[b];
+  int wide = INT_MAX;
+  unsigned long long entries = 0;
+  if ((m >> lane) & 1) {
+    build_row(B, nbr, fid, b, lane, bptr[b] + __popcll(m & ((1ull << lane) - 1)), rlen, rrow, rcol,
+              rval, wide, entries);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    entries += __shfl_xor(entries, o);
+    wide = min(wide, __shfl_xor(wide, o));
+  }
+  if (lane == 0) {
+    atomicAdd(&e->nnz, entries);
+    atomicMin(&e->wide, wide);
+  }
+}
+
+__global__ __launch_bounds__(AWG) void ka_iota(int n, int *a) {
+  const int i = blockIdx.x * AWG + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+// Bit patterns of every row slot (0.0 for the unused ones) plus one 0.0, so
+// that code 0 is 0.0.
+__global__ __launch_bounds__(AWG) void ka_keys(long long n_exc, const unsigned *__restrict__ rlen,
+                                               const double *__restrict__ rval,
+                                               unsigned long long *keys) {
+  const long long q = blockIdx.x * (long long)AWG + threadIdx.x;
+  if (q > n_exc * RCAP) return;
+  const long long er = q / RCAP;
+  const int w = (int)(q % RCAP);
+  keys[q] = (er < n_exc && w < (int)rlen[er]) ? (unsigned long long)__double_as_longlong(rval[q]) : 0ull;
+}
+
+__global__ __launch_bounds__(AWG) void ka_ell(int n_exc, int W, const int *__restrict__ order,
+                                              const unsigned *__restrict__ rlen,
+                                              const int *__restrict__ rrow, const int *__restrict__ rcol,
+                                              const double *__restrict__ rval,
+                                              const unsigned long long *__restrict__ table, int n_codes,
+                                              int *erow, int *ecol, uint16_t *ecode, uint8_t *ewid) {
+  const int s = blockIdx.x * AWG + threadIdx.x;
+  if (s >= n_exc) return;
+  const int r = order[s];
+  const int len = (int)rlen[r], row = rrow[r];
+  erow[s] = row;
+  for (int w = 0; w < W; w++) {
+    int col = row, code = 0;
+    if (w < len) {
+      col = rcol[(long long)r * RCAP + w];
+      const unsigned long long key = (unsigned long long)__double_as_longlong(rval[(long long)r * RCAP + w]);
+      int lo = 0, hi = n_codes - 1;  // the key is in the table
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (table[mid] < key) lo = mid + 1;
+        else hi = mid;
+      }
+      code = lo;
+    }
+    ecol[(long long)w * n_exc + s] = col;
+    ecode[(long long)w * n_exc + s] = (uint16_t)code;
+  }
+  // rows are in ascending length: the last row of a run is its widest
+  if ((s & 63) == 63 || s == n_exc - 1) ewid[s >> 6] = (uint8_t)len;
+}
+
+// Device temporaries of one assembly, freed on every return.
+struct Temps {
+  std::vector<void *> p;
+  ~Temps() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  hipError_t get(T **out, size_t count) {
+    *out = nullptr;
+    hipError_t e = hipMalloc((void **)out, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) p.push_back(*out);
+    return e;
+  }
+};
+
+}  // namespace
+
+#define ACHK(expr)                                                                      \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      err = std::string(#expr) + ": " + hipGetErrorString(_e);                          \
+      return _e == hipErrorOutOfMemory ? CUP2D_PS_ENOMEM : CUP2D_PS_EHIP;               \
+    }                                                                                   \
+  } while (0)
+
+int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
+                       const DeviceAlloc &alloc, DeviceOperator &out, std::string &err) {
+  const long long nb = m.n_blocks;
+  if (nb <= 0 || m.bpdx <= 0 || m.bpdy <= 0 || m.levels <= 0 || m.levels > 24 || m.extent <= 0 ||
+      !m.level || !m.bx || !m.by) {
+    err = "invalid mesh description";
+    return CUP2D_PS_EINVAL;
+  }
+  if (nb * BLEN >= (1LL << 31) / 8) {
+    err = "mesh too large for 32-bit column indices";
+    return CUP2D_PS_EINVAL;
+  }
+  out = DeviceOperator();
+  out.nb = nb;
+  out.n = nb * BLEN;
+  auto give = [&](auto **p, size_t count) {
+    *p = static_cast<std::remove_pointer_t<decltype(p)>>(
+        alloc(std::max<size_t>(count, 1) * sizeof(**p)));
+    return *p != nullptr;
+  };
+  Temps T;
+
+  // ---- blocks into the hash table ---------------------------------------------
+  int *d_lvl, *d_bx, *d_by;
+  ACHK(T.get(&d_lvl, nb));
+  ACHK(T.get(&d_bx, nb));
+  ACHK(T.get(&d_by, nb));
+  ACHK(hipMemcpyAsync(d_lvl, m.level, nb * sizeof(int), hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(d_bx, m.bx, nb * sizeof(int), hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(d_by, m.by, nb * sizeof(int), hipMemcpyHostToDevice, st));
+  unsigned long long slots = 64;
+  while (slots < 2ull * (unsigned long long)nb) slots <<= 1;
+  Hash H;
+  H.mask = slots - 1;
+  ACHK(T.get(&H.keys, slots));
+  ACHK(T.get(&H.vals, slots));
+  Errs *d_e;
+  int *d_status;
+  ACHK(T.get(&d_e, 1));
+  ACHK(T.get(&d_status, nb));
+  const Blocks B{d_lvl, d_bx, d_by, nb, m.levels, m.bpdx, m.bpdy, m.periodic_x, m.periodic_y};
+  const int gb = (int)((nb + AWG - 1) / AWG);
+  ka_init<<<(int)std::min<unsigned long long>(1024, slots / AWG + 1), AWG, 0, st>>>(H, d_e);
+  ka_insert<<<gb, AWG, 0, st>>>(B, H, d_status);
+  ka_dups<<<gb, AWG, 0, st>>>(B, H, d_status);
+  ka_status<<<gb, AWG, 0, st>>>(nb, d_status, d_e);
+  ACHK(hipGetLastError());
+  Errs e;
+  ACHK(hipMemcpyAsync(&e, d_e, sizeof e, hipMemcpyDeviceToHost, st));
+  ACHK(hipStreamSynchronize(st));
+  const int eb = std::min({e.lvl, e.idx, e.dup});
+  if (eb != INT_MAX) {
+    err = "block " + std::to_string(eb) +
+          (eb == e.lvl ? ": level out of range" : eb == e.idx ? ": index out of range" : ": duplicate");
+    return CUP2D_PS_EINVAL;
+  }
+
+  // ---- faces, explicit cells, h^2 -----------------------------------------------
+  if (!give(&out.nbr, nb) || !give(&out.mask, nb) || !give(&out.h2, nb)) {
+    err = "device allocation failed";
+    return CUP2D_PS_ENOMEM;
+  }
+  int2 *d_fid;
+  int *d_cnt, *d_bptr;
+  ACHK(T.get(&d_fid, 4 * nb));
+  ACHK(T.get(&d_cnt, nb + 1));
+  ACHK(T.get(&d_bptr, nb + 1));
+  const double h0 = m.extent / (double(std::max(m.bpdx, m.bpdy)) * BS);
+  ka_faces<<<gb, AWG, 0, st>>>(B, H, h0, out.nbr, d_fid, out.mask, d_cnt, out.h2, d_e);
+  ACHK(hipGetLastError());
+  size_t tb = 0;
+  char *d_tmp = nullptr;
+  ACHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_cnt, d_bptr, (int)(nb + 1), st));
+  ACHK(T.get(&d_tmp, tb));
+  ACHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tb, d_cnt, d_bptr, (int)(nb + 1), st));
+  int n_exc = 0;
+  ACHK(hipMemcpyAsync(&e, d_e, sizeof e, hipMemcpyDeviceToHost, st));
+  ACHK(hipMemcpyAsync(&n_exc, d_bptr + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+  ACHK(hipStreamSynchronize(st));
+  if (e.face != INT_MAX) {
+    err = "block " + std::to_string(e.face / 4) + " face " + std::to_string(e.face % 4) +
+          ": neighbour is neither same-level, coarser nor finer (mesh not 2:1 balanced)";
+    return CUP2D_PS_EINVAL;
+  }
+  out.corner_block = e.corner;
+  out.mean_row = (mean_constraint && e.corner >= 0) ? (long long)e.corner * BLEN : -1;
+  out.n_exc = n_exc;
+  out.nnz = (long long)e.nnz;
+  if (n_exc == 0) {
+    const double zero = 0.0;
+    out.n_codes = 1;
+    if (!give(&out.erow, 1) || !give(&out.ecol, 1) || !give(&out.ecode, 1) || !give(&out.etab, 1) ||
+        !give(&out.ewid, 1)) {
+      err = "device allocation failed";
+      return CUP2D_PS_ENOMEM;
+    }
+    ACHK(hipMemcpyAsync(out.etab, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+    ACHK(hipStreamSynchronize(st));
+    return CUP2D_PS_OK;
+  }
+
+  // ---- explicit rows -------------------------------------------------------------
+  unsigned *d_rlen, *d_rlen_s;
+  int *d_rrow, *d_rcol, *d_iota, *d_order;
+  double *d_rval;
+  ACHK(T.get(&d_rlen, n_exc));
+  ACHK(T.get(&d_rlen_s, n_exc));
+  ACHK(T.get(&d_rrow, n_exc));
+  ACHK(T.get(&d_rcol, (size_t)n_exc * RCAP));
+  ACHK(T.get(&d_rval, (size_t)n_exc * RCAP));
+  ACHK(T.get(&d_iota, n_exc));
+  ACHK(T.get(&d_order, n_exc));
+  ka_rows<<<(int)((nb * 64 + AWG - 1) / AWG), AWG, 0, st>>>(B, out.nbr, d_fid, out.mask, d_bptr, d_rlen,
+                                                          d_rrow, d_rcol, d_rval, d_e);
+  ka_iota<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(n_exc, d_iota);
+  ACHK(hipGetLastError());
+  // stable: rows of one length keep their mesh order (lengths < 32: 5 bits)
+  tb = 0;
+  ACHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_rlen, d_rlen_s, d_iota, d_order, n_exc, 0, 5, st));
+  ACHK(T.get(&d_tmp, tb));
+  ACHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, tb, d_rlen, d_rlen_s, d_iota, d_order, n_exc, 0, 5, st));
+  // distinct coefficients by bit pattern, ascending
+  const long long nk = (long long)n_exc * RCAP + 1;
+  unsigned long long *d_keys, *d_skeys, *d_table;
+  int *d_ncodes;
+  ACHK(T.get(&d_keys, nk));
+  ACHK(T.get(&d_skeys, nk));
+  ACHK(T.get(&d_table, nk));
+  ACHK(T.get(&d_ncodes, 1));
+  ka_keys<<<(int)((nk + AWG - 1) / AWG), AWG, 0, st>>>(n_exc, d_rlen, d_rval, d_keys);
+  ACHK(hipGetLastError());
+  tb = 0;
+  ACHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_keys, d_skeys, (int)nk, 0, 64, st));
+  ACHK(T.get(&d_tmp, tb));
+  ACHK(hipcub::DeviceRadixSort::SortKeys(d_tmp, tb, d_keys, d_skeys, (int)nk, 0, 64, st));
+  tb = 0;
+  ACHK(hipcub::DeviceSelect::Unique(nullptr, tb, d_skeys, d_table, d_ncodes, (int)nk, st));
+  ACHK(T.get(&d_tmp, tb));
+  ACHK(hipcub::DeviceSelect::Unique(d_tmp, tb, d_skeys, d_table, d_ncodes, (int)nk, st));
+  unsigned W = 0;
+  int n_codes = 0;
+  ACHK(hipMemcpyAsync(&e, d_e, sizeof e, hipMemcpyDeviceToHost, st));
+  ACHK(hipMemcpyAsync(&W, d_rlen_s + n_exc - 1, sizeof W, hipMemcpyDeviceToHost, st));
+  ACHK(hipMemcpyAsync(&n_codes, d_ncodes, sizeof n_codes, hipMemcpyDeviceToHost, st));
+  ACHK(hipStreamSynchronize(st));
+  if (e.wide != INT_MAX || W > (unsigned)ELL_MAX) {
+    err = "explicit row with more than " + std::to_string(ELL_MAX) + " entries";
+    return CUP2D_PS_EINVAL;
+  }
+  if (n_codes > 0x10000) {
+    err = "more than 65536 distinct coefficients in the explicit rows";
+    return CUP2D_PS_EINVAL;
+  }
+  out.nnz = (long long)e.nnz;
+  out.ell_w = (int)W;
+  out.n_codes = n_codes;
+  if (!give(&out.erow, n_exc) || !give(&out.ecol, (size_t)W * n_exc) ||
+      !give(&out.ecode, (size_t)W * n_exc) || !give(&out.etab, n_codes) ||
+      !give(&out.ewid, (n_exc + 63) / 64)) {
+    err = "device allocation failed";
+    return CUP2D_PS_ENOMEM;
+  }
+  ACHK(hipMemcpyAsync(out.etab, d_table, n_codes * sizeof(double), hipMemcpyDeviceToDevice, st));
+  ka_ell<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(n_exc, (int)W, d_order, d_rlen, d_rrow, d_rcol, d_rval,
+                                                  d_table, n_codes, out.erow, out.ecol, out.ecode,
+                                                  out.ewid);
+  ACHK(hipGetLastError());
+  ACHK(hipStreamSynchronize(st));
+  return CUP2D_PS_OK;
+}
+
+}  // namespace cup2d

@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../../include/cup2d_poisson.h"
+#include "assemble_device.h"
 #include "assembly.h"
 #include "kernels.h"
 
@@ -794,6 +795,8 @@ struct cup2d_ps {
   uint8_t *d_ewid = nullptr;
   double *d_etab = nullptr, *d_h2 = nullptr;
   int ell_w = 0;
+  long long n_exc = 0;                   // explicit rows
+  bool has_csr = false;                  // plain CSR copy present (host-assembled meshes)
   int *d_csr_rp = nullptr, *d_csr_col = nullptr;
   double *d_csr_val = nullptr;
   double *d_P = nullptr;
@@ -882,7 +885,7 @@ OpView view(const cup2d_ps *h) {
   A.mean_row = (int)h->op.mean_row;
   A.nbr = h->d_nbr;
   A.mask = h->d_mask;
-  A.n_exc = (int)h->op.n_exc();
+  A.n_exc = (int)h->n_exc;
   A.ell_w = h->ell_w;
   A.erow = h->d_erow;
   A.ecol = h->d_ecol;
@@ -1213,6 +1216,8 @@ int cup2d_ps_destroy(cup2d_ps *h) {
   return CUP2D_PS_OK;
 }
 
+static int finish_mesh(cup2d_ps *h);
+
 static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   if (!h || !m) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   HIPCHK(h, hipSetDevice(h->device));
@@ -1239,18 +1244,27 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   std::vector<uint8_t> ewid((n_exc + 63) / 64);
   for (size_t r = 0; r < ewid.size(); r++)
     ewid[r] = (uint8_t)ell.width((long long)(64 * r), (long long)std::min(n_exc, 64 * r + 64));
-  const std::vector<double> P = make_pinv();
-  const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = upload(h, &h->d_nbr, nbr)) || (rc = upload(h, &h->d_mask, mask)) ||
       (rc = upload(h, &h->d_erow, ell.erow)) ||
       (rc = upload(h, &h->d_ecol, ell_transpose(ell.ecol_rm, h->ell_w))) ||
       (rc = upload(h, &h->d_ecode, ell_transpose(ell.ecode_rm, h->ell_w))) ||
       (rc = upload(h, &h->d_etab, ell.values)) || (rc = upload(h, &h->d_ewid, ewid)) ||
       (rc = upload(h, &h->d_h2, op.h2)) || (rc = upload(h, &h->d_csr_rp, op.rp)) ||
-      (rc = upload(h, &h->d_csr_col, op.col)) || (rc = upload(h, &h->d_csr_val, op.val)) ||
-      (rc = upload(h, &h->d_P, P)))
+      (rc = upload(h, &h->d_csr_col, op.col)) || (rc = upload(h, &h->d_csr_val, op.val)))
     return rc;
-  if ((rc = dalloc(h, &h->d_pf, 8 * 64))) return rc;
+  h->n_exc = (long long)n_exc;
+  h->has_csr = true;
+  return finish_mesh(h);
+}
+
+// The block preconditioner, the work vectors and the launch shapes: the part
+// of a mesh change that does not depend on where the operator was assembled.
+static int finish_mesh(cup2d_ps *h) {
+  const Operator &op = h->op;
+  int rc = 0;
+  const std::vector<double> P = make_pinv();
+  const std::vector<uint16_t> pf = make_pfrag(P);
+  if ((rc = upload(h, &h->d_P, P)) || (rc = dalloc(h, &h->d_pf, 8 * 64))) return rc;
   HIPCHK(h, hipMemcpy(h->d_pf, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   const size_t n = op.n;
   double **vecs[] = {&h->d_x, &h->d_xopt, &h->d_r, &h->d_rhat, &h->d_p, &h->d_nu,
@@ -1268,7 +1282,7 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   HIPCHK(h, hipMemset(h->d_s, 0, 2 * sizeof(Scal)));
   const long long waves = (op.nb + SPMV_U - 1) / SPMV_U;
   h->Gb = (int)std::max<long long>(1, std::min<long long>(MAXG, (waves + WAVES - 1) / WAVES));
-  const int Ge = op.n_exc() > 0 ? (int)std::min<long long>(MAXGE, (op.n_exc() + WG - 1) / WG) : 0;
+  const int Ge = h->n_exc > 0 ? (int)std::min<long long>(MAXGE, (h->n_exc + WG - 1) / WG) : 0;
   h->G = h->Gb + Ge;
   h->nbp = h->cfg.reserved_i[0] > 0 ? h->cfg.reserved_i[0] : (op.nb >= 8192 ? 4 : 2);
   if (h->nbp != 1 && h->nbp != 2 && h->nbp != 4) h->nbp = 2;
@@ -1278,6 +1292,48 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   h->Gm = (int)std::max<long long>(1, std::min<long long>(MAXG, (tiles + WAVES - 1) / WAVES));
   h->have_mesh = true;
   return CUP2D_PS_OK;
+}
+
+// The operator assembled by kernels (assemble_device.hip) straight into the
+// arrays the solver reads; no host assembly, no plain CSR copy.
+static int set_mesh_on_device_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
+  if (!h || !m) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  free_mesh(h);
+  const MeshDesc md{m->bpdx, m->bpdy, m->levels, m->periodic_x, m->periodic_y, m->extent,
+                    m->n_blocks, m->level, m->bx, m->by};
+  DeviceOperator D;
+  std::string err;
+  const DeviceAlloc alloc = [h](size_t bytes) -> void * {
+    uint8_t *p = nullptr;
+    return dalloc(h, &p, bytes) ? nullptr : p;
+  };
+  const int rc = assemble_on_device(md, h->cfg.mean_constraint, h->stream, alloc, D, err);
+  if (rc) {
+    free_mesh(h);
+    return fail(h, rc, err);
+  }
+  h->op = Operator();
+  h->op.nb = D.nb;
+  h->op.n = D.n;
+  h->op.nnz_rows = D.nnz;
+  h->op.mean_row = D.mean_row;
+  h->op.corner_block = D.corner_block;
+  h->d_nbr = D.nbr;
+  h->d_mask = D.mask;
+  h->d_h2 = D.h2;
+  h->d_erow = D.erow;
+  h->d_ecol = D.ecol;
+  h->d_ecode = D.ecode;
+  h->d_etab = D.etab;
+  h->d_ewid = D.ewid;
+  h->ell_w = D.ell_w;
+  h->n_exc = D.n_exc;
+  h->has_csr = false;
+  h->d_csr_rp = h->d_csr_col = nullptr;
+  h->d_csr_val = nullptr;
+  return finish_mesh(h);
 }
 
 int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs,
@@ -1307,6 +1363,7 @@ int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind) 
   if (!h || !d_x || !d_y) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   if (kind != 0 && kind != 1) return fail(h, CUP2D_PS_EINVAL, "kind must be 0 or 1");
+  if (kind == 1 && !h->has_csr) return fail(h, CUP2D_PS_EINVAL, "no CSR copy: the mesh was assembled on the device");
   HIPCHK(h, hipSetDevice(h->device));
   const long n = (long)h->op.n;
   if (kind == 0) {
@@ -1329,6 +1386,7 @@ int cup2d_ps_spmv_time(cup2d_ps *h, const double *d_x, double *d_y, int kind, in
   if (!h || !d_x || !d_y || !ms_per_launch || reps <= 0) return fail(h, CUP2D_PS_EINVAL, "bad argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   if (kind != 0 && kind != 1) return fail(h, CUP2D_PS_EINVAL, "kind must be 0 or 1");
+  if (kind == 1 && !h->has_csr) return fail(h, CUP2D_PS_EINVAL, "no CSR copy: the mesh was assembled on the device");
   HIPCHK(h, hipSetDevice(h->device));
   const long n = (long)h->op.n;
   const int Gc = (int)std::min<long>(4096, (n + WG - 1) / WG);
@@ -1483,7 +1541,7 @@ int cup2d_ps_sizes(const cup2d_ps *h, long long *n, long long *nnz, long long *n
   if (!h->have_mesh) return CUP2D_PS_ENOMESH;
   if (n) *n = h->op.n;
   if (nnz) *nnz = h->op.nnz();
-  if (n_explicit) *n_explicit = h->op.n_exc();
+  if (n_explicit) *n_explicit = h->n_exc;
   if (mean_row) *mean_row = h->op.mean_row;
   return CUP2D_PS_OK;
 }
@@ -1491,6 +1549,7 @@ int cup2d_ps_sizes(const cup2d_ps *h, long long *n, long long *nnz, long long *n
 int cup2d_ps_get_csr(const cup2d_ps *h, long long *row_ptr, int *col, double *val) {
   if (!h || !row_ptr || !col || !val) return CUP2D_PS_EINVAL;
   if (!h->have_mesh) return CUP2D_PS_ENOMESH;
+  if (!h->has_csr) return CUP2D_PS_EINVAL;
   for (size_t i = 0; i < h->op.rp.size(); i++) row_ptr[i] = h->op.rp[i];
   std::memcpy(col, h->op.col.data(), h->op.col.size() * sizeof(int));
   std::memcpy(val, h->op.val.data(), h->op.val.size() * sizeof(double));
@@ -1555,6 +1614,10 @@ int cup2d_ps_create(const cup2d_ps_config *cfg, cup2d_ps **out) {
 
 int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
   return guarded(h, [&] { return set_mesh_body(h, m); });
+}
+
+int cup2d_ps_set_mesh_on_device(cup2d_ps *h, const cup2d_mesh_desc *m) {
+  return guarded(h, [&] { return set_mesh_on_device_body(h, m); });
 }
 
 int cup2d_ps_kernel_time(cup2d_ps *h, int which, int reps, double *ms_per_launch) {

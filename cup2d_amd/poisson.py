@@ -143,9 +143,14 @@ class Solver:
             pass
 
     # -- mesh ------------------------------------------------------------
-    def set_mesh(self, mesh: Mesh):
+    def set_mesh(self, mesh: Mesh, on_device: bool = False):
+        """Assembles the operator of `mesh`: on the host (default, keeps a
+        plain CSR copy) or by GPU kernels (on_device=True, same operator bit
+        for bit, no CSR copy)."""
         d, self._keep = _mesh_desc(mesh)
-        _check(self.h, _lib.load().cup2d_ps_set_mesh(self.h, ctypes.byref(d)))
+        lib = _lib.load()
+        fn = lib.cup2d_ps_set_mesh_on_device if on_device else lib.cup2d_ps_set_mesh
+        _check(self.h, fn(self.h, ctypes.byref(d)))
         self.mesh = mesh
         return self
 

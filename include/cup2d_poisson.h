@@ -99,6 +99,15 @@ int cup2d_ps_destroy(cup2d_ps *h);
  * ExpAMRSolver.cpp:180-313, LocalSpMatDnVec.cpp:81-189, BiCGSTAB.cu:142-244). */
 int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *mesh);
 
+/* Same operator, assembled by GPU kernels from the block description (the
+ * same getMat / makeFlux / interpolate restatement, ExpAMRSolver.cpp:100-313,
+ * run per block and per coarse-fine row on the device; only the 12 bytes per
+ * block of `mesh` cross PCIe).  The result equals cup2d_ps_set_mesh's bit for
+ * bit; the same meshes are rejected with the same messages.  No plain CSR
+ * copy is kept: spmv/spmv_time kind 1 and cup2d_ps_get_csr return
+ * CUP2D_PS_EINVAL on such a mesh. */
+int cup2d_ps_set_mesh_on_device(cup2d_ps *h, const cup2d_mesh_desc *mesh);
+
 /* Solve A x = b (reference: ExpAMRSolver::solve, ExpAMRSolver.cpp:354-416).
  * b: right-hand side (not modified; with mean_constraint its mean row is
  * taken as 0, ExpAMRSolver.cpp:340-345); x: initial guess on entry, solution
