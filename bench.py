@@ -260,6 +260,11 @@ def bench_batch(PS, W, steps, use_graph):
 def kernel_only(PS, W, which, reps):
     """Launches one kernel `reps` times (for rocprofv3 counter runs) and
     prints its mean time and algorithmic bytes."""
+    if which == "assembly":
+        # mesh changes on the ~16k-block 6-level mesh: host assembly + upload
+        # against the device-side assembly
+        print(json.dumps({"kernel": "assembly", **bench_assembly(PS, W.four_disk_amr6_mesh(), reps)}))
+        return
     if which == "copy":
         # the library's streaming copy over the batch's 13,881,344 unknowns
         # (8-byte loads, the access pattern of the solver's vector kernels):
@@ -329,7 +334,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "iter0", "iter1",
+    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "iter0", "iter1",
                                               "iter2", "iter3", "iter4"], default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
                          "(spmv: operator on the bench mesh; batch_spmv: batched operator on the "
@@ -337,7 +342,8 @@ def main():
                          "batched solve, every kernel launched from the stream; iterK: iteration "
                          "kernel K of the bench solve; copy: the library's streaming copy over "
                          "13,881,344 doubles, the bandwidth ceiling and the traffic counters' "
-                         "calibration)")
+                         "calibration; assembly: host against device-side assembly of the "
+                         "6-level 16k-block mesh)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "

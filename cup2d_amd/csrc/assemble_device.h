@@ -37,16 +37,19 @@ struct DeviceOperator {
   uint8_t *ewid = nullptr;            // [(n_exc + 63) / 64] width of each run of 64 rows
 };
 
-// Device memory for the outputs: returns nullptr on failure; the caller owns
-// what it hands out.
+// Device memory: returns nullptr on failure; the caller owns what it hands
+// out.
 using DeviceAlloc = std::function<void *(size_t bytes)>;
 
 // Assembles the operator of mesh m (host arrays level/bx/by) on `stream`.
+// `alloc` provides the outputs, `temp` the work arrays (which the caller may
+// reuse once the call has returned and the stream has drained).
 // Returns 0, or a CUP2D_PS_* code with a message in err: EINVAL for the
 // meshes build_operator rejects (same messages) and for an explicit row
 // wider than ELL_MAX, ENOMEM / EHIP for device failures.  Blocks until the
 // outputs are complete.
 int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t stream,
-                       const DeviceAlloc &alloc, DeviceOperator &out, std::string &err);
+                       const DeviceAlloc &alloc, const DeviceAlloc &temp, DeviceOperator &out,
+                       std::string &err);
 
 }  // namespace cup2d

@@ -11,10 +11,11 @@
 //      and its count of plain-row nonzeros;
 //   3. an exclusive scan of the explicit rows per block gives each row its
 //      place in mesh order (block, then cell);
-//   4. ka_rows: one wavefront per block, lane = cell; the lanes of explicit
-//      cells build their row (ghost interpolation ExpAMRSolver.cpp:100-128,
-//      D1/D2 ExpAMRSolver.h:268-300) with the host's sequence of additions
-//      and sort it by column;
+//   4. ka_rows: one thread per explicit row (its block by binary search in
+//      the scan, its cell as the k-th set bit of the block's mask) builds the
+//      row in registers (ghost interpolation ExpAMRSolver.cpp:100-128, D1/D2
+//      ExpAMRSolver.h:268-300) with the host's sequence of additions and
+//      stores it sorted by column;
 //   5. a stable radix sort orders the rows by length; a sort + unique of the
 //      coefficients' bit patterns gives the coefficient table;
 //   6. ka_ell writes the column-major ELL arrays, each coefficient's code (a
@@ -247,23 +248,39 @@ __global__ __launch_bounds__(AWG) void ka_faces(Blocks B, Hash H, double h0, int
 }
 
 // Coefficients of one explicit row, per column in order of first appearance.
+// Every slot is addressed with a compile-time index (unrolled loops), so
+// the row stays in registers.
 struct RowB {
   int c[RCAP];
   double v[RCAP];
   int n = 0;
   bool over = false;
-  __device__ void add(long long col, double val) {
-    for (int k = 0; k < n; k++)
-      if (c[k] == col) {
+  __device__ __forceinline__ RowB() {
+#pragma unroll
+    for (int k = 0; k < RCAP; k++) {
+      c[k] = -1;
+      v[k] = 0.0;
+    }
+  }
+  __device__ __forceinline__ void add(long long col, double val) {
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < RCAP; k++)
+      if (k < n && c[k] == (int)col) {
         v[k] += val;
-        return;
+        found = true;
       }
+    if (found) return;
     if (n == RCAP) {
       over = true;
       return;
     }
-    c[n] = (int)col;
-    v[n] = val;
+#pragma unroll
+    for (int k = 0; k < RCAP; k++)
+      if (k == n) {
+        c[k] = (int)col;
+        v[k] = val;
+      }
     n++;
   }
 };
@@ -273,61 +290,58 @@ struct RowB {
 // 0, 4 (forward).  The additions are those of assembly.cpp's ghost(), in the
 // same order.
 template <class Step>
-__device__ void ghost(RowB &row, long long close, long long far_, long long c, int t, Step step,
+__device__ __forceinline__ void ghost(RowB &row, long long close, long long far_, long long c, int t, Step step,
                       double sI, double sT) {
   row.add(close, sI * 2. / 3.);
   row.add(far_, -sI * 1. / 5.);
   const double w = sI * 8. / 15.;
   row.add(c, w);
+  // The three cases add to the same pattern of cells (two tangential
+  // neighbours d0, d1 and c, twice), so one sequence of additions with
+  // selected offsets and weights serves all of them; each coefficient is
+  // still (sT w) * k or w * k with the host's constant k.
   const bool bwd = (t == BS - 1) || (t == BS / 2 - 1);
   const bool fwd = (t == 0) || (t == BS / 2);
-  if (bwd) {
-    row.add(step(-2), sT * w * (1. / 8.));
-    row.add(step(-1), sT * w * (-1. / 2.));
-    row.add(c, sT * w * (3. / 8.));
-    row.add(step(-2), w * (1. / 32.));
-    row.add(step(-1), w * (-1. / 16.));
-    row.add(c, w * (1. / 32.));
-  } else if (fwd) {
-    row.add(step(2), sT * w * (-1. / 8.));
-    row.add(step(1), sT * w * (1. / 2.));
-    row.add(c, sT * w * (-3. / 8.));
-    row.add(step(2), w * (1. / 32.));
-    row.add(step(1), w * (-1. / 16.));
-    row.add(c, w * (1. / 32.));
-  } else {
-    row.add(step(-1), sT * w * (-1. / 8.));
-    row.add(step(1), sT * w * (1. / 8.));
-    row.add(c, sT * w * 0.);
-    row.add(step(-1), w * (1. / 32.));
-    row.add(step(1), w * (1. / 32.));
-    row.add(c, w * (-1. / 16.));
-  }
+  const int d0 = bwd ? -2 : fwd ? 2 : -1;
+  const int d1 = bwd ? -1 : 1;
+  const double k0 = bwd ? 1. / 8. : -1. / 8.;
+  const double k1 = bwd ? -1. / 2. : fwd ? 1. / 2. : 1. / 8.;
+  const double k2 = bwd ? 3. / 8. : fwd ? -3. / 8. : 0.;
+  const double k4 = (bwd || fwd) ? -1. / 16. : 1. / 32.;
+  const double k5 = (bwd || fwd) ? 1. / 32. : -1. / 16.;
+  const double sw = sT * w;
+  row.add(step(d0), sw * k0);  // first tangential difference (D1)
+  row.add(step(d1), sw * k1);
+  row.add(c, sw * k2);
+  row.add(step(d0), w * (1. / 32.));  // second tangential difference (D2)
+  row.add(step(d1), w * k4);
+  row.add(c, w * k5);
 }
 
 // Row of cell `lane` of block b, stored at explicit row er.
-__device__ void build_row(const Blocks &B, const int4 *__restrict__ nbr,
+__device__ __forceinline__ void build_row(const Blocks &B, const int4 *__restrict__ nbr,
                                        const int2 *__restrict__ fid, long long b, int lane, int er,
                                        unsigned *rlen, int *rrow, int *rcol, double *rval, int &wide,
                                        unsigned long long &entries) {
   const int ix = lane & 7, iy = lane >> 3;
   const long long self = gidx(b, ix, iy);
   const int4 n4 = nbr[b];
-  const int nv[4] = {n4.x, n4.y, n4.z, n4.w};
-  const bool edge[4] = {ix == 0, ix == BS - 1, iy == 0, iy == BS - 1};
   RowB row;
   for (int f = 0; f < 4; f++) {
-    if (!edge[f]) {
+    // selects, not arrays indexed by f: nothing of the row leaves registers
+    const int nvf = f == 0 ? n4.x : f == 1 ? n4.y : f == 2 ? n4.z : n4.w;
+    const bool edgef = f == 0 ? ix == 0 : f == 1 ? ix == BS - 1 : f == 2 ? iy == 0 : iy == BS - 1;
+    if (!edgef) {
       const long long inner = f == 0 ? gidx(b, ix - 1, iy) : f == 1 ? gidx(b, ix + 1, iy)
                             : f == 2 ? gidx(b, ix, iy - 1) : gidx(b, ix, iy + 1);
       row.add(inner, 1);
       row.add(self, -1);
       continue;
     }
-    if (nv[f] == -1) continue;  // wall
+    if (nvf == -1) continue;  // wall
     const bool xf = f < 2;
-    if (nv[f] >= 0) {
-      const long long s = nv[f];
+    if (nvf >= 0) {
+      const long long s = nvf;
       const long long c = f == 0 ? gidx(s, BS - 1, iy) : f == 1 ? gidx(s, 0, iy)
                         : f == 2 ? gidx(s, ix, BS - 1) : gidx(s, ix, 0);
       row.add(c, 1);
@@ -356,6 +370,7 @@ __device__ void build_row(const Blocks &B, const int4 *__restrict__ nbr,
       const long long fbk = ((xf ? iy : ix) >= BS / 2) ? fd.y : fd.x;
       const int t0 = ((xf ? iy : ix) % (BS / 2)) * 2;
       auto step = [&](int d) { return xf ? gidx(b, ix, iy + d) : gidx(b, ix + d, iy); };
+#pragma unroll 1
       for (int hh = 0; hh < 2; hh++) {
         const int t = t0 + hh;
         long long close, far_;
@@ -376,20 +391,17 @@ __device__ void build_row(const Blocks &B, const int4 *__restrict__ nbr,
     rlen[er] = RCAP + 1;
     return;
   }
-  // ascending columns (columns of a row are distinct)
-  for (int k = 1; k < row.n; k++)
-    for (int q = k; q > 0 && row.c[q - 1] > row.c[q]; q--) {
-      const int tc = row.c[q];
-      row.c[q] = row.c[q - 1];
-      row.c[q - 1] = tc;
-      const double tv = row.v[q];
-      row.v[q] = row.v[q - 1];
-      row.v[q - 1] = tv;
-    }
+  // ascending columns: an entry's place is the number of smaller columns
+  // (the columns of a row are distinct)
   rlen[er] = (unsigned)row.n;
-  for (int k = 0; k < row.n; k++) {
-    rcol[(long long)er * RCAP + k] = row.c[k];
-    rval[(long long)er * RCAP + k] = row.v[k];
+#pragma unroll
+  for (int k = 0; k < RCAP; k++) {
+    if (k >= row.n) continue;
+    int rank = 0;
+#pragma unroll
+    for (int q = 0; q < RCAP; q++) rank += (q < row.n && row.c[q] < row.c[k]) ? 1 : 0;
+    rcol[(long long)er * RCAP + rank] = row.c[k];
+    rval[(long long)er * RCAP + rank] = row.v[k];
   }
   entries = (unsigned long long)row.n;
 }
@@ -397,23 +409,28 @@ __device__ void build_row(const Blocks &B, const int4 *__restrict__ nbr,
 __global__ __launch_bounds__(AWG) void ka_rows(Blocks B, const int4 *__restrict__ nbr,
                                                const int2 *__restrict__ fid,
                                                const unsigned long long *__restrict__ mask,
-                                               const int *__restrict__ bptr, unsigned *rlen, int *rrow,
-                                               int *rcol, double *rval, Errs *e) {
-  const long long b = (blockIdx.x * (long long)AWG + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (b >= B.nb) return;  // uniform over the wavefront
-  const unsigned long long m = mask[b];
+                                               const int *__restrict__ bptr, int n_exc, unsigned *rlen,
+                                               int *rrow, int *rcol, double *rval, Errs *e) {
+  const int er = blockIdx.x * AWG + threadIdx.x;
   int wide = INT_MAX;
   unsigned long long entries = 0;
-  if ((m >> lane) & 1) {
-    build_row(B, nbr, fid, b, lane, bptr[b] + __popcll(m & ((1ull << lane) - 1)), rlen, rrow, rcol,
-              rval, wide, entries);
+  if (er < n_exc) {
+    // block: the last b with bptr[b] <= er; cell: the k-th explicit cell of b
+    long long lo = 0, hi = B.nb - 1;
+    while (lo < hi) {
+      const long long mid = (lo + hi + 1) >> 1;
+      if (bptr[mid] <= er) lo = mid;
+      else hi = mid - 1;
+    }
+    unsigned long long m = mask[lo];
+    for (int k = er - bptr[lo]; k > 0; k--) m &= m - 1;  // drop the k lowest set bits
+    build_row(B, nbr, fid, lo, __ffsll((long long)m) - 1, er, rlen, rrow, rcol, rval, wide, entries);
   }
   for (int o = 32; o > 0; o >>= 1) {
     entries += __shfl_xor(entries, o);
     wide = min(wide, __shfl_xor(wide, o));
   }
-  if (lane == 0) {
+  if ((threadIdx.x & 63) == 0) {
     atomicAdd(&e->nnz, entries);
     atomicMin(&e->wide, wide);
   }
@@ -467,18 +484,13 @@ __global__ __launch_bounds__(AWG) void ka_ell(int n_exc, int W, const int *__res
   if ((s & 63) == 63 || s == n_exc - 1) ewid[s >> 6] = (uint8_t)len;
 }
 
-// Device temporaries of one assembly, freed on every return.
+// Work arrays of one assembly, from the caller's allocator.
 struct Temps {
-  std::vector<void *> p;
-  ~Temps() {
-    for (void *q : p) (void)hipFree(q);
-  }
+  const DeviceAlloc &a;
   template <class T>
   hipError_t get(T **out, size_t count) {
-    *out = nullptr;
-    hipError_t e = hipMalloc((void **)out, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) p.push_back(*out);
-    return e;
+    *out = static_cast<T *>(a(std::max<size_t>(count, 1) * sizeof(T)));
+    return *out ? hipSuccess : hipErrorOutOfMemory;
   }
 };
 
@@ -494,7 +506,8 @@ struct Temps {
   } while (0)
 
 int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
-                       const DeviceAlloc &alloc, DeviceOperator &out, std::string &err) {
+                       const DeviceAlloc &alloc, const DeviceAlloc &temp, DeviceOperator &out,
+                       std::string &err) {
   const long long nb = m.n_blocks;
   if (nb <= 0 || m.bpdx <= 0 || m.bpdy <= 0 || m.levels <= 0 || m.levels > 24 || m.extent <= 0 ||
       !m.level || !m.bx || !m.by) {
@@ -513,7 +526,7 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
         alloc(std::max<size_t>(count, 1) * sizeof(**p)));
     return *p != nullptr;
   };
-  Temps T;
+  Temps T{temp};
 
   // ---- blocks into the hash table ---------------------------------------------
   int *d_lvl, *d_bx, *d_by;
@@ -605,8 +618,8 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
   ACHK(T.get(&d_rval, (size_t)n_exc * RCAP));
   ACHK(T.get(&d_iota, n_exc));
   ACHK(T.get(&d_order, n_exc));
-  ka_rows<<<(int)((nb * 64 + AWG - 1) / AWG), AWG, 0, st>>>(B, out.nbr, d_fid, out.mask, d_bptr, d_rlen,
-                                                          d_rrow, d_rcol, d_rval, d_e);
+  ka_rows<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(B, out.nbr, d_fid, out.mask, d_bptr, n_exc, d_rlen,
+                                                 d_rrow, d_rcol, d_rval, d_e);
   ka_iota<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(n_exc, d_iota);
   ACHK(hipGetLastError());
   // stable: rows of one length keep their mesh order (lengths < 32: 5 bits)

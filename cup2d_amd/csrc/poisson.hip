@@ -778,6 +778,11 @@ std::string g_create_err;
 
 }  // namespace
 
+struct DevBuf {
+  void *p;
+  size_t bytes;
+};
+
 struct cup2d_ps {
   cup2d_ps_config cfg{};
   std::string err;
@@ -819,7 +824,9 @@ struct cup2d_ps {
   // graph of one chunk of iterations
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
-  std::vector<void *> allocs;
+  // device buffers of the current mesh, and those of earlier meshes kept
+  // for reuse: a mesh change of similar size allocates (and frees) nothing
+  std::vector<DevBuf> allocs, spare;
 };
 
 namespace {
@@ -852,13 +859,50 @@ auto guarded(cup2d_ps *h, F &&body) -> decltype(body()) {
                   std::string(#expr) + ": " + hipGetErrorString(_e));                        \
   } while (0)
 
+// A buffer of at least `bytes`: the smallest spare one that is not more
+// than twice as large, else a new one.
+int take(cup2d_ps *h, size_t bytes, DevBuf &out) {
+  size_t best = h->spare.size();
+  for (size_t i = 0; i < h->spare.size(); i++) {
+    const size_t b = h->spare[i].bytes;
+    if (b >= bytes && b <= 2 * bytes + 4096 && (best == h->spare.size() || b < h->spare[best].bytes))
+      best = i;
+  }
+  if (best < h->spare.size()) {
+    out = h->spare[best];
+    h->spare.erase(h->spare.begin() + best);
+    return CUP2D_PS_OK;
+  }
+  out = DevBuf{nullptr, bytes};
+  HIPCHK(h, hipMalloc(&out.p, bytes));
+  return CUP2D_PS_OK;
+}
+
 template <class T>
 int dalloc(cup2d_ps *h, T **p, size_t count) {
   *p = nullptr;
-  if (count == 0) count = 1;
-  HIPCHK(h, hipMalloc((void **)p, count * sizeof(T)));
-  h->allocs.push_back(*p);
+  DevBuf b;
+  const int rc = take(h, std::max<size_t>(count, 1) * sizeof(T), b);
+  if (rc) return rc;
+  h->allocs.push_back(b);
+  *p = static_cast<T *>(b.p);
   return CUP2D_PS_OK;
+}
+
+// Frees spare buffers, largest first, until they hold no more than the
+// current mesh's buffers.
+void trim_spare(cup2d_ps *h) {
+  size_t live = 0, kept = 0;
+  for (const DevBuf &b : h->allocs) live += b.bytes;
+  for (const DevBuf &b : h->spare) kept += b.bytes;
+  std::sort(h->spare.begin(), h->spare.end(),
+            [](const DevBuf &a, const DevBuf &b) { return a.bytes > b.bytes; });
+  size_t i = 0;
+  for (; i < h->spare.size() && kept > live; i++) {
+    (void)hipFree(h->spare[i].p);
+    kept -= h->spare[i].bytes;
+  }
+  h->spare.erase(h->spare.begin(), h->spare.begin() + i);
 }
 
 template <class T>
@@ -874,7 +918,9 @@ void free_mesh(cup2d_ps *h) {
     (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
   }
-  for (void *p : h->allocs) (void)hipFree(p);
+  // the stream is idle (callers synchronise first): the buffers can be
+  // handed out again in stream order
+  h->spare.insert(h->spare.end(), h->allocs.begin(), h->allocs.end());
   h->allocs.clear();
   h->have_mesh = false;
 }
@@ -1208,6 +1254,8 @@ int cup2d_ps_destroy(cup2d_ps *h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_mesh(h);
+  for (const DevBuf &b : h->spare) (void)hipFree(b.p);
+  h->spare.clear();
   if (h->h_s) (void)hipHostFree(h->h_s);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1262,8 +1310,8 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
 static int finish_mesh(cup2d_ps *h) {
   const Operator &op = h->op;
   int rc = 0;
-  const std::vector<double> P = make_pinv();
-  const std::vector<uint16_t> pf = make_pfrag(P);
+  static const std::vector<double> P = make_pinv();  // the same for every mesh
+  static const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = upload(h, &h->d_P, P)) || (rc = dalloc(h, &h->d_pf, 8 * 64))) return rc;
   HIPCHK(h, hipMemcpy(h->d_pf, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   const size_t n = op.n;
@@ -1290,6 +1338,7 @@ static int finish_mesh(cup2d_ps *h) {
   h->Gp = (int)std::max<long long>(1, std::min<long long>(MAXG, (groups + WAVES - 1) / WAVES));
   const long long tiles = (op.nb + TILE - 1) / TILE;
   h->Gm = (int)std::max<long long>(1, std::min<long long>(MAXG, (tiles + WAVES - 1) / WAVES));
+  trim_spare(h);
   h->have_mesh = true;
   return CUP2D_PS_OK;
 }
@@ -1309,7 +1358,16 @@ static int set_mesh_on_device_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
     uint8_t *p = nullptr;
     return dalloc(h, &p, bytes) ? nullptr : p;
   };
-  const int rc = assemble_on_device(md, h->cfg.mean_constraint, h->stream, alloc, D, err);
+  std::vector<DevBuf> temps;  // the assembly's work arrays, spare again afterwards
+  const DeviceAlloc temp = [h, &temps](size_t bytes) -> void * {
+    DevBuf b;
+    if (take(h, bytes, b)) return nullptr;
+    temps.push_back(b);
+    return b.p;
+  };
+  const int rc = assemble_on_device(md, h->cfg.mean_constraint, h->stream, alloc, temp, D, err);
+  (void)hipStreamSynchronize(h->stream);
+  h->spare.insert(h->spare.end(), temps.begin(), temps.end());
   if (rc) {
     free_mesh(h);
     return fail(h, rc, err);

@@ -138,3 +138,23 @@ def test_device_assembled_mesh_has_no_csr():
     s.set_mesh(m)
     assert np.array_equal(s.spmv(x, kind=1), s.spmv(x, kind=0)) or \
         np.abs(s.spmv(x, kind=1) - s.spmv(x, kind=0)).max() <= 1e-14
+
+
+@pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
+def test_mesh_changes_on_one_handle(precision):
+    """Mesh changes of growing and shrinking size on one handle, alternating
+    host and device assembly (the handle reuses its device buffers across
+    meshes), give the operators and solves of fresh handles."""
+    meshes = [W.graded_disk_mesh(), _tiny_cf(), W.four_disk_amr6_mesh(target=3000),
+              W.graded_disk_mesh(), W.uniform_16x8_mesh(), W.four_disk_amr6_mesh(target=3000)]
+    rng = np.random.default_rng(4)
+    s = PS.Solver(norm=PS.NORM_L2, precision=precision, max_iter=4000)
+    for k, m in enumerate(meshes):
+        s.set_mesh(m, on_device=bool(k % 2))
+        fresh = PS.Solver(norm=PS.NORM_L2, precision=precision, max_iter=4000).set_mesh(m)
+        x = rng.normal(size=m.n)
+        assert np.array_equal(s.spmv(x), fresh.spmv(x)), k
+        b = W.vortex_rhs(m, k)
+        x1, st1 = s.solve(b, rel_tol=1e-10)
+        x2, st2 = fresh.solve(b, rel_tol=1e-10)
+        assert st1["converged"] and np.array_equal(x1, x2), (k, st1, st2)
