@@ -42,6 +42,10 @@ MESHES = {
     "periodic_x_only": lambda: M.graded(8, 8, 4, 1.0, [(0.02, 0.5, 0.05), (0.7, 0.98, 0.03)],
                                         [0.2, 0.1, 0.05], True, False),
     "amr6_1M": W.four_disk_amr6_mesh,
+    # one block column: a block is its own periodic neighbour
+    "self_periodic": lambda: M.graded(1, 2, 1, 1.0, [], [], True, False),
+    "self_periodic_amr": lambda: M.graded(1, 1, 3, 1.0, [(0.3, 0.3, 0.1)], [0.3, 0.15], True, True),
+    "periodic_cf_wrap": lambda: M.graded(2, 1, 4, 1.0, [(0.05, 0.5, 0.05)], [0.3, 0.15, 0.08], True, True),
 }
 
 
