@@ -654,6 +654,57 @@ __global__ __launch_bounds__(WG) void k_final_write(int nb, const double *__rest
 }
 
 // ----------------------------------------------------------------------------
+// the pressure step around the solve (PressureSingle.cpp:842-879)
+// ----------------------------------------------------------------------------
+
+// b = rhs - L p_old with L the operator without the mean-row replacement:
+// the old pressure's Laplacian taken off the RHS (updatePressureRHS1,
+// PressureSingle.cpp:401-456, applied at :848-849).
+__global__ __launch_bounds__(WG) void k_rhs_update(OpView A, int Gb, const double *__restrict__ p_old,
+                                                   const double *__restrict__ rhs,
+                                                   double *__restrict__ b) {
+  if ((int)blockIdx.x < Gb) {
+    const int c = lane_id();
+    block_groups<double>(A, Gb, p_old, rhs, false, [&](const BlockRows<SPMV_U, double> &R, int u) {
+      bool plain;
+      const double yi = R.y(u, c, plain);
+      if (plain) b[R.i(u, c)] = R.a(u) - yi;
+    });
+  } else {
+    const int ge = gridDim.x - Gb;
+    for (int e = (blockIdx.x - Gb) * WG + threadIdx.x; e < A.n_exc; e += ge * WG) {
+      const int r = A.erow[e];
+      b[r] = rhs[r] - explicit_row<double>(A, p_old, e, ell_run_width(A, e));
+    }
+  }
+}
+
+// Partials of sum h^2 dp and sum h^2 (PressureSingle.cpp:853-868).
+__global__ __launch_bounds__(WG) void k_h2mean_part(int nb, const double *__restrict__ dp,
+                                                    const double *__restrict__ h2, double *pf) {
+  const long n = (long)nb * 64;
+  double v[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double hh = h2[i >> 6];
+    v[0] += dp[i] * hh;
+    v[1] += hh;
+  }
+  store_partials<2, 0u>(v, pf);
+}
+
+// p = dp + (p_old - avg), avg the h^2-weighted mean of dp (PressureSingle.cpp:869-879).
+__global__ __launch_bounds__(WG) void k_pressure_add(int nb, double *__restrict__ p,
+                                                     const double *__restrict__ dp, const double *pf,
+                                                     int Gf) {
+  double o[2];
+  load_totals<2, 0u>(pf, Gf, o);
+  const double avg = o[0] / o[1];
+  const long n = (long)nb * 64;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    p[i] = dp[i] + (p[i] - avg);
+}
+
+// ----------------------------------------------------------------------------
 // operator pieces exposed by the C ABI
 // ----------------------------------------------------------------------------
 
@@ -1417,6 +1468,29 @@ int cup2d_ps_solve(cup2d_ps *h, const double *b, double *x, double tol_abs, doub
   return CUP2D_PS_OK;
 }
 
+// The pressure step of PressureSingle::operator() around the solve, on the
+// device: b = rhs - L p_old, dp from x0 = 0, p = p_old + dp - <dp>_h2.
+static int pressure_step_body(cup2d_ps *h, const double *d_rhs, double *d_p, double tol_abs,
+                              double tol_rel, int max_restarts, cup2d_ps_stats *st) {
+  if (!h || !d_rhs || !d_p) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "pressure step before cup2d_ps_set_mesh");
+  if (!(tol_abs >= 0) || !(tol_rel >= 0) || max_restarts < 0)
+    return fail(h, CUP2D_PS_EINVAL, "negative tolerance or restart count");
+  if (d_rhs == d_p) return fail(h, CUP2D_PS_EINVAL, "rhs and p must not alias");
+  HIPCHK(h, hipSetDevice(h->device));
+  const int nb = (int)h->op.nb, G = h->G;
+  k_rhs_update<<<G, WG, 0, h->stream>>>(view(h), h->Gb, d_p, d_rhs, h->d_b);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemsetAsync(h->d_xio, 0, h->op.n * sizeof(double), h->stream));  // x0 = 0 (:845)
+  const int rc = solve_impl(h, h->d_b, h->d_xio, h->d_xio, tol_abs, tol_rel, max_restarts, st);
+  if (rc) return rc;
+  k_h2mean_part<<<G, WG, 0, h->stream>>>(nb, h->d_xio, h->d_h2, h->d_pfin);
+  k_pressure_add<<<G, WG, 0, h->stream>>>(nb, d_p, h->d_xio, h->d_pfin, G);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
 int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind) {
   if (!h || !d_x || !d_y) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
@@ -1676,6 +1750,13 @@ int cup2d_ps_set_mesh(cup2d_ps *h, const cup2d_mesh_desc *m) {
 
 int cup2d_ps_set_mesh_on_device(cup2d_ps *h, const cup2d_mesh_desc *m) {
   return guarded(h, [&] { return set_mesh_on_device_body(h, m); });
+}
+
+int cup2d_ps_pressure_step_device(cup2d_ps *h, const double *d_rhs, double *d_p, double tol_abs,
+                                  double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+  return guarded(h, [&] {
+    return pressure_step_body(h, d_rhs, d_p, tol_abs, tol_rel, max_restarts, stats);
+  });
 }
 
 int cup2d_ps_kernel_time(cup2d_ps *h, int which, int reps, double *ms_per_launch) {

@@ -185,6 +185,23 @@ class Solver:
                                                          ctypes.byref(st)))
         return st.as_dict()
 
+    def pressure_step_device(self, d_rhs: DeviceArray, d_p: DeviceArray, tol=0.0, rel_tol=1e-10,
+                             max_restarts=100):
+        """The pressure step around the solve (PressureSingle.cpp:842-879) on
+        device arrays: b = rhs - L p_old, dp solved from 0, p <- p_old + dp -
+        <dp>_h2.  d_p holds p_old on entry and p on exit; d_rhs is kept."""
+        st = _lib.Stats()
+        _check(self.h, _lib.load().cup2d_ps_pressure_step_device(
+            self.h, d_rhs.ptr, d_p.ptr, float(tol), float(rel_tol), int(max_restarts), ctypes.byref(st)))
+        return st.as_dict()
+
+    def pressure_step(self, rhs, p_old, tol=0.0, rel_tol=1e-10, max_restarts=100):
+        """Host arrays in, (p, stats) out."""
+        d_rhs = DeviceArray.from_host(self, np.ascontiguousarray(rhs, dtype=np.float64))
+        d_p = DeviceArray.from_host(self, np.ascontiguousarray(p_old, dtype=np.float64))
+        st = self.pressure_step_device(d_rhs, d_p, tol, rel_tol, max_restarts)
+        return d_p.numpy(), st
+
     # -- operator pieces ---------------------------------------------------
     def spmv(self, x, kind=0):
         dx = DeviceArray.from_host(self, x)

@@ -121,6 +121,18 @@ int cup2d_ps_solve(cup2d_ps *h, const double *b, double *x, double tol_abs, doub
 int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs,
                           double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
+/* The pressure step around the solve, device-resident (reference
+ * PressureSingle::operator(), PressureSingle.cpp:842-879):
+ *   b  = rhs - L p_old   (updatePressureRHS1, :401-456, applied at :848-849;
+ *                         L = the mesh's operator without the mean-row replacement)
+ *   dp = solve A dp = b from dp = 0 (:845, :851; as cup2d_ps_solve_device)
+ *   p  = dp + (p_old - <dp>_h2), <.>_h2 the h^2-weighted mean (:853-879).
+ * d_rhs: the divergence part of the RHS (updatePressureRHS's output, PressureSingle.cpp:324-398),
+ * n doubles, not modified.  d_p: p_old on entry, p on exit (must not alias d_rhs).
+ * The stats are the solve's. */
+int cup2d_ps_pressure_step_device(cup2d_ps *h, const double *d_rhs, double *d_p, double tol_abs,
+                                  double tol_rel, int max_restarts, cup2d_ps_stats *stats);
+
 /* y = A x on the device (the operator of the last cup2d_ps_set_mesh, with the
  * mean-constraint row if enabled).  kind 0: stencil kernel used by the solver,
  * kind 1: plain CSR kernel.  Reference: BiCGSTABSolver::hd_cusparseSpMV
