@@ -1,10 +1,14 @@
-// assemble_device.hip -- the AMR Poisson operator assembled on the GPU.
+// assemble_device.hip -- the AMR Poisson operator assembled on the GPU, for
+// one mesh or for a batch of independent meshes at once.
 //
 // Restates the host assembly of assembly.cpp (reference
-// source/Poisson/ExpAMRSolver.cpp:100-313) as a short pipeline of kernels:
-//   1. ka_insert / ka_dups: the blocks go into an open-addressing hash table
-//      keyed by (level, i, j), the lookup that getBlockInfoAll(level, Z)
-//      gives the reference, with the host's range and duplicate checks;
+// source/Poisson/ExpAMRSolver.cpp:100-313) as a short pipeline of kernels
+// over the concatenated blocks of all meshes (global block ids, so columns
+// come out shifted by each mesh's offset):
+//   1. ka_insert / ka_dups: the blocks go into open-addressing hash tables
+//      keyed by (level, i, j), one region per mesh -- the lookup that
+//      getBlockInfoAll(level, Z) gives the reference -- with the host's
+//      range and duplicate checks;
 //   2. ka_faces: one thread per block classifies its four faces (same level,
 //      wall, one level coarser, one level finer; ExpAMRSolver.cpp:221-249,
 //      :141-177) and writes the neighbour table, the explicit-cell mask, h^2
@@ -16,8 +20,8 @@
 //      row in registers (ghost interpolation ExpAMRSolver.cpp:100-128, D1/D2
 //      ExpAMRSolver.h:268-300) with the host's sequence of additions and
 //      stores it sorted by column;
-//   5. a stable radix sort orders the rows by length; a sort + unique of the
-//      coefficients' bit patterns gives the coefficient table;
+//   5. a stable radix sort orders the rows by (mesh, length); a sort + unique
+//      of the coefficients' bit patterns gives the coefficient table;
 //   6. ka_ell writes the column-major ELL arrays, each coefficient's code (a
 //      binary search in the table) and the width of every run of 64 rows.
 // Floating-point contraction is off in this file: every coefficient is the
@@ -45,26 +49,41 @@ constexpr int AWG = 256;                    // threads per workgroup
 constexpr unsigned long long EMPTY = ~0ull;  // free hash slot
 constexpr int RCAP = ELL_MAX;               // entries a row can hold; one more is an error
 
-struct Blocks {
-  const int *lvl, *bx, *by;
-  long long nb;
+// One mesh of the batch.
+struct Inst {
+  long long boff;           // its first block
+  unsigned long long toff;  // its first hash slot
+  unsigned long long mask;  // its hash slots - 1 (a power of two minus one)
+  double h0;                // cell size at level 0
   int levels, bpdx, bpdy, px, py;
+};
+
+struct Blocks {
+  const int *lvl, *bx, *by, *inst;  // [nb]: level, index, mesh
+  const Inst *I;                    // [K]
+  long long nb;
 };
 
 struct Hash {
   unsigned long long *keys;  // pos_key, EMPTY when free
   int *vals;                 // smallest block with that key
-  unsigned long long mask;   // slots - 1 (a power of two minus one)
 };
 
-// Failure reports and counters, reduced with atomics (all integers, so the
-// result does not depend on the order of the updates).
+// Failure reports and counters of one mesh, reduced with integer atomics
+// (the result does not depend on the order of the updates).  Block numbers
+// are local to the mesh.
 struct Errs {
-  int lvl, idx, dup;  // smallest block with a level / index out of range, or a duplicate
-  int face;           // smallest 4 b + f whose neighbour is neither same, coarser nor finer
-  int wide;           // smallest explicit row wider than RCAP
-  int corner;         // last block with index (0, 0), -1 when none
+  int blk[3];  // smallest block with a level / index out of range, or a duplicate
+  int face;    // smallest 4 b + f whose neighbour is neither same, coarser nor finer
+  int corner;  // last block with index (0, 0), -1 when none
+  int pad;
   unsigned long long nnz;
+};
+
+// Over all meshes.
+struct Glob {
+  int wide;    // smallest explicit row wider than RCAP, or INT_MAX
+  int maxlen;  // widest explicit row
 };
 
 __device__ __forceinline__ unsigned long long pos_key(int l, long long i, long long j) {
@@ -81,13 +100,13 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   return z;
 }
 
-__device__ long long lookup(const Hash &H, int levels, int l, long long i, long long j) {
-  if (l < 0 || l >= levels) return -1;
+__device__ long long lookup(const Hash &H, const Inst &in, int l, long long i, long long j) {
+  if (l < 0 || l >= in.levels) return -1;
   const unsigned long long key = pos_key(l, i, j);
-  unsigned long long s = mix64(key) & H.mask;
-  for (unsigned long long p = 0; p <= H.mask; p++, s = (s + 1) & H.mask) {
-    const unsigned long long k = H.keys[s];
-    if (k == key) return H.vals[s];
+  unsigned long long s = mix64(key) & in.mask;
+  for (unsigned long long p = 0; p <= in.mask; p++, s = (s + 1) & in.mask) {
+    const unsigned long long k = H.keys[in.toff + s];
+    if (k == key) return H.vals[in.toff + s];
     if (k == EMPTY) return -1;
   }
   return -1;
@@ -95,16 +114,22 @@ __device__ long long lookup(const Hash &H, int levels, int l, long long i, long 
 
 __device__ __forceinline__ long long gidx(long long b, int ix, int iy) { return b * BLEN + iy * BS + ix; }
 
-__global__ __launch_bounds__(AWG) void ka_init(Hash H, Errs *e) {
+__global__ __launch_bounds__(AWG) void ka_init(Hash H, unsigned long long slots, Errs *E, int K, Glob *g) {
   const unsigned long long stride = (unsigned long long)gridDim.x * AWG;
-  for (unsigned long long s = blockIdx.x * (unsigned long long)AWG + threadIdx.x; s <= H.mask; s += stride) {
+  const unsigned long long t = blockIdx.x * (unsigned long long)AWG + threadIdx.x;
+  for (unsigned long long s = t; s < slots; s += stride) {
     H.keys[s] = EMPTY;
     H.vals[s] = INT_MAX;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    e->lvl = e->idx = e->dup = e->face = e->wide = INT_MAX;
-    e->corner = -1;
-    e->nnz = 0;
+  for (unsigned long long k = t; k < (unsigned long long)K; k += stride) {
+    E[k].blk[0] = E[k].blk[1] = E[k].blk[2] = E[k].face = INT_MAX;
+    E[k].corner = -1;
+    E[k].pad = 0;
+    E[k].nnz = 0;
+  }
+  if (t == 0) {
+    g->wide = INT_MAX;
+    g->maxlen = 0;
   }
 }
 
@@ -113,27 +138,30 @@ __global__ __launch_bounds__(AWG) void ka_init(Hash H, Errs *e) {
 enum : int { ST_OK = 0, ST_LEVEL = 1, ST_INDEX = 2, ST_DUP = 3 };
 
 // Failures are recorded per block with plain stores and reduced by
-// ka_status, whose atomics are straight-line code: error atomics on several
-// divergent paths of one kernel (the hash insertion's atomicMin among them)
-// were merged by the compiler into one instruction whose address register
-// was overwritten on one of the paths.
+// ka_status through one atomic whose address is computed, not chosen by
+// branches: error atomics on several divergent paths of one kernel (the
+// hash insertion's atomicMin among them) were merged by the compiler into
+// one instruction whose address register was overwritten on one of the
+// paths.  For the same reason each kernel below has at most one atomic site
+// per operation.
 __global__ __launch_bounds__(AWG) void ka_insert(Blocks B, Hash H, int *status) {
   const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
   if (b >= B.nb) return;
+  const Inst in = B.I[B.inst[b]];
   const int l = B.lvl[b], i = B.bx[b], j = B.by[b];
-  const bool lvl_ok = l >= 0 && l < B.levels;
+  const bool lvl_ok = l >= 0 && l < in.levels;
   const int ls = lvl_ok ? l : 0;
-  const long long nx = (long long)B.bpdx << ls, ny = (long long)B.bpdy << ls;
+  const long long nx = (long long)in.bpdx << ls, ny = (long long)in.bpdy << ls;
   const bool idx_ok = i >= 0 && i < nx && j >= 0 && j < ny;
   const int st = !lvl_ok ? ST_LEVEL : !idx_ok ? ST_INDEX : ST_OK;
   status[b] = st;
   if (st != ST_OK) return;
   const unsigned long long key = pos_key(l, i, j);
-  unsigned long long s = mix64(key) & H.mask;
-  for (unsigned long long p = 0; p <= H.mask; p++, s = (s + 1) & H.mask) {
-    const unsigned long long prev = atomicCAS(&H.keys[s], EMPTY, key);
+  unsigned long long s = mix64(key) & in.mask;
+  for (unsigned long long p = 0; p <= in.mask; p++, s = (s + 1) & in.mask) {
+    const unsigned long long prev = atomicCAS(&H.keys[in.toff + s], EMPTY, key);
     if (prev == EMPTY || prev == key) {
-      atomicMin(&H.vals[s], (int)b);  // the first occurrence owns the position
+      atomicMin(&H.vals[in.toff + s], (int)b);  // the first occurrence owns the position
       return;
     }
   }
@@ -144,107 +172,98 @@ __global__ __launch_bounds__(AWG) void ka_insert(Blocks B, Hash H, int *status) 
 __global__ __launch_bounds__(AWG) void ka_dups(Blocks B, Hash H, int *status) {
   const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
   if (b >= B.nb || status[b] != ST_OK) return;
-  if (lookup(H, B.levels, B.lvl[b], B.bx[b], B.by[b]) != b) status[b] = ST_DUP;
+  const Inst in = B.I[B.inst[b]];
+  if (lookup(H, in, B.lvl[b], B.bx[b], B.by[b]) != b) status[b] = ST_DUP;
 }
 
-__device__ __forceinline__ int wave_min_int(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
-}
-
-// Smallest block of each failure kind.
-__global__ __launch_bounds__(AWG) void ka_status(long long nb, const int *__restrict__ status, Errs *e) {
+// Smallest block of each failure kind, per mesh.
+__global__ __launch_bounds__(AWG) void ka_status(Blocks B, const int *__restrict__ status, Errs *E) {
   const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
-  const int st = b < nb ? status[b] : ST_OK;
-  const int lv = wave_min_int(st == ST_LEVEL ? (int)b : INT_MAX);
-  const int ix = wave_min_int(st == ST_INDEX ? (int)b : INT_MAX);
-  const int du = wave_min_int(st == ST_DUP ? (int)b : INT_MAX);
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&e->lvl, lv);
-    atomicMin(&e->idx, ix);
-    atomicMin(&e->dup, du);
+  if (b >= B.nb) return;
+  const int st = status[b];
+  if (st != ST_OK) {
+    const int k = B.inst[b];
+    atomicMin(&E[k].blk[st - 1], (int)(b - B.I[k].boff));
   }
 }
 
 // Faces of block b: nbr (same-level id, -1 wall, -2 coarse-fine), fid (the
 // coarser block, or the two finer blocks low/high along the face), the
-// explicit cells, h^2, and the nonzeros of the plain rows.
-__global__ __launch_bounds__(AWG) void ka_faces(Blocks B, Hash H, double h0, int4 *nbr, int2 *fid,
-                                                unsigned long long *mask, int *cnt, double *h2, Errs *e) {
+// explicit cells, h^2, and the nonzeros of the plain rows.  Blocks of meshes
+// with a block failure are skipped (no rows).
+__global__ __launch_bounds__(AWG) void ka_faces(Blocks B, Hash H, int4 *nbr, int2 *fid,
+                                                unsigned long long *mask, int *cnt, double *h2, Errs *E) {
   const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
-  long long plain = 0;
-  int bad_face = INT_MAX, corner = -1;
-  if (b < B.nb) {
-    const int l = B.lvl[b];
-    const long long i = B.bx[b], j = B.by[b];
-    const long long nx = (long long)B.bpdx << l, ny = (long long)B.bpdy << l;
-    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
-    int v[4];
+  if (b == 0) cnt[B.nb] = 0;
+  if (b >= B.nb) return;
+  const int k = B.inst[b];
+  const Errs ek = E[k];
+  if (ek.blk[0] != INT_MAX || ek.blk[1] != INT_MAX || ek.blk[2] != INT_MAX) {
+    cnt[b] = 0;
+    return;
+  }
+  const Inst in = B.I[k];
+  const int l = B.lvl[b];
+  const long long i = B.bx[b], j = B.by[b];
+  const long long nx = (long long)in.bpdx << l, ny = (long long)in.bpdy << l;
+  const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+  int v[4];
+  int bad_face = INT_MAX;
 #pragma unroll
-    for (int f = 0; f < 4; f++) {
-      long long I = i + di[f], J = j + dj[f];
-      const bool outx = I < 0 || I >= nx, outy = J < 0 || J >= ny;
-      int2 fd = make_int2(-1, -1);
-      v[f] = -1;
-      if (!((outx && !B.px) || (outy && !B.py))) {
-        I = (I + nx) % nx;
-        J = (J + ny) % ny;
-        long long s;
-        if ((s = lookup(H, B.levels, l, I, J)) >= 0) {
-          v[f] = (int)s;
-          fd.x = (int)s;
-        } else if ((s = lookup(H, B.levels, l - 1, I >> 1, J >> 1)) >= 0) {
-          v[f] = -2;
-          fd.x = (int)s;
+  for (int f = 0; f < 4; f++) {
+    long long I = i + di[f], J = j + dj[f];
+    const bool outx = I < 0 || I >= nx, outy = J < 0 || J >= ny;
+    int2 fd = make_int2(-1, -1);
+    v[f] = -1;
+    if (!((outx && !in.px) || (outy && !in.py))) {
+      I = (I + nx) % nx;
+      J = (J + ny) % ny;
+      long long s;
+      if ((s = lookup(H, in, l, I, J)) >= 0) {
+        v[f] = (int)s;
+        fd.x = (int)s;
+      } else if ((s = lookup(H, in, l - 1, I >> 1, J >> 1)) >= 0) {
+        v[f] = -2;
+        fd.x = (int)s;
+      } else {
+        long long c[2];
+        for (int hh = 0; hh < 2; hh++) {
+          const long long ci = f < 2 ? 2 * I + (f == 0 ? 1 : 0) : 2 * I + hh;
+          const long long cj = f < 2 ? 2 * J + hh : 2 * J + (f == 2 ? 1 : 0);
+          c[hh] = lookup(H, in, l + 1, ci, cj);
+        }
+        if (c[0] < 0 || c[1] < 0) {
+          bad_face = min(bad_face, (int)(4 * (b - in.boff) + f));
         } else {
-          long long c[2];
-          for (int hh = 0; hh < 2; hh++) {
-            const long long ci = f < 2 ? 2 * I + (f == 0 ? 1 : 0) : 2 * I + hh;
-            const long long cj = f < 2 ? 2 * J + hh : 2 * J + (f == 2 ? 1 : 0);
-            c[hh] = lookup(H, B.levels, l + 1, ci, cj);
-          }
-          if (c[0] < 0 || c[1] < 0) {
-            bad_face = min(bad_face, (int)(4 * b + f));
-          } else {
-            v[f] = -2;
-            fd = make_int2((int)c[0], (int)c[1]);
-          }
+          v[f] = -2;
+          fd = make_int2((int)c[0], (int)c[1]);
         }
       }
-      fid[4 * b + f] = fd;
     }
-    nbr[b] = make_int4(v[0], v[1], v[2], v[3]);
-    const unsigned long long C0 = 0x0101010101010101ull, R0 = 0xffull;
-    unsigned long long m = 0;
-    if (v[0] == -2) m |= C0;
-    if (v[1] == -2) m |= C0 << 7;
-    if (v[2] == -2) m |= R0;
-    if (v[3] == -2) m |= R0 << 56;
-    mask[b] = m;
-    cnt[b] = __popcll(m);
-    if (b == 0) cnt[B.nb] = 0;
-    // a plain row: the cell and each face that is inside the block or same-level
-    for (int c = 0; c < BLEN; c++) {
-      if ((m >> c) & 1) continue;
-      const int ix = c & 7, iy = c >> 3;
-      plain += 1 + (ix > 0 || v[0] >= 0) + (ix < BS - 1 || v[1] >= 0) + (iy > 0 || v[2] >= 0) +
-               (iy < BS - 1 || v[3] >= 0);
-    }
-    const double h = h0 / double(1LL << l);
-    h2[b] = h * h;
-    if (i == 0 && j == 0) corner = (int)b;
+    fid[4 * b + f] = fd;
   }
-  // one lane per wavefront reports, in straight-line code
-  for (int o = 32; o > 0; o >>= 1) {
-    plain += __shfl_xor(plain, o);
-    bad_face = min(bad_face, __shfl_xor(bad_face, o));
-    corner = max(corner, __shfl_xor(corner, o));
+  nbr[b] = make_int4(v[0], v[1], v[2], v[3]);
+  const unsigned long long C0 = 0x0101010101010101ull, R0 = 0xffull;
+  unsigned long long m = 0;
+  if (v[0] == -2) m |= C0;
+  if (v[1] == -2) m |= C0 << 7;
+  if (v[2] == -2) m |= R0;
+  if (v[3] == -2) m |= R0 << 56;
+  mask[b] = m;
+  cnt[b] = __popcll(m);
+  // a plain row: the cell and each face that is inside the block or same-level
+  long long plain = 0;
+  for (int c = 0; c < BLEN; c++) {
+    if ((m >> c) & 1) continue;
+    const int ix = c & 7, iy = c >> 3;
+    plain += 1 + (ix > 0 || v[0] >= 0) + (ix < BS - 1 || v[1] >= 0) + (iy > 0 || v[2] >= 0) +
+             (iy < BS - 1 || v[3] >= 0);
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&e->nnz, (unsigned long long)plain);
-    atomicMin(&e->face, bad_face);
-    atomicMax(&e->corner, corner);
-  }
+  const double h = in.h0 / double(1LL << l);
+  h2[b] = h * h;
+  atomicAdd(&E[k].nnz, (unsigned long long)plain);
+  if (bad_face != INT_MAX) atomicMin(&E[k].face, bad_face);
+  if (i == 0 && j == 0) atomicMax(&E[k].corner, (int)(b - in.boff));
 }
 
 // Coefficients of one explicit row, per column in order of first appearance.
@@ -410,10 +429,10 @@ __global__ __launch_bounds__(AWG) void ka_rows(Blocks B, const int4 *__restrict_
                                                const int2 *__restrict__ fid,
                                                const unsigned long long *__restrict__ mask,
                                                const int *__restrict__ bptr, int n_exc, unsigned *rlen,
-                                               int *rrow, int *rcol, double *rval, Errs *e) {
+                                               unsigned *rkey, int *rrow, int *rcol, double *rval,
+                                               Glob *g) {
   const int er = blockIdx.x * AWG + threadIdx.x;
-  int wide = INT_MAX;
-  unsigned long long entries = 0;
+  int wide = INT_MAX, len = 0;
   if (er < n_exc) {
     // block: the last b with bptr[b] <= er; cell: the k-th explicit cell of b
     long long lo = 0, hi = B.nb - 1;
@@ -424,15 +443,19 @@ __global__ __launch_bounds__(AWG) void ka_rows(Blocks B, const int4 *__restrict_
     }
     unsigned long long m = mask[lo];
     for (int k = er - bptr[lo]; k > 0; k--) m &= m - 1;  // drop the k lowest set bits
+    unsigned long long entries = 0;
     build_row(B, nbr, fid, lo, __ffsll((long long)m) - 1, er, rlen, rrow, rcol, rval, wide, entries);
+    len = (int)rlen[er];
+    rkey[er] = ((unsigned)B.inst[lo] << 5) | (unsigned)len;  // sort key: mesh, then length
   }
+  // one lane per wavefront reports, in straight-line code
   for (int o = 32; o > 0; o >>= 1) {
-    entries += __shfl_xor(entries, o);
+    len = max(len, __shfl_xor(len, o));
     wide = min(wide, __shfl_xor(wide, o));
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&e->nnz, entries);
-    atomicMin(&e->wide, wide);
+    atomicMax(&g->maxlen, len);
+    atomicMin(&g->wide, wide);
   }
 }
 
@@ -453,35 +476,50 @@ __global__ __launch_bounds__(AWG) void ka_keys(long long n_exc, const unsigned *
   keys[q] = (er < n_exc && w < (int)rlen[er]) ? (unsigned long long)__double_as_longlong(rval[q]) : 0ull;
 }
 
+// ELL slot s (rows in (mesh, length) order): row, columns, codes; the width
+// of each run of 64 rows (a wavefront's rows) and each row's length.
 __global__ __launch_bounds__(AWG) void ka_ell(int n_exc, int W, const int *__restrict__ order,
                                               const unsigned *__restrict__ rlen,
                                               const int *__restrict__ rrow, const int *__restrict__ rcol,
                                               const double *__restrict__ rval,
                                               const unsigned long long *__restrict__ table, int n_codes,
-                                              int *erow, int *ecol, uint16_t *ecode, uint8_t *ewid) {
+                                              int *erow, int *ecol, uint16_t *ecode, uint8_t *ewid,
+                                              uint8_t *elen) {
   const int s = blockIdx.x * AWG + threadIdx.x;
-  if (s >= n_exc) return;
-  const int r = order[s];
-  const int len = (int)rlen[r], row = rrow[r];
-  erow[s] = row;
-  for (int w = 0; w < W; w++) {
-    int col = row, code = 0;
-    if (w < len) {
-      col = rcol[(long long)r * RCAP + w];
-      const unsigned long long key = (unsigned long long)__double_as_longlong(rval[(long long)r * RCAP + w]);
-      int lo = 0, hi = n_codes - 1;  // the key is in the table
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (table[mid] < key) lo = mid + 1;
-        else hi = mid;
+  int len = 0;
+  if (s < n_exc) {
+    const int r = order[s];
+    len = (int)rlen[r];
+    const int row = rrow[r];
+    erow[s] = row;
+    elen[s] = (uint8_t)len;
+    for (int w = 0; w < W; w++) {
+      int col = row, code = 0;
+      if (w < len) {
+        col = rcol[(long long)r * RCAP + w];
+        const unsigned long long key =
+            (unsigned long long)__double_as_longlong(rval[(long long)r * RCAP + w]);
+        int lo = 0, hi = n_codes - 1;  // the key is in the table
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (table[mid] < key) lo = mid + 1;
+          else hi = mid;
+        }
+        code = lo;
       }
-      code = lo;
+      ecol[(long long)w * n_exc + s] = col;
+      ecode[(long long)w * n_exc + s] = (uint16_t)code;
     }
-    ecol[(long long)w * n_exc + s] = col;
-    ecode[(long long)w * n_exc + s] = (uint16_t)code;
   }
-  // rows are in ascending length: the last row of a run is its widest
-  if ((s & 63) == 63 || s == n_exc - 1) ewid[s >> 6] = (uint8_t)len;
+  for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+  if ((threadIdx.x & 63) == 0 && s < n_exc) ewid[s >> 6] = (uint8_t)len;
+}
+
+// First explicit row of each mesh: eoff[k] = bptr[boff[k]], k = 0..K.
+__global__ __launch_bounds__(AWG) void ka_eoff(int K, const long long *__restrict__ boff,
+                                               const int *__restrict__ bptr, long long *eoff) {
+  const int k = blockIdx.x * AWG + threadIdx.x;
+  if (k <= K) eoff[k] = bptr[boff[k]];
 }
 
 // Work arrays of one assembly, from the caller's allocator.
@@ -494,6 +532,17 @@ struct Temps {
   }
 };
 
+// The host's message for a mesh's first failure (build_operator's order:
+// block checks, then faces).
+std::string mesh_error(const Errs &e) {
+  const int eb = std::min({e.blk[0], e.blk[1], e.blk[2]});
+  if (eb != INT_MAX)
+    return "block " + std::to_string(eb) +
+           (eb == e.blk[0] ? ": level out of range" : eb == e.blk[1] ? ": index out of range" : ": duplicate");
+  return "block " + std::to_string(e.face / 4) + " face " + std::to_string(e.face % 4) +
+         ": neighbour is neither same-level, coarser nor finer (mesh not 2:1 balanced)";
+}
+
 }  // namespace
 
 #define ACHK(expr)                                                                      \
@@ -505,20 +554,42 @@ struct Temps {
     }                                                                                   \
   } while (0)
 
-int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
+int assemble_on_device(const MeshDesc *meshes, int K, int mean_constraint, hipStream_t st,
                        const DeviceAlloc &alloc, const DeviceAlloc &temp, DeviceOperator &out,
-                       std::string &err) {
-  const long long nb = m.n_blocks;
-  if (nb <= 0 || m.bpdx <= 0 || m.bpdy <= 0 || m.levels <= 0 || m.levels > 24 || m.extent <= 0 ||
-      !m.level || !m.bx || !m.by) {
-    err = "invalid mesh description";
-    return CUP2D_PS_EINVAL;
-  }
-  if (nb * BLEN >= (1LL << 31) / 8) {
-    err = "mesh too large for 32-bit column indices";
+                       int &bad_mesh, std::string &err) {
+  bad_mesh = -1;
+  if (!meshes || K <= 0 || K > MAX_MESHES) {
+    err = "need 1.." + std::to_string(MAX_MESHES) + " meshes";
     return CUP2D_PS_EINVAL;
   }
   out = DeviceOperator();
+  std::vector<Inst> inst(K);
+  long long nb = 0;
+  unsigned long long slots = 0;
+  for (int k = 0; k < K; k++) {
+    const MeshDesc &m = meshes[k];
+    if (m.n_blocks <= 0 || m.bpdx <= 0 || m.bpdy <= 0 || m.levels <= 0 || m.levels > 24 ||
+        m.extent <= 0 || !m.level || !m.bx || !m.by) {
+      bad_mesh = k;
+      err = "invalid mesh description";
+      return CUP2D_PS_EINVAL;
+    }
+    if (m.n_blocks * BLEN >= (1LL << 31) / 8) {
+      bad_mesh = k;
+      err = "mesh too large for 32-bit column indices";
+      return CUP2D_PS_EINVAL;
+    }
+    unsigned long long sk = 64;
+    while (sk < 2ull * (unsigned long long)m.n_blocks) sk <<= 1;
+    inst[k] = Inst{nb, slots, sk - 1, m.extent / (double(std::max(m.bpdx, m.bpdy)) * BS), m.levels,
+                   m.bpdx, m.bpdy, m.periodic_x, m.periodic_y};
+    nb += m.n_blocks;
+    slots += sk;
+  }
+  if (nb * BLEN >= (1LL << 31) - 64) {
+    err = "batch too large for 32-bit cell indices";
+    return CUP2D_PS_EINVAL;
+  }
   out.nb = nb;
   out.n = nb * BLEN;
   auto give = [&](auto **p, size_t count) {
@@ -528,40 +599,47 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
   };
   Temps T{temp};
 
-  // ---- blocks into the hash table ---------------------------------------------
-  int *d_lvl, *d_bx, *d_by;
+  // ---- blocks into the hash tables -------------------------------------------------
+  std::vector<int> owner(nb);
+  std::vector<long long> boff(K + 1);
+  for (int k = 0; k < K; k++) {
+    boff[k] = inst[k].boff;
+    std::fill(owner.begin() + inst[k].boff, owner.begin() + inst[k].boff + meshes[k].n_blocks, k);
+  }
+  boff[K] = nb;
+  int *d_lvl, *d_bx, *d_by, *d_inst, *d_status;
+  Inst *d_I;
+  long long *d_boff;
+  Hash H;
+  Errs *d_E;
+  Glob *d_g;
   ACHK(T.get(&d_lvl, nb));
   ACHK(T.get(&d_bx, nb));
   ACHK(T.get(&d_by, nb));
-  ACHK(hipMemcpyAsync(d_lvl, m.level, nb * sizeof(int), hipMemcpyHostToDevice, st));
-  ACHK(hipMemcpyAsync(d_bx, m.bx, nb * sizeof(int), hipMemcpyHostToDevice, st));
-  ACHK(hipMemcpyAsync(d_by, m.by, nb * sizeof(int), hipMemcpyHostToDevice, st));
-  unsigned long long slots = 64;
-  while (slots < 2ull * (unsigned long long)nb) slots <<= 1;
-  Hash H;
-  H.mask = slots - 1;
+  ACHK(T.get(&d_inst, nb));
+  ACHK(T.get(&d_status, nb));
+  ACHK(T.get(&d_I, K));
+  ACHK(T.get(&d_boff, K + 1));
   ACHK(T.get(&H.keys, slots));
   ACHK(T.get(&H.vals, slots));
-  Errs *d_e;
-  int *d_status;
-  ACHK(T.get(&d_e, 1));
-  ACHK(T.get(&d_status, nb));
-  const Blocks B{d_lvl, d_bx, d_by, nb, m.levels, m.bpdx, m.bpdy, m.periodic_x, m.periodic_y};
+  ACHK(T.get(&d_E, K));
+  ACHK(T.get(&d_g, 1));
+  for (int k = 0; k < K; k++) {
+    const size_t o = (size_t)inst[k].boff, c = (size_t)meshes[k].n_blocks * sizeof(int);
+    ACHK(hipMemcpyAsync(d_lvl + o, meshes[k].level, c, hipMemcpyHostToDevice, st));
+    ACHK(hipMemcpyAsync(d_bx + o, meshes[k].bx, c, hipMemcpyHostToDevice, st));
+    ACHK(hipMemcpyAsync(d_by + o, meshes[k].by, c, hipMemcpyHostToDevice, st));
+  }
+  ACHK(hipMemcpyAsync(d_inst, owner.data(), nb * sizeof(int), hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(d_I, inst.data(), K * sizeof(Inst), hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(d_boff, boff.data(), (K + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
+  const Blocks B{d_lvl, d_bx, d_by, d_inst, d_I, nb};
   const int gb = (int)((nb + AWG - 1) / AWG);
-  ka_init<<<(int)std::min<unsigned long long>(1024, slots / AWG + 1), AWG, 0, st>>>(H, d_e);
+  ka_init<<<(int)std::min<unsigned long long>(1024, slots / AWG + 1), AWG, 0, st>>>(H, slots, d_E, K, d_g);
   ka_insert<<<gb, AWG, 0, st>>>(B, H, d_status);
   ka_dups<<<gb, AWG, 0, st>>>(B, H, d_status);
-  ka_status<<<gb, AWG, 0, st>>>(nb, d_status, d_e);
+  ka_status<<<gb, AWG, 0, st>>>(B, d_status, d_E);
   ACHK(hipGetLastError());
-  Errs e;
-  ACHK(hipMemcpyAsync(&e, d_e, sizeof e, hipMemcpyDeviceToHost, st));
-  ACHK(hipStreamSynchronize(st));
-  const int eb = std::min({e.lvl, e.idx, e.dup});
-  if (eb != INT_MAX) {
-    err = "block " + std::to_string(eb) +
-          (eb == e.lvl ? ": level out of range" : eb == e.idx ? ": index out of range" : ": duplicate");
-    return CUP2D_PS_EINVAL;
-  }
 
   // ---- faces, explicit cells, h^2 -----------------------------------------------
   if (!give(&out.nbr, nb) || !give(&out.mask, nb) || !give(&out.h2, nb)) {
@@ -573,30 +651,36 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
   ACHK(T.get(&d_fid, 4 * nb));
   ACHK(T.get(&d_cnt, nb + 1));
   ACHK(T.get(&d_bptr, nb + 1));
-  const double h0 = m.extent / (double(std::max(m.bpdx, m.bpdy)) * BS);
-  ka_faces<<<gb, AWG, 0, st>>>(B, H, h0, out.nbr, d_fid, out.mask, d_cnt, out.h2, d_e);
+  ka_faces<<<gb, AWG, 0, st>>>(B, H, out.nbr, d_fid, out.mask, d_cnt, out.h2, d_E);
   ACHK(hipGetLastError());
   size_t tb = 0;
   char *d_tmp = nullptr;
   ACHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_cnt, d_bptr, (int)(nb + 1), st));
   ACHK(T.get(&d_tmp, tb));
   ACHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tb, d_cnt, d_bptr, (int)(nb + 1), st));
+  std::vector<Errs> E(K);
   int n_exc = 0;
-  ACHK(hipMemcpyAsync(&e, d_e, sizeof e, hipMemcpyDeviceToHost, st));
+  ACHK(hipMemcpyAsync(E.data(), d_E, K * sizeof(Errs), hipMemcpyDeviceToHost, st));
   ACHK(hipMemcpyAsync(&n_exc, d_bptr + nb, sizeof(int), hipMemcpyDeviceToHost, st));
   ACHK(hipStreamSynchronize(st));
-  if (e.face != INT_MAX) {
-    err = "block " + std::to_string(e.face / 4) + " face " + std::to_string(e.face % 4) +
-          ": neighbour is neither same-level, coarser nor finer (mesh not 2:1 balanced)";
-    return CUP2D_PS_EINVAL;
-  }
-  out.corner_block = e.corner;
-  out.mean_row = (mean_constraint && e.corner >= 0) ? (long long)e.corner * BLEN : -1;
+  for (int k = 0; k < K; k++)
+    if (std::min({E[k].blk[0], E[k].blk[1], E[k].blk[2], E[k].face}) != INT_MAX) {
+      bad_mesh = k;
+      err = mesh_error(E[k]);
+      return CUP2D_PS_EINVAL;
+    }
+  out.boff = boff;
+  out.mean_rows.assign(K, -1);
+  for (int k = 0; k < K; k++)
+    if (mean_constraint && E[k].corner >= 0) out.mean_rows[k] = (boff[k] + E[k].corner) * BLEN;
+  out.corner_block = E[0].corner;
+  out.mean_row = out.mean_rows[0];
   out.n_exc = n_exc;
-  out.nnz = (long long)e.nnz;
+  out.eoff.assign(K + 1, 0);
   if (n_exc == 0) {
     const double zero = 0.0;
     out.n_codes = 1;
+    for (const Errs &e : E) out.nnz += (long long)e.nnz;
     if (!give(&out.erow, 1) || !give(&out.ecol, 1) || !give(&out.ecode, 1) || !give(&out.etab, 1) ||
         !give(&out.ewid, 1)) {
       err = "device allocation failed";
@@ -608,25 +692,35 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
   }
 
   // ---- explicit rows -------------------------------------------------------------
-  unsigned *d_rlen, *d_rlen_s;
+  unsigned *d_rlen, *d_rkey, *d_rkey_s;
   int *d_rrow, *d_rcol, *d_iota, *d_order;
   double *d_rval;
+  long long *d_eoff;
+  uint8_t *d_elen;
   ACHK(T.get(&d_rlen, n_exc));
-  ACHK(T.get(&d_rlen_s, n_exc));
+  ACHK(T.get(&d_rkey, n_exc));
+  ACHK(T.get(&d_rkey_s, n_exc));
   ACHK(T.get(&d_rrow, n_exc));
   ACHK(T.get(&d_rcol, (size_t)n_exc * RCAP));
   ACHK(T.get(&d_rval, (size_t)n_exc * RCAP));
   ACHK(T.get(&d_iota, n_exc));
   ACHK(T.get(&d_order, n_exc));
+  ACHK(T.get(&d_eoff, K + 1));
+  ACHK(T.get(&d_elen, n_exc));
   ka_rows<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(B, out.nbr, d_fid, out.mask, d_bptr, n_exc, d_rlen,
-                                                 d_rrow, d_rcol, d_rval, d_e);
+                                                 d_rkey, d_rrow, d_rcol, d_rval, d_g);
   ka_iota<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(n_exc, d_iota);
+  ka_eoff<<<(K + AWG) / AWG, AWG, 0, st>>>(K, d_boff, d_bptr, d_eoff);
   ACHK(hipGetLastError());
-  // stable: rows of one length keep their mesh order (lengths < 32: 5 bits)
+  // stable: rows of one mesh and length keep their mesh order (length < 32: 5 bits)
+  int kbits = 0;
+  while ((1 << kbits) < K) kbits++;
   tb = 0;
-  ACHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_rlen, d_rlen_s, d_iota, d_order, n_exc, 0, 5, st));
+  ACHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_rkey, d_rkey_s, d_iota, d_order, n_exc, 0,
+                                          5 + kbits, st));
   ACHK(T.get(&d_tmp, tb));
-  ACHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, tb, d_rlen, d_rlen_s, d_iota, d_order, n_exc, 0, 5, st));
+  ACHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, tb, d_rkey, d_rkey_s, d_iota, d_order, n_exc, 0,
+                                          5 + kbits, st));
   // distinct coefficients by bit pattern, ascending
   const long long nk = (long long)n_exc * RCAP + 1;
   unsigned long long *d_keys, *d_skeys, *d_table;
@@ -645,13 +739,13 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
   ACHK(hipcub::DeviceSelect::Unique(nullptr, tb, d_skeys, d_table, d_ncodes, (int)nk, st));
   ACHK(T.get(&d_tmp, tb));
   ACHK(hipcub::DeviceSelect::Unique(d_tmp, tb, d_skeys, d_table, d_ncodes, (int)nk, st));
-  unsigned W = 0;
+  Glob g;
   int n_codes = 0;
-  ACHK(hipMemcpyAsync(&e, d_e, sizeof e, hipMemcpyDeviceToHost, st));
-  ACHK(hipMemcpyAsync(&W, d_rlen_s + n_exc - 1, sizeof W, hipMemcpyDeviceToHost, st));
+  ACHK(hipMemcpyAsync(&g, d_g, sizeof g, hipMemcpyDeviceToHost, st));
   ACHK(hipMemcpyAsync(&n_codes, d_ncodes, sizeof n_codes, hipMemcpyDeviceToHost, st));
+  ACHK(hipMemcpyAsync(out.eoff.data(), d_eoff, (K + 1) * sizeof(long long), hipMemcpyDeviceToHost, st));
   ACHK(hipStreamSynchronize(st));
-  if (e.wide != INT_MAX || W > (unsigned)ELL_MAX) {
+  if (g.wide != INT_MAX || g.maxlen > ELL_MAX) {
     err = "explicit row with more than " + std::to_string(ELL_MAX) + " entries";
     return CUP2D_PS_EINVAL;
   }
@@ -659,8 +753,8 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
     err = "more than 65536 distinct coefficients in the explicit rows";
     return CUP2D_PS_EINVAL;
   }
-  out.nnz = (long long)e.nnz;
-  out.ell_w = (int)W;
+  const int W = g.maxlen;
+  out.ell_w = W;
   out.n_codes = n_codes;
   if (!give(&out.erow, n_exc) || !give(&out.ecol, (size_t)W * n_exc) ||
       !give(&out.ecode, (size_t)W * n_exc) || !give(&out.etab, n_codes) ||
@@ -669,11 +763,16 @@ int assemble_on_device(const MeshDesc &m, int mean_constraint, hipStream_t st,
     return CUP2D_PS_ENOMEM;
   }
   ACHK(hipMemcpyAsync(out.etab, d_table, n_codes * sizeof(double), hipMemcpyDeviceToDevice, st));
-  ka_ell<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(n_exc, (int)W, d_order, d_rlen, d_rrow, d_rcol, d_rval,
+  ka_ell<<<(n_exc + AWG - 1) / AWG, AWG, 0, st>>>(n_exc, W, d_order, d_rlen, d_rrow, d_rcol, d_rval,
                                                   d_table, n_codes, out.erow, out.ecol, out.ecode,
-                                                  out.ewid);
+                                                  out.ewid, d_elen);
   ACHK(hipGetLastError());
+  out.elen.resize(n_exc);
+  ACHK(hipMemcpyAsync(out.elen.data(), d_elen, n_exc, hipMemcpyDeviceToHost, st));
   ACHK(hipStreamSynchronize(st));
+  // nonzeros: the plain rows' (counted per block) and the explicit rows' lengths
+  for (const Errs &e : E) out.nnz += (long long)e.nnz;
+  for (uint8_t l : out.elen) out.nnz += l;
   return CUP2D_PS_OK;
 }
 

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/cup2d_poisson.h"
+#include "assemble_device.h"
 #include "assembly.h"
 #include "kernels.h"
 
@@ -1295,7 +1296,11 @@ int cup2d_psb_destroy(cup2d_psb *h) {
   return CUP2D_PS_OK;
 }
 
-static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
+static int build_host(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes);
+static int build_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes);
+static int finish_meshes(cup2d_psb *h, int n_inst);
+
+static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes, bool on_device) {
   if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
   if (!meshes || n_inst <= 0 || n_inst > MAX_INST)
     return bfail(h, CUP2D_PS_EINVAL, "need 1.." + std::to_string(MAX_INST) + " meshes");
@@ -1305,6 +1310,12 @@ static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *mesh
   bfree_pool(h->unit_allocs);
   bfree_pool(h->allocs);
   h->ready = false;
+  const int rc = on_device ? build_device(h, n_inst, meshes) : build_host(h, n_inst, meshes);
+  return rc ? rc : finish_meshes(h, n_inst);
+}
+
+// The concatenated operator assembled on host threads.
+static int build_host(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
   // the instances' operators are independent: built on host threads, without
   // the plain CSR copy (the batch never reads it)
   std::vector<cup2d::Operator> ops(n_inst);
@@ -1393,16 +1404,71 @@ static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *mesh
   ops.clear();
   auto &pool = h->allocs;
   int rc;
-  const std::vector<double> P = make_pinv();
-  const std::vector<uint16_t> pf = make_pfrag(P);
   if ((rc = bupload(h, pool, &h->d_nbr, nbr)) || (rc = bupload(h, pool, &h->d_mask, mask)) ||
       (rc = bupload(h, pool, &h->d_erow, ell.erow)) ||
       (rc = bupload(h, pool, &h->d_ecol, ell_transpose(ell.ecol_rm, W))) ||
       (rc = bupload(h, pool, &h->d_ecode, ell_transpose(ell.ecode_rm, W))) ||
-      (rc = bupload(h, pool, &h->d_etab, ell.values)) ||
-      (rc = bupload(h, pool, &h->d_h2, h2)) ||
-      (rc = bupload(h, pool, &h->d_P, P)) ||
-      (rc = bupload(h, pool, &h->d_mean_row, h->h_mean_row)))
+      (rc = bupload(h, pool, &h->d_etab, ell.values)) || (rc = bupload(h, pool, &h->d_h2, h2)))
+    return rc;
+  return CUP2D_PS_OK;
+}
+
+// The same operator assembled by the device-side assembly, all meshes in one
+// pipeline (assemble_device.hip); only the meshes' block lists cross PCIe.
+static int build_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
+  std::vector<MeshDesc> md(n_inst);
+  for (int k = 0; k < n_inst; k++) {
+    const cup2d_mesh_desc &m = meshes[k];
+    md[k] = MeshDesc{m.bpdx, m.bpdy, m.levels, m.periodic_x, m.periodic_y, m.extent, m.n_blocks,
+                     m.level, m.bx, m.by};
+  }
+  auto &pool = h->allocs;
+  const DeviceAlloc alloc = [h, &pool](size_t bytes) -> void * {
+    uint8_t *p = nullptr;
+    return balloc(h, pool, &p, bytes) ? nullptr : p;
+  };
+  std::vector<void *> temps;  // the assembly's work arrays
+  const DeviceAlloc temp = [&temps](size_t bytes) -> void * {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    temps.push_back(p);
+    return p;
+  };
+  DeviceOperator D;
+  std::string err;
+  int bad = -1;
+  const int rc = assemble_on_device(md.data(), n_inst, h->cfg.mean_constraint, h->stream, alloc, temp,
+                                    D, bad, err);
+  (void)hipStreamSynchronize(h->stream);
+  for (void *p : temps) (void)hipFree(p);
+  if (rc) return bfail(h, rc, bad >= 0 ? "mesh " + std::to_string(bad) + ": " + err : err);
+  h->K = n_inst;
+  h->boff = D.boff;
+  h->nnz = D.nnz;
+  h->nb = D.nb;
+  h->n = D.n;
+  h->ell_w = D.ell_w;
+  h->eoff = D.eoff;
+  h->elen = std::move(D.elen);
+  h->h_mean_row = D.mean_rows;
+  h->d_nbr = D.nbr;
+  h->d_mask = D.mask;
+  h->d_h2 = D.h2;
+  h->d_erow = D.erow;
+  h->d_ecol = D.ecol;
+  h->d_ecode = D.ecode;
+  h->d_etab = D.etab;
+  return CUP2D_PS_OK;
+}
+
+// Preconditioner, work vectors and work units: the part of a mesh change
+// that does not depend on where the operator was assembled.
+static int finish_meshes(cup2d_psb *h, int n_inst) {
+  auto &pool = h->allocs;
+  int rc;
+  static const std::vector<double> P = make_pinv();  // the same for every mesh
+  static const std::vector<uint16_t> pf = make_pfrag(P);
+  if ((rc = bupload(h, pool, &h->d_P, P)) || (rc = bupload(h, pool, &h->d_mean_row, h->h_mean_row)))
     return rc;
   if ((rc = balloc(h, pool, &h->d_pfrag, pf.size() / 8))) return rc;
   BCHK(h, hipMemcpy(h->d_pfrag, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -1563,7 +1629,11 @@ int cup2d_psb_create(const cup2d_ps_config *cfg, cup2d_psb **out) {
 }
 
 int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
-  return bguarded(h, [&] { return set_meshes_body(h, n_inst, meshes); });
+  return bguarded(h, [&] { return set_meshes_body(h, n_inst, meshes, false); });
+}
+
+int cup2d_psb_set_meshes_on_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
+  return bguarded(h, [&] { return set_meshes_body(h, n_inst, meshes, true); });
 }
 
 int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter) {

@@ -1416,7 +1416,8 @@ static int set_mesh_on_device_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
     temps.push_back(b);
     return b.p;
   };
-  const int rc = assemble_on_device(md, h->cfg.mean_constraint, h->stream, alloc, temp, D, err);
+  int bad = -1;
+  const int rc = assemble_on_device(&md, 1, h->cfg.mean_constraint, h->stream, alloc, temp, D, bad, err);
   (void)hipStreamSynchronize(h->stream);
   h->spare.insert(h->spare.end(), temps.begin(), temps.end());
   if (rc) {

@@ -282,14 +282,19 @@ class BatchSolver:
         except Exception:
             pass
 
-    def set_meshes(self, meshes):
+    def set_meshes(self, meshes, on_device: bool = False):
+        """Assembles the instances' operators on host threads (default) or by
+        GPU kernels in one pipeline (on_device=True, the same operators bit
+        for bit)."""
         descs, keep = [], []
         for m in meshes:
             d, k = _mesh_desc(m)
             descs.append(d)
             keep.append(k)
         arr = (_lib.MeshDesc * len(descs))(*descs)
-        _check_b(self.h, _lib.load().cup2d_psb_set_meshes(self.h, len(descs), arr))
+        lib = _lib.load()
+        fn = lib.cup2d_psb_set_meshes_on_device if on_device else lib.cup2d_psb_set_meshes
+        _check_b(self.h, fn(self.h, len(descs), arr))
         off = np.empty(len(descs) + 1, np.int64)
         _check_b(self.h, _lib.load().cup2d_psb_offsets(self.h, off.ctypes.data))
         self.meshes = list(meshes)

@@ -212,6 +212,12 @@ int cup2d_psb_destroy(cup2d_psb *h);
  * handle's mean_constraint (per instance as cup2d_ps_set_mesh). */
 int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes);
 
+/* Same operators, assembled by GPU kernels for all instances in one pipeline
+ * (as cup2d_ps_set_mesh_on_device; reference ExpAMRSolver.cpp:100-313 per
+ * instance).  Bit-identical to cup2d_psb_set_meshes's; the same meshes are
+ * rejected with the same messages ("mesh k: ..."). */
+int cup2d_psb_set_meshes_on_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes);
+
 /* offsets[k] = first unknown of instance k, offsets[n_inst] = total (n_inst+1
  * entries). */
 int cup2d_psb_offsets(const cup2d_psb *h, long long *offsets);
