@@ -176,6 +176,24 @@ def bench_amr6(PS, W, steps, use_graph):
     return out
 
 
+def bench_batch_assembly(PS, meshes, reps=3):
+    """Mesh changes of the 64-instance batch: host assembly on threads +
+    upload against the device-side assembly of all instances (median of
+    `reps` after one warm-up each)."""
+    bs = PS.BatchSolver(PS.default_config(norm=PS.NORM_L2))
+    res = {}
+    for key, dev in (("host_set_meshes_ms", False), ("device_set_meshes_ms", True)):
+        bs.set_meshes(meshes, on_device=dev)
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            bs.set_meshes(meshes, on_device=dev)
+            t.append(time.perf_counter() - t0)
+        res[key] = float(np.median(t)) * 1e3
+    res["speedup"] = res["host_set_meshes_ms"] / res["device_set_meshes_ms"]
+    return res
+
+
 def bench_assembly(PS, mesh, reps=5):
     """A mesh change on the host (assembly + upload) against the device-side
     assembly, each the median of `reps` after one warm-up, and whether the two
@@ -238,6 +256,7 @@ def bench_batch(PS, W, steps, use_graph):
         if prec == PS.FP64:
             roof = batch_spmv_roofline(PS, meshes, reps=20)
         del bs, d_b, d_x
+    out["assembly"] = bench_batch_assembly(PS, meshes)
     # the same 64 solves one after another (single-solve FP64 solver)
     s = PS.Solver(norm=PS.NORM_L2, max_iter=4000, use_graph=use_graph)
     tot = 0.0
@@ -263,7 +282,8 @@ def kernel_only(PS, W, which, reps):
     if which == "assembly":
         # mesh changes on the ~16k-block 6-level mesh: host assembly + upload
         # against the device-side assembly
-        print(json.dumps({"kernel": "assembly", **bench_assembly(PS, W.four_disk_amr6_mesh(), reps)}))
+        print(json.dumps({"kernel": "assembly", **bench_assembly(PS, W.four_disk_amr6_mesh(), reps),
+                          "batch": bench_batch_assembly(PS, W.batch64_meshes())}))
         return
     if which == "copy":
         # the library's streaming copy over the batch's 13,881,344 unknowns
