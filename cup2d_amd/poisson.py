@@ -430,7 +430,9 @@ class HipPoissonSolver(PoissonSolver):
         restarts = 100 if sim.step < 10 else sim.maxPoissonRestarts
         if output.UpdateFluxCorrection or self._mesh_id is not id(output.mesh):
             output.UpdateFluxCorrection = False
-            self.solver.set_mesh(output.mesh)
+            # the operator is assembled by GPU kernels (the same matrix as the
+            # host assembly, bit for bit); only the block list crosses PCIe
+            self.solver.set_mesh(output.mesh, on_device=True)
             self._mesh_id = id(output.mesh)
         if input.data.shape != output.data.shape:
             raise ValueError("input and output grids differ")
