@@ -810,6 +810,11 @@ std::string g_berr;
 // host side
 // ----------------------------------------------------------------------------
 
+struct BBuf {
+  void *p;
+  size_t bytes;
+};
+
 struct cup2d_psb {
   cup2d_ps_config cfg{};
   std::string err;
@@ -850,8 +855,9 @@ struct cup2d_psb {
   std::vector<Scal> h_st;
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
-  std::vector<void *> allocs;       // mesh-dependent device memory
-  std::vector<void *> unit_allocs;  // work-unit lists and partials (depend on ib)
+  std::vector<BBuf> allocs;       // mesh-dependent device memory
+  std::vector<BBuf> unit_allocs;  // work-unit lists and partials (depend on ib)
+  std::vector<BBuf> spare;        // released buffers, reused by size (mesh changes, retuning)
 };
 
 namespace {
@@ -885,25 +891,63 @@ int bguarded(cup2d_psb *h, F &&body) {
   } while (0)
 
 template <class T>
-int balloc(cup2d_psb *h, std::vector<void *> &pool, T **p, size_t count) {
+int balloc(cup2d_psb *h, std::vector<BBuf> &pool, T **p, size_t count) {
   *p = nullptr;
-  if (count == 0) count = 1;
-  BCHK(h, hipMalloc((void **)p, count * sizeof(T)));
-  pool.push_back(*p);
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  // the smallest spare buffer that holds it and is not more than twice as large
+  size_t best = h->spare.size();
+  for (size_t i = 0; i < h->spare.size(); i++) {
+    const size_t b = h->spare[i].bytes;
+    if (b >= bytes && b <= 2 * bytes + 4096 && (best == h->spare.size() || b < h->spare[best].bytes))
+      best = i;
+  }
+  BBuf buf{nullptr, bytes};
+  if (best < h->spare.size()) {
+    buf = h->spare[best];
+    h->spare.erase(h->spare.begin() + best);
+  } else {
+    BCHK(h, hipMalloc(&buf.p, bytes));
+  }
+  pool.push_back(buf);
+  *p = static_cast<T *>(buf.p);
   return CUP2D_PS_OK;
 }
 
 template <class T>
-int bupload(cup2d_psb *h, std::vector<void *> &pool, T **p, const std::vector<T> &v) {
+int bupload(cup2d_psb *h, std::vector<BBuf> &pool, T **p, const std::vector<T> &v) {
   int rc = balloc(h, pool, p, v.size());
   if (rc) return rc;
   if (!v.empty()) BCHK(h, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return CUP2D_PS_OK;
 }
 
-void bfree_pool(std::vector<void *> &pool) {
-  for (void *p : pool) (void)hipFree(p);
+void bfree_pool(std::vector<BBuf> &pool) {
+  for (const BBuf &b : pool) (void)hipFree(b.p);
   pool.clear();
+}
+
+// Buffers of a pool back to the spare list (callers have synchronised the
+// stream, so they are handed out again in stream order).
+void brecycle(cup2d_psb *h, std::vector<BBuf> &pool) {
+  h->spare.insert(h->spare.end(), pool.begin(), pool.end());
+  pool.clear();
+}
+
+// Frees spare buffers, largest first, until they hold no more than the live
+// ones.
+void btrim(cup2d_psb *h) {
+  size_t live = 0, kept = 0;
+  for (const BBuf &b : h->allocs) live += b.bytes;
+  for (const BBuf &b : h->unit_allocs) live += b.bytes;
+  for (const BBuf &b : h->spare) kept += b.bytes;
+  std::sort(h->spare.begin(), h->spare.end(),
+            [](const BBuf &a, const BBuf &b) { return a.bytes > b.bytes; });
+  size_t i = 0;
+  for (; i < h->spare.size() && kept > live; i++) {
+    (void)hipFree(h->spare[i].p);
+    kept -= h->spare[i].bytes;
+  }
+  h->spare.erase(h->spare.begin(), h->spare.begin() + i);
 }
 
 void drop_graph(cup2d_psb *h) {
@@ -949,7 +993,8 @@ int grid_tiles(const cup2d_psb *h) {
 // rows follow its block items as items of up to WG rows.
 int build_units(cup2d_psb *h, int ib) {
   drop_graph(h);
-  bfree_pool(h->unit_allocs);
+  (void)hipStreamSynchronize(h->stream);
+  brecycle(h, h->unit_allocs);
   h->ib = ib;
   h->item_inst.clear();
   h->item_b0.clear();
@@ -1288,6 +1333,7 @@ int cup2d_psb_destroy(cup2d_psb *h) {
   drop_graph(h);
   bfree_pool(h->unit_allocs);
   bfree_pool(h->allocs);
+  bfree_pool(h->spare);
   if (h->h_all_done) (void)hipHostFree(h->h_all_done);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1307,8 +1353,8 @@ static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *mesh
   BCHK(h, hipSetDevice(h->cfg.device));
   BCHK(h, hipStreamSynchronize(h->stream));
   drop_graph(h);
-  bfree_pool(h->unit_allocs);
-  bfree_pool(h->allocs);
+  brecycle(h, h->unit_allocs);
+  brecycle(h, h->allocs);
   h->ready = false;
   const int rc = on_device ? build_device(h, n_inst, meshes) : build_host(h, n_inst, meshes);
   return rc ? rc : finish_meshes(h, n_inst);
@@ -1427,12 +1473,10 @@ static int build_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes)
     uint8_t *p = nullptr;
     return balloc(h, pool, &p, bytes) ? nullptr : p;
   };
-  std::vector<void *> temps;  // the assembly's work arrays
-  const DeviceAlloc temp = [&temps](size_t bytes) -> void * {
-    void *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    temps.push_back(p);
-    return p;
+  std::vector<BBuf> temps;  // the assembly's work arrays, spare again afterwards
+  const DeviceAlloc temp = [h, &temps](size_t bytes) -> void * {
+    uint8_t *p = nullptr;
+    return balloc(h, temps, &p, bytes) ? nullptr : p;
   };
   DeviceOperator D;
   std::string err;
@@ -1440,7 +1484,7 @@ static int build_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes)
   const int rc = assemble_on_device(md.data(), n_inst, h->cfg.mean_constraint, h->stream, alloc, temp,
                                     D, bad, err);
   (void)hipStreamSynchronize(h->stream);
-  for (void *p : temps) (void)hipFree(p);
+  brecycle(h, temps);
   if (rc) return bfail(h, rc, bad >= 0 ? "mesh " + std::to_string(bad) + ": " + err : err);
   h->K = n_inst;
   h->boff = D.boff;
@@ -1491,6 +1535,7 @@ static int finish_meshes(cup2d_psb *h, int n_inst) {
     ib = it != g_tuned.end() ? it->second : 8;
   }
   if ((rc = build_units(h, ib))) return rc;
+  btrim(h);
   h->ready = true;
   return CUP2D_PS_OK;
 }
@@ -1551,6 +1596,7 @@ static int autotune_body(cup2d_psb *h, const double *d_b, int *item_blocks, doub
   }
   int rc = build_units(h, best);
   if (rc) return rc;
+  btrim(h);
   g_tuned[{(int)h->mixed, h->K, h->nb}] = best;
   if (item_blocks) *item_blocks = best;
   if (ms_per_iter) *ms_per_iter = best_ms;

@@ -206,3 +206,27 @@ def test_batch_device_assembly_rejects_like_host():
     bh = PS.BatchSolver().set_meshes(good)
     x = np.random.default_rng(5).normal(size=bh.n)
     assert np.array_equal(bd.spmv(x), bh.spmv(x))
+
+
+def test_batch_mesh_changes_on_one_handle():
+    """Batches of changing size on one handle, alternating host and device
+    assembly, with a retuning in between (the handle reuses its device
+    buffers): the operators and solves of fresh handles."""
+    all_m = W.batch64_meshes()
+    sets = [all_m[:8], all_m[8:40], all_m[:3], all_m[40:]]
+    rng = np.random.default_rng(11)
+    bs = PS.BatchSolver(norm=PS.NORM_L2, max_iter=3000)
+    for k, ms in enumerate(sets):
+        bs.set_meshes(ms, on_device=bool(k % 2))
+        fresh = PS.BatchSolver(norm=PS.NORM_L2, max_iter=3000).set_meshes(ms)
+        x = rng.normal(size=fresh.n)
+        assert np.array_equal(bs.spmv(x), fresh.spmv(x)), k
+        b = np.concatenate([W.vortex_rhs(m, 60 + j) for j, m in enumerate(ms)])
+        xb, stb = bs.solve(b, rel_tol=1e-10)
+        xf, stf = fresh.solve(b, rel_tol=1e-10)
+        assert np.array_equal(xb, xf), k
+        if k == 1:
+            d_b = bs.device_array(b)
+            bs.autotune(d_b)
+            _, st = bs.solve(b, rel_tol=1e-10)
+            assert all(s["converged"] for s in st)
