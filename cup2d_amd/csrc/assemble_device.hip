@@ -195,75 +195,87 @@ __global__ __launch_bounds__(AWG) void ka_faces(Blocks B, Hash H, int4 *nbr, int
                                                 unsigned long long *mask, int *cnt, double *h2, Errs *E) {
   const long long b = blockIdx.x * (long long)AWG + threadIdx.x;
   if (b == 0) cnt[B.nb] = 0;
-  if (b >= B.nb) return;
-  const int k = B.inst[b];
-  const Errs ek = E[k];
-  if (ek.blk[0] != INT_MAX || ek.blk[1] != INT_MAX || ek.blk[2] != INT_MAX) {
-    cnt[b] = 0;
-    return;
-  }
-  const Inst in = B.I[k];
-  const int l = B.lvl[b];
-  const long long i = B.bx[b], j = B.by[b];
-  const long long nx = (long long)in.bpdx << l, ny = (long long)in.bpdy << l;
-  const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
-  int v[4];
-  int bad_face = INT_MAX;
-#pragma unroll
-  for (int f = 0; f < 4; f++) {
-    long long I = i + di[f], J = j + dj[f];
-    const bool outx = I < 0 || I >= nx, outy = J < 0 || J >= ny;
-    int2 fd = make_int2(-1, -1);
-    v[f] = -1;
-    if (!((outx && !in.px) || (outy && !in.py))) {
-      I = (I + nx) % nx;
-      J = (J + ny) % ny;
-      long long s;
-      if ((s = lookup(H, in, l, I, J)) >= 0) {
-        v[f] = (int)s;
-        fd.x = (int)s;
-      } else if ((s = lookup(H, in, l - 1, I >> 1, J >> 1)) >= 0) {
-        v[f] = -2;
-        fd.x = (int)s;
-      } else {
-        long long c[2];
-        for (int hh = 0; hh < 2; hh++) {
-          const long long ci = f < 2 ? 2 * I + (f == 0 ? 1 : 0) : 2 * I + hh;
-          const long long cj = f < 2 ? 2 * J + hh : 2 * J + (f == 2 ? 1 : 0);
-          c[hh] = lookup(H, in, l + 1, ci, cj);
-        }
-        if (c[0] < 0 || c[1] < 0) {
-          bad_face = min(bad_face, (int)(4 * (b - in.boff) + f));
-        } else {
-          v[f] = -2;
-          fd = make_int2((int)c[0], (int)c[1]);
-        }
-      }
-    }
-    fid[4 * b + f] = fd;
-  }
-  nbr[b] = make_int4(v[0], v[1], v[2], v[3]);
-  const unsigned long long C0 = 0x0101010101010101ull, R0 = 0xffull;
-  unsigned long long m = 0;
-  if (v[0] == -2) m |= C0;
-  if (v[1] == -2) m |= C0 << 7;
-  if (v[2] == -2) m |= R0;
-  if (v[3] == -2) m |= R0 << 56;
-  mask[b] = m;
-  cnt[b] = __popcll(m);
-  // a plain row: the cell and each face that is inside the block or same-level
+  int k = -1;  // the block's mesh, -1 when there is nothing to count
   long long plain = 0;
-  for (int c = 0; c < BLEN; c++) {
-    if ((m >> c) & 1) continue;
-    const int ix = c & 7, iy = c >> 3;
-    plain += 1 + (ix > 0 || v[0] >= 0) + (ix < BS - 1 || v[1] >= 0) + (iy > 0 || v[2] >= 0) +
-             (iy < BS - 1 || v[3] >= 0);
+  if (b < B.nb) {
+    const int kb = B.inst[b];
+    const Errs ek = E[kb];
+    if (ek.blk[0] != INT_MAX || ek.blk[1] != INT_MAX || ek.blk[2] != INT_MAX) {
+      cnt[b] = 0;
+    } else {
+      k = kb;
+      const Inst in = B.I[k];
+      const int l = B.lvl[b];
+      const long long i = B.bx[b], j = B.by[b];
+      const long long nx = (long long)in.bpdx << l, ny = (long long)in.bpdy << l;
+      const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+      int v[4];
+      int bad_face = INT_MAX;
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        long long I = i + di[f], J = j + dj[f];
+        const bool outx = I < 0 || I >= nx, outy = J < 0 || J >= ny;
+        int2 fd = make_int2(-1, -1);
+        v[f] = -1;
+        if (!((outx && !in.px) || (outy && !in.py))) {
+          I = (I + nx) % nx;
+          J = (J + ny) % ny;
+          long long s;
+          if ((s = lookup(H, in, l, I, J)) >= 0) {
+            v[f] = (int)s;
+            fd.x = (int)s;
+          } else if ((s = lookup(H, in, l - 1, I >> 1, J >> 1)) >= 0) {
+            v[f] = -2;
+            fd.x = (int)s;
+          } else {
+            long long c[2];
+            for (int hh = 0; hh < 2; hh++) {
+              const long long ci = f < 2 ? 2 * I + (f == 0 ? 1 : 0) : 2 * I + hh;
+              const long long cj = f < 2 ? 2 * J + hh : 2 * J + (f == 2 ? 1 : 0);
+              c[hh] = lookup(H, in, l + 1, ci, cj);
+            }
+            if (c[0] < 0 || c[1] < 0) {
+              bad_face = min(bad_face, (int)(4 * (b - in.boff) + f));
+            } else {
+              v[f] = -2;
+              fd = make_int2((int)c[0], (int)c[1]);
+            }
+          }
+        }
+        fid[4 * b + f] = fd;
+      }
+      nbr[b] = make_int4(v[0], v[1], v[2], v[3]);
+      const unsigned long long C0 = 0x0101010101010101ull, R0 = 0xffull;
+      unsigned long long m = 0;
+      if (v[0] == -2) m |= C0;
+      if (v[1] == -2) m |= C0 << 7;
+      if (v[2] == -2) m |= R0;
+      if (v[3] == -2) m |= R0 << 56;
+      mask[b] = m;
+      cnt[b] = __popcll(m);
+      // a plain row: the cell and each face that is inside the block or same-level
+      for (int c = 0; c < BLEN; c++) {
+        if ((m >> c) & 1) continue;
+        const int ix = c & 7, iy = c >> 3;
+        plain += 1 + (ix > 0 || v[0] >= 0) + (ix < BS - 1 || v[1] >= 0) + (iy > 0 || v[2] >= 0) +
+                 (iy < BS - 1 || v[3] >= 0);
+      }
+      const double h = in.h0 / double(1LL << l);
+      h2[b] = h * h;
+      if (bad_face != INT_MAX) atomicMin(&E[k].face, bad_face);
+      if (i == 0 && j == 0) atomicMax(&E[k].corner, (int)(b - in.boff));
+    }
   }
-  const double h = in.h0 / double(1LL << l);
-  h2[b] = h * h;
-  atomicAdd(&E[k].nnz, (unsigned long long)plain);
-  if (bad_face != INT_MAX) atomicMin(&E[k].face, bad_face);
-  if (i == 0 && j == 0) atomicMax(&E[k].corner, (int)(b - in.boff));
+  // the plain-row count: summed over the wavefront when all its blocks are
+  // of one mesh (the usual case: meshes are contiguous), else per lane --
+  // one atomic site either way
+  const int k0 = __shfl(k, 0);
+  const bool uni = __all(k == k0);
+  if (uni)
+    for (int o = 32; o > 0; o >>= 1) plain += __shfl_xor(plain, o);
+  const bool issue = uni ? ((threadIdx.x & 63) == 0 && k0 >= 0) : k >= 0;
+  const int kk = uni ? k0 : k;
+  if (issue) atomicAdd(&E[kk].nnz, (unsigned long long)plain);
 }
 
 // Coefficients of one explicit row, per column in order of first appearance.
