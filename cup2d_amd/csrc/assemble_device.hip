@@ -20,9 +20,8 @@
 //      row in registers (ghost interpolation ExpAMRSolver.cpp:100-128, D1/D2
 //      ExpAMRSolver.h:268-300) with the host's sequence of additions and
 //      stores it sorted by column;
-//   5. a stable radix sort orders the rows by (mesh, length); the
-//      coefficients' distinct bit patterns, collected in a hash set and
-//      sorted, are the coefficient table;
+//   5. a stable radix sort orders the rows by (mesh, length); a sort + unique
+//      of the coefficients' bit patterns gives the coefficient table;
 //   6. ka_ell writes the column-major ELL arrays, each coefficient's code (a
 //      binary search in the table) and the width of every run of 64 rows.
 // Floating-point contraction is off in this file: every coefficient is the
@@ -477,42 +476,16 @@ __global__ __launch_bounds__(AWG) void ka_iota(int n, int *a) {
   if (i < n) a[i] = i;
 }
 
-// The distinct coefficients, by bit pattern, into an open-addressing set of
-// VSLOTS slots (EMPTY = all ones, a NaN pattern no coefficient has): every
-// entry of every row, plus 0.0 (the padding's value).  A slot is read before
-// it is claimed, so once the few dozen values are in, the threads only read.
-constexpr int VSLOTS = 1 << 17;  // twice the most codes a 16-bit code can name
-__global__ __launch_bounds__(AWG) void ka_values(long long n_exc, const unsigned *__restrict__ rlen,
-                                                 const double *__restrict__ rval,
-                                                 unsigned long long *set, int *count) {
+// Bit patterns of every row slot (0.0 for the unused ones) plus one 0.0, so
+// that code 0 is 0.0.
+__global__ __launch_bounds__(AWG) void ka_keys(long long n_exc, const unsigned *__restrict__ rlen,
+                                               const double *__restrict__ rval,
+                                               unsigned long long *keys) {
   const long long q = blockIdx.x * (long long)AWG + threadIdx.x;
   if (q > n_exc * RCAP) return;
   const long long er = q / RCAP;
   const int w = (int)(q % RCAP);
-  unsigned long long key = 0;  // q == n_exc * RCAP: the 0.0 of the padding
-  if (er < n_exc) {
-    const unsigned len = rlen[er];
-    if (len > (unsigned)RCAP || w >= (int)len) return;  // over-wide rows are an error anyway
-    key = (unsigned long long)__double_as_longlong(rval[q]);
-  }
-  unsigned long long s = mix64(key) & (VSLOTS - 1);
-  for (int p = 0; p < VSLOTS; p++, s = (s + 1) & (VSLOTS - 1)) {
-    const unsigned long long cur = set[s];
-    if (cur == key) return;
-    if (cur == EMPTY) {
-      const unsigned long long prev = atomicCAS(&set[s], EMPTY, key);
-      if (prev == EMPTY) {
-        atomicAdd(count, 1);
-        return;
-      }
-      if (prev == key) return;
-    }
-  }
-}
-
-__global__ __launch_bounds__(AWG) void ka_fill(unsigned long long *a, int n, unsigned long long v) {
-  const int i = blockIdx.x * AWG + threadIdx.x;
-  if (i < n) a[i] = v;
+  keys[q] = (er < n_exc && w < (int)rlen[er]) ? (unsigned long long)__double_as_longlong(rval[q]) : 0ull;
 }
 
 // ELL slot s (rows in (mesh, length) order): row, columns, codes; the width
@@ -760,22 +733,24 @@ int assemble_on_device(const MeshDesc *meshes, int K, int mean_constraint, hipSt
   ACHK(T.get(&d_tmp, tb));
   ACHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, tb, d_rkey, d_rkey_s, d_iota, d_order, n_exc, 0,
                                           5 + kbits, st));
-  // distinct coefficients: a set of bit patterns, then sorted ascending (the
-  // EMPTY slots, all ones, sort after every finite coefficient)
+  // distinct coefficients by bit pattern, ascending
   const long long nk = (long long)n_exc * RCAP + 1;
-  unsigned long long *d_set, *d_table;
+  unsigned long long *d_keys, *d_skeys, *d_table;
   int *d_ncodes;
-  ACHK(T.get(&d_set, VSLOTS));
-  ACHK(T.get(&d_table, VSLOTS));
+  ACHK(T.get(&d_keys, nk));
+  ACHK(T.get(&d_skeys, nk));
+  ACHK(T.get(&d_table, nk));
   ACHK(T.get(&d_ncodes, 1));
-  ka_fill<<<VSLOTS / AWG, AWG, 0, st>>>(d_set, VSLOTS, EMPTY);
-  ACHK(hipMemsetAsync(d_ncodes, 0, sizeof(int), st));
-  ka_values<<<(int)((nk + AWG - 1) / AWG), AWG, 0, st>>>(n_exc, d_rlen, d_rval, d_set, d_ncodes);
+  ka_keys<<<(int)((nk + AWG - 1) / AWG), AWG, 0, st>>>(n_exc, d_rlen, d_rval, d_keys);
   ACHK(hipGetLastError());
   tb = 0;
-  ACHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_set, d_table, VSLOTS, 0, 64, st));
+  ACHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_keys, d_skeys, (int)nk, 0, 64, st));
   ACHK(T.get(&d_tmp, tb));
-  ACHK(hipcub::DeviceRadixSort::SortKeys(d_tmp, tb, d_set, d_table, VSLOTS, 0, 64, st));
+  ACHK(hipcub::DeviceRadixSort::SortKeys(d_tmp, tb, d_keys, d_skeys, (int)nk, 0, 64, st));
+  tb = 0;
+  ACHK(hipcub::DeviceSelect::Unique(nullptr, tb, d_skeys, d_table, d_ncodes, (int)nk, st));
+  ACHK(T.get(&d_tmp, tb));
+  ACHK(hipcub::DeviceSelect::Unique(d_tmp, tb, d_skeys, d_table, d_ncodes, (int)nk, st));
   Glob g;
   int n_codes = 0;
   ACHK(hipMemcpyAsync(&g, d_g, sizeof g, hipMemcpyDeviceToHost, st));
