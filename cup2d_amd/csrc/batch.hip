@@ -488,6 +488,26 @@ __global__ __launch_bounds__(WG) void kb_fwrite(BView V, const double *__restric
   }
 }
 
+// The pressure step around the solve, per instance (PressureSingle.cpp:842-879):
+// b = rhs - L p_old (y = L p_old from kb_spmv without the mean-row
+// replacement; the solve ignores b at a mean row), and after the solve
+// p = dp + (p_old - <dp>_h2).
+__global__ __launch_bounds__(WG) void kb_rhs_sub(long n, const double *__restrict__ rhs,
+                                                 double *__restrict__ b) {
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    b[i] = rhs[i] - b[i];
+}
+__global__ __launch_bounds__(WG) void kb_padd(BView V, const double *__restrict__ dp,
+                                              double *__restrict__ p, const Scal *st) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int4 mt = V.item[item];
+    const double avg = st[mt.x].mean;
+    long i0, i1;
+    item_cells(mt, i0, i1);
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) p[i] = dp[i] + (p[i] - avg);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // tile kernels (preconditioner; a wavefront per 16-block tile)
 // ----------------------------------------------------------------------------
@@ -1601,6 +1621,36 @@ static int autotune_body(cup2d_psb *h, const double *d_b, int *item_blocks, doub
   if (item_blocks) *item_blocks = best;
   if (ms_per_iter) *ms_per_iter = best_ms;
   return CUP2D_PS_OK;
+}
+
+static int pressure_step_b(cup2d_psb *h, const double *d_rhs, double *d_p, double tol_abs,
+                           double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!d_rhs || !d_p) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+  if (d_rhs == d_p) return bfail(h, CUP2D_PS_EINVAL, "rhs and p must not alias");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "pressure step before cup2d_psb_set_meshes");
+  if (!(tol_abs >= 0) || !(tol_rel >= 0) || max_restarts < 0)
+    return bfail(h, CUP2D_PS_EINVAL, "negative tolerance or restart count");
+  BCHK(h, hipSetDevice(h->cfg.device));
+  const BView V = bview(h);
+  const int Gi = grid_items(h);
+  kb_spmv<<<Gi, WG, 0, h->stream>>>(V, d_p, h->d_b, h->d_pm);  // L p_old (no mean-row fix)
+  kb_rhs_sub<<<Gi, WG, 0, h->stream>>>((long)h->n, d_rhs, h->d_b);
+  BCHK(h, hipGetLastError());
+  BCHK(h, hipMemsetAsync(h->d_xio, 0, (size_t)h->n * sizeof(double), h->stream));  // dp0 = 0
+  const int rc = solve_b(h, h->d_b, h->d_xio, h->d_xio, tol_abs, tol_rel, max_restarts, stats);
+  if (rc) return rc;
+  kb_fmean<<<Gi, WG, 0, h->stream>>>(V, h->d_xio, h->d_pm);
+  kb_mean<<<h->K, WG, 0, h->stream>>>(V, h->d_st, h->d_pm);
+  kb_padd<<<Gi, WG, 0, h->stream>>>(V, h->d_xio, d_p, h->d_st);
+  BCHK(h, hipGetLastError());
+  BCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
+int cup2d_psb_pressure_step_device(cup2d_psb *h, const double *d_rhs, double *d_p, double tol_abs,
+                                   double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
+  return bguarded(h, [&] { return pressure_step_b(h, d_rhs, d_p, tol_abs, tol_rel, max_restarts, stats); });
 }
 
 int cup2d_psb_spmv_device(cup2d_psb *h, const double *d_x, double *d_y) {

@@ -329,6 +329,15 @@ class BatchSolver:
                                                             float(rel_tol), int(max_restarts), st))
         return [s.as_dict() for s in st]
 
+    def pressure_step_device(self, d_rhs: DeviceArray, d_p: DeviceArray, tol=0.0, rel_tol=1e-10,
+                             max_restarts=100):
+        """The pressure step around the solve for every instance (as
+        Solver.pressure_step_device); d_p: p_old in, p out."""
+        st = (_lib.Stats * len(self.meshes))()
+        _check_b(self.h, _lib.load().cup2d_psb_pressure_step_device(
+            self.h, d_rhs.ptr, d_p.ptr, float(tol), float(rel_tol), int(max_restarts), st))
+        return [s.as_dict() for s in st]
+
     def autotune(self, d_b: DeviceArray):
         ib = ctypes.c_int()
         ms = ctypes.c_double()

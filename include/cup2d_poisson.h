@@ -218,6 +218,12 @@ int cup2d_psb_set_meshes(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes
  * rejected with the same messages ("mesh k: ..."). */
 int cup2d_psb_set_meshes_on_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes);
 
+/* The pressure step around the solve for every instance at once (as
+ * cup2d_ps_pressure_step_device, PressureSingle.cpp:842-879): device vectors
+ * of all instances concatenated; stats[n_inst] are the solves'. */
+int cup2d_psb_pressure_step_device(cup2d_psb *h, const double *d_rhs, double *d_p, double tol_abs,
+                                   double tol_rel, int max_restarts, cup2d_ps_stats *stats);
+
 /* offsets[k] = first unknown of instance k, offsets[n_inst] = total (n_inst+1
  * entries). */
 int cup2d_psb_offsets(const cup2d_psb *h, long long *offsets);

@@ -112,3 +112,44 @@ def test_pressure_step_errors():
         s.pressure_step_device(d_a, d_a)
     with pytest.raises(_lib.SolverError):
         s.pressure_step(np.zeros(m.n), np.zeros(m.n), rel_tol=-1.0)
+
+
+@pytest.mark.parametrize("mc", [0, 1])
+def test_batch_pressure_step_matches_oracle(oracle, mc):
+    """The batched pressure step, each instance against the oracle's pieces
+    (as the single-mesh test), instances of different sizes in one call."""
+    meshes = [MESHES[n]() for n in ("graded", "tiny_cf", "amr6_small", "periodic")]
+    bs = PS.BatchSolver(mean_constraint=mc, norm=PS.NORM_L2, max_iter=4000).set_meshes(meshes, on_device=True)
+    rhs = np.concatenate([W.vortex_rhs(m, 20 + k) for k, m in enumerate(meshes)])
+    p_old = np.concatenate([_p_old(m, 30 + k) for k, m in enumerate(meshes)])
+    d_rhs = bs.device_array(rhs)
+    d_p = bs.device_array(p_old)
+    st = bs.pressure_step_device(d_rhs, d_p, rel_tol=1e-12)
+    p = d_p.numpy()
+    assert np.array_equal(d_rhs.numpy(), rhs)
+    off = bs.offsets
+    for k, m in enumerate(meshes):
+        assert st[k]["converged"], (k, st[k])
+        sl = slice(off[k], off[k + 1])
+        h2 = oracle.h2(m)
+        L = oracle.assemble(m, 0)
+        b = rhs[sl] - oracle.spmv(L, p_old[sl])
+        A = oracle.assemble(m, mc)
+        if mc:
+            b[A.mean_row] = 0.0
+        dp, _ = oracle.bicgstab(A, b, mean_mode=mc, h2v=h2, rel_tol=1e-13, norm="l2", max_iter=4000)
+        dp = oracle.mean_remove(h2, dp)
+        inc = p[sl] - p_old[sl]
+        assert np.linalg.norm(inc - dp) <= 1e-8 * np.linalg.norm(dp), k
+        w = np.repeat(h2, 64)
+        assert abs(np.dot(w, inc)) <= 1e-12 * np.abs(inc).max() * w.sum(), k
+
+
+def test_batch_pressure_step_errors():
+    bs = PS.BatchSolver()
+    with pytest.raises(_lib.SolverError):  # no meshes yet
+        bs.pressure_step_device(PS.DeviceArray(bs.mem, 64), PS.DeviceArray(bs.mem, 64))
+    bs.set_meshes([_tiny_cf()])
+    d_a = bs.device_array(np.zeros(bs.n))
+    with pytest.raises(_lib.SolverError):  # rhs and p alias
+        bs.pressure_step_device(d_a, d_a)
