@@ -153,3 +153,17 @@ def test_batch_pressure_step_errors():
     d_a = bs.device_array(np.zeros(bs.n))
     with pytest.raises(_lib.SolverError):  # rhs and p alias
         bs.pressure_step_device(d_a, d_a)
+
+
+def test_pressure_step_mixed_precision(oracle):
+    """The mixed-precision handle: the same step to the same FP64 tolerance."""
+    m = W.graded_disk_mesh()
+    rhs = W.vortex_rhs(m, 12)
+    p_old = _p_old(m, 13)
+    s = PS.Solver(precision=PS.MIXED, norm=PS.NORM_L2, max_iter=6000).set_mesh(m, on_device=True)
+    p, st = s.pressure_step(rhs, p_old, rel_tol=1e-10)
+    assert st["converged"], st
+    L = oracle.assemble(m, 0)
+    b = rhs - oracle.spmv(L, p_old)
+    res = rhs - oracle.spmv(L, p)
+    assert np.linalg.norm(res) <= 1.05e-10 * np.linalg.norm(b)
