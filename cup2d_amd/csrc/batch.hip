@@ -32,6 +32,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -822,7 +823,7 @@ __global__ __launch_bounds__(WG) void kb_mean(BView V, Scal *st, const double *p
   if (threadIdx.x == 0) st[k].mean = o[0] / o[1];
 }
 
-std::string g_berr;
+thread_local std::string g_berr;  // failures without a handle, per calling thread
 
 }  // namespace
 
@@ -1302,7 +1303,9 @@ int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, 
 }
 
 // Tuned item sizes of this process, by (precision, instances, blocks).
+// Handles on different threads may tune and read it at once: g_tuned_mu.
 std::map<std::tuple<int, int, long long>, int> g_tuned;
+std::mutex g_tuned_mu;
 
 bool config_ok(const cup2d_ps_config &c) {
   return c.max_iter > 0 && c.chunk >= 0 && c.eps >= 0 && c.breakdown_tol >= 0 &&
@@ -1551,6 +1554,7 @@ static int finish_meshes(cup2d_psb *h, int n_inst) {
   h->h_st.assign(n_inst, Scal{});
   int ib = h->cfg.reserved_i[1];
   if (ib <= 0) {
+    std::lock_guard<std::mutex> lock(g_tuned_mu);
     auto it = g_tuned.find({(int)h->mixed, n_inst, h->nb});
     ib = it != g_tuned.end() ? it->second : 8;
   }
@@ -1617,7 +1621,10 @@ static int autotune_body(cup2d_psb *h, const double *d_b, int *item_blocks, doub
   int rc = build_units(h, best);
   if (rc) return rc;
   btrim(h);
-  g_tuned[{(int)h->mixed, h->K, h->nb}] = best;
+  {
+    std::lock_guard<std::mutex> lock(g_tuned_mu);
+    g_tuned[{(int)h->mixed, h->K, h->nb}] = best;
+  }
   if (item_blocks) *item_blocks = best;
   if (ms_per_iter) *ms_per_iter = best_ms;
   return CUP2D_PS_OK;

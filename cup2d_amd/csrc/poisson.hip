@@ -825,7 +825,7 @@ __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restric
 // host side
 // ----------------------------------------------------------------------------
 
-std::string g_create_err;
+thread_local std::string g_create_err;  // failures without a handle, per calling thread
 
 }  // namespace
 

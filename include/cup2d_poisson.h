@@ -188,8 +188,9 @@ int cup2d_ps_dfree(cup2d_ps *h, void *ptr);
 int cup2d_ps_h2d(cup2d_ps *h, void *dst, const void *src, size_t bytes);
 int cup2d_ps_d2h(cup2d_ps *h, void *dst, const void *src, size_t bytes);
 
-/* Message of the last failure on this handle (or of cup2d_ps_create when
- * h is NULL). */
+/* Message of the last failure on this handle (or, when h is NULL, of the
+ * calling thread's last failure without a handle: cup2d_ps_create,
+ * cup2d_op_csr). */
 const char *cup2d_ps_last_error(const cup2d_ps *h);
 
 /* ------------------------------------------------------------------------
@@ -263,6 +264,7 @@ int cup2d_psb_copy_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
 /* Totals over the instances: unknowns, nonzeros, explicit (coarse-fine) rows. */
 int cup2d_psb_sizes(const cup2d_psb *h, long long *n, long long *nnz, long long *n_explicit);
 
+/* As cup2d_ps_last_error, for batch handles and cup2d_psb_create. */
 const char *cup2d_psb_last_error(const cup2d_psb *h);
 
 #ifdef __cplusplus
