@@ -49,6 +49,8 @@ SIGNATURES = {
     "cup2d_ps_solve": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_ps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_ps_pressure_step_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
+    "cup2d_ps_pressure_rhs_device": (_I, [_P, _P, _P, _P, _P, _P, _D, _P]),
+    "cup2d_ps_pressure_correct_device": (_I, [_P, _P, _D, _P, _P]),
     "cup2d_ps_spmv_device": (_I, [_P, _P, _P, _I]),
     "cup2d_ps_spmv_time": (_I, [_P, _P, _P, _I, _I, _P]),
     "cup2d_ps_kernel_time": (_I, [_P, _I, _I, _P]),

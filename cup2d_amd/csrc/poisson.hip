@@ -704,6 +704,70 @@ __global__ __launch_bounds__(WG) void k_pressure_add(int nb, double *__restrict_
     p[i] = dp[i] + (p[i] - avg);
 }
 
+// Values of a cell's four neighbours (E, W, N, S) in a mesh without
+// coarse-fine faces: inside the block, in the same-level neighbour block, or
+// the ghost of a domain face -- `wall_sign` times the cell's own value (the
+// reference's labs mirror the boundary cell: -1 for the face-normal velocity
+// component, Definitions.h:84-117; +1 for the zero-gradient pressure).
+struct Nb4 {
+  double e, w, n, s;
+};
+__device__ __forceinline__ Nb4 neighbours(const double *__restrict__ f, long i, int4 nb, double xs,
+                                          double ys) {
+  const int c = (int)(i & 63), ix = c & 7, iy = c >> 3;
+  const double self = f[i];
+  Nb4 r;
+  r.e = ix < 7 ? f[i + 1] : nb.y >= 0 ? f[(long)nb.y * 64 + iy * 8] : xs * self;
+  r.w = ix > 0 ? f[i - 1] : nb.x >= 0 ? f[(long)nb.x * 64 + iy * 8 + 7] : xs * self;
+  r.n = iy < 7 ? f[i + 8] : nb.w >= 0 ? f[(long)nb.w * 64 + ix] : ys * self;
+  r.s = iy > 0 ? f[i - 8] : nb.z >= 0 ? f[(long)nb.z * 64 + 56 + ix] : ys * self;
+  return r;
+}
+
+// rhs = h/(2 dt) [(u_E - u_W) + (v_N - v_S)] - h/(2 dt) chi [(ud_E - ud_W) + (vd_N - vd_S)]
+// (updatePressureRHS, PressureSingle.cpp:336-349), in the reference's order of
+// operations (no contraction).
+__global__ __launch_bounds__(WG) void k_pressure_rhs(int nb, const int4 *__restrict__ nbr,
+                                                     const double *__restrict__ hb,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ v,
+                                                     const double *__restrict__ ud,
+                                                     const double *__restrict__ vd,
+                                                     const double *__restrict__ chi, double dt,
+                                                     double *__restrict__ rhs) {
+#pragma clang fp contract(off)
+  const long n = (long)nb * 64;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const int4 f = nbr[i >> 6];
+    const double facDiv = 0.5 * hb[i >> 6] / dt;
+    const Nb4 U = neighbours(u, i, f, -1.0, 1.0), V = neighbours(v, i, f, 1.0, -1.0);
+    const Nb4 UD = neighbours(ud, i, f, -1.0, 1.0), VD = neighbours(vd, i, f, 1.0, -1.0);
+    double t = facDiv * ((U.e - U.w) + (V.n - V.s));
+    t += -facDiv * chi[i] * ((UD.e - UD.w) + (VD.n - VD.s));
+    rhs[i] = t;
+  }
+}
+
+// u += (-dt/2 h (p_E - p_W)) / h^2, v likewise (pressureCorrectionKernel and
+// PressureSingle::pressureCorrection, PressureSingle.cpp:122-210), zero-gradient
+// pressure ghosts, in the reference's order of operations (no contraction).
+__global__ __launch_bounds__(WG) void k_pressure_correct(int nb, const int4 *__restrict__ nbr,
+                                                         const double *__restrict__ hb,
+                                                         const double *__restrict__ p, double dt,
+                                                         double *__restrict__ u,
+                                                         double *__restrict__ v) {
+#pragma clang fp contract(off)
+  const long n = (long)nb * 64;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double h = hb[i >> 6];
+    const double pFac = -0.5 * dt * h, ih2 = 1.0 / h / h;
+    const Nb4 P = neighbours(p, i, nbr[i >> 6], 1.0, 1.0);
+    const double tu = pFac * (P.e - P.w), tv = pFac * (P.n - P.s);
+    u[i] += tu * ih2;
+    v[i] += tv * ih2;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // operator pieces exposed by the C ABI
 // ----------------------------------------------------------------------------
@@ -850,6 +914,7 @@ struct cup2d_ps {
   uint16_t *d_ecode = nullptr;
   uint8_t *d_ewid = nullptr;
   double *d_etab = nullptr, *d_h2 = nullptr;
+  double *d_h = nullptr;                 // cell size per block
   int ell_w = 0;
   long long n_exc = 0;                   // explicit rows
   bool has_csr = false;                  // plain CSR copy present (host-assembled meshes)
@@ -1317,6 +1382,15 @@ int cup2d_ps_destroy(cup2d_ps *h) {
 
 static int finish_mesh(cup2d_ps *h);
 
+// Cell size per block, h0 / 2^level (the BlockInfo::h of the reference's
+// operators), for the kernels either side of the solve.
+static int upload_h(cup2d_ps *h, const cup2d_mesh_desc *m) {
+  const double h0 = m->extent / (double(std::max(m->bpdx, m->bpdy)) * BS);
+  std::vector<double> hb((size_t)m->n_blocks);
+  for (long long b = 0; b < m->n_blocks; b++) hb[b] = h0 / double(1LL << m->level[b]);
+  return upload(h, &h->d_h, hb);
+}
+
 static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   if (!h || !m) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   HIPCHK(h, hipSetDevice(h->device));
@@ -1353,6 +1427,7 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
     return rc;
   h->n_exc = (long long)n_exc;
   h->has_csr = true;
+  if ((rc = upload_h(h, m))) return rc;
   return finish_mesh(h);
 }
 
@@ -1441,6 +1516,7 @@ static int set_mesh_on_device_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   h->ell_w = D.ell_w;
   h->n_exc = D.n_exc;
   h->has_csr = false;
+  if (const int hr = upload_h(h, m)) return hr;
   h->d_csr_rp = h->d_csr_col = nullptr;
   h->d_csr_val = nullptr;
   return finish_mesh(h);
@@ -1490,6 +1566,51 @@ static int pressure_step_body(cup2d_ps *h, const double *d_rhs, double *d_p, dou
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
+}
+
+static int projection_checks(cup2d_ps *h, double dt) {
+  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+  if (h->n_exc > 0)
+    return fail(h, CUP2D_PS_EINVAL,
+                "mesh has coarse-fine faces: their ghost values come from the Cubism lab's "
+                "interpolation, which this operator does not restate");
+  if (!(dt > 0)) return fail(h, CUP2D_PS_EINVAL, "dt must be positive");
+  return CUP2D_PS_OK;
+}
+
+static int pressure_rhs_body(cup2d_ps *h, const double *d_u, const double *d_v, const double *d_ud,
+                             const double *d_vd, const double *d_chi, double dt, double *d_rhs) {
+  if (!h || !d_u || !d_v || !d_ud || !d_vd || !d_chi || !d_rhs)
+    return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  if (const int rc = projection_checks(h, dt)) return rc;
+  HIPCHK(h, hipSetDevice(h->device));
+  k_pressure_rhs<<<h->G, WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_u, d_v, d_ud, d_vd,
+                                             d_chi, dt, d_rhs);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
+static int pressure_correct_body(cup2d_ps *h, const double *d_p, double dt, double *d_u, double *d_v) {
+  if (!h || !d_p || !d_u || !d_v) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  if (d_u == d_v || d_p == d_u || d_p == d_v) return fail(h, CUP2D_PS_EINVAL, "p, u and v must not alias");
+  if (const int rc = projection_checks(h, dt)) return rc;
+  HIPCHK(h, hipSetDevice(h->device));
+  k_pressure_correct<<<h->G, WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_p, dt, d_u, d_v);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CUP2D_PS_OK;
+}
+
+int cup2d_ps_pressure_rhs_device(cup2d_ps *h, const double *d_u, const double *d_v,
+                                 const double *d_udef_u, const double *d_udef_v, const double *d_chi,
+                                 double dt, double *d_rhs) {
+  return guarded(h, [&] { return pressure_rhs_body(h, d_u, d_v, d_udef_u, d_udef_v, d_chi, dt, d_rhs); });
+}
+
+int cup2d_ps_pressure_correct_device(cup2d_ps *h, const double *d_p, double dt, double *d_u,
+                                     double *d_v) {
+  return guarded(h, [&] { return pressure_correct_body(h, d_p, dt, d_u, d_v); });
 }
 
 int cup2d_ps_spmv_device(cup2d_ps *h, const double *d_x, double *d_y, int kind) {

@@ -195,6 +195,26 @@ class Solver:
             self.h, d_rhs.ptr, d_p.ptr, float(tol), float(rel_tol), int(max_restarts), ctypes.byref(st)))
         return st.as_dict()
 
+    def pressure_rhs(self, u, v, udef_u, udef_v, chi, dt):
+        """The divergence RHS of the pressure equation (PressureSingle.cpp:324-349)
+        for host arrays; meshes without coarse-fine faces."""
+        d = [DeviceArray.from_host(self, np.ascontiguousarray(a, dtype=np.float64))
+             for a in (u, v, udef_u, udef_v, chi)]
+        d_rhs = DeviceArray(self, d[0].n)
+        _check(self.h, _lib.load().cup2d_ps_pressure_rhs_device(self.h, *[a.ptr for a in d], float(dt),
+                                                                d_rhs.ptr))
+        return d_rhs.numpy()
+
+    def pressure_correct(self, p, u, v, dt):
+        """The velocity correction (PressureSingle.cpp:122-210) for host arrays;
+        returns the corrected (u, v)."""
+        d_p = DeviceArray.from_host(self, np.ascontiguousarray(p, dtype=np.float64))
+        d_u = DeviceArray.from_host(self, np.ascontiguousarray(u, dtype=np.float64))
+        d_v = DeviceArray.from_host(self, np.ascontiguousarray(v, dtype=np.float64))
+        _check(self.h, _lib.load().cup2d_ps_pressure_correct_device(self.h, d_p.ptr, float(dt), d_u.ptr,
+                                                                    d_v.ptr))
+        return d_u.numpy(), d_v.numpy()
+
     def pressure_step(self, rhs, p_old, tol=0.0, rel_tol=1e-10, max_restarts=100):
         """Host arrays in, (p, stats) out."""
         d_rhs = DeviceArray.from_host(self, np.ascontiguousarray(rhs, dtype=np.float64))

@@ -133,6 +133,27 @@ int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double to
 int cup2d_ps_pressure_step_device(cup2d_ps *h, const double *d_rhs, double *d_p, double tol_abs,
                                   double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
+/* The two operators either side of the pressure step on device arrays, for
+ * meshes without coarse-fine faces (CUP2D_PS_EINVAL otherwise: there the
+ * reference's values come from the Cubism lab's interpolation and flux
+ * correction).  Ghost values at non-periodic domain faces mirror the boundary
+ * cell: the face-normal velocity component negated (Definitions.h:84-117, both
+ * wall and free-slip), the pressure copied (zero gradient).
+ *
+ * Divergence RHS (updatePressureRHS, PressureSingle.cpp:324-349):
+ *   rhs = h/(2 dt) [(u_E - u_W) + (v_N - v_S)] - h/(2 dt) chi [(ud_E - ud_W) + (vd_N - vd_S)]
+ * d_u, d_v: velocity components; d_udef_u, d_udef_v: the obstacles' deformation
+ * velocity; d_chi: the characteristic function; all n doubles, block-major. */
+int cup2d_ps_pressure_rhs_device(cup2d_ps *h, const double *d_u, const double *d_v,
+                                 const double *d_udef_u, const double *d_udef_v, const double *d_chi,
+                                 double dt, double *d_rhs);
+
+/* Velocity correction (pressureCorrectionKernel + pressureCorrection,
+ * PressureSingle.cpp:122-210): u += (-dt/2 h (p_E - p_W)) / h^2, v likewise
+ * with p_N - p_S.  d_p is not modified. */
+int cup2d_ps_pressure_correct_device(cup2d_ps *h, const double *d_p, double dt, double *d_u,
+                                     double *d_v);
+
 /* y = A x on the device (the operator of the last cup2d_ps_set_mesh, with the
  * mean-constraint row if enabled).  kind 0: stencil kernel used by the solver,
  * kind 1: plain CSR kernel.  Reference: BiCGSTABSolver::hd_cusparseSpMV
