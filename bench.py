@@ -279,6 +279,27 @@ def bench_batch(PS, W, steps, use_graph):
 def kernel_only(PS, W, which, reps):
     """Launches one kernel `reps` times (for rocprofv3 counter runs) and
     prints its mean time and algorithmic bytes."""
+    if which == "projection":
+        # the operators either side of the pressure solve on a uniform
+        # 128x128-block mesh (1,048,576 cells, walls): algorithmic bytes per
+        # cell 48 (rhs: u, v, udef_u, udef_v, chi in, rhs out) and 40
+        # (correction: p, u, v in, u, v out)
+        from cup2d_amd import mesh as M
+        m = M.uniform(128, 128, 1.0)
+        s = PS.Solver().set_mesh(m, on_device=True)
+        rng = np.random.default_rng(0)
+        d = [PS.DeviceArray.from_host(s, rng.normal(size=m.n)) for _ in range(8)]
+        out = {}
+        for name, fn, byt in (("pressure_rhs", lambda: s.pressure_rhs_device(d[0], d[1], d[2], d[3], d[4], 0.01, d[5]), 48),
+                              ("pressure_correct", lambda: s.pressure_correct_device(d[6], 0.01, d[0], d[1]), 40)):
+            fn()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+            out[name] = {"ms_per_call": ms, "GB_per_s": byt * m.n / (ms * 1e-3) / 1e9}
+        print(json.dumps({"kernel": "projection", "cells": m.n, **out}))
+        return
     if which == "assembly":
         # mesh changes on the ~16k-block 6-level mesh: host assembly + upload
         # against the device-side assembly
@@ -354,7 +375,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "iter0", "iter1",
+    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection", "iter0", "iter1",
                                               "iter2", "iter3", "iter4"], default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
                          "(spmv: operator on the bench mesh; batch_spmv: batched operator on the "

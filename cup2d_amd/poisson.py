@@ -195,6 +195,14 @@ class Solver:
             self.h, d_rhs.ptr, d_p.ptr, float(tol), float(rel_tol), int(max_restarts), ctypes.byref(st)))
         return st.as_dict()
 
+    def pressure_rhs_device(self, d_u, d_v, d_ud, d_vd, d_chi, dt, d_rhs):
+        _check(self.h, _lib.load().cup2d_ps_pressure_rhs_device(self.h, d_u.ptr, d_v.ptr, d_ud.ptr, d_vd.ptr,
+                                                                d_chi.ptr, float(dt), d_rhs.ptr))
+
+    def pressure_correct_device(self, d_p, dt, d_u, d_v):
+        _check(self.h, _lib.load().cup2d_ps_pressure_correct_device(self.h, d_p.ptr, float(dt), d_u.ptr,
+                                                                    d_v.ptr))
+
     def pressure_rhs(self, u, v, udef_u, udef_v, chi, dt):
         """The divergence RHS of the pressure equation (PressureSingle.cpp:324-349)
         for host arrays; meshes without coarse-fine faces."""
