@@ -1568,6 +1568,12 @@ static int pressure_step_body(cup2d_ps *h, const double *d_rhs, double *d_p, dou
   return CUP2D_PS_OK;
 }
 
+// One thread per cell (no partials to re-reduce, so the grid is not capped
+// like the solver's), up to 16384 workgroups.
+static int cell_grid(const cup2d_ps *h) {
+  return (int)std::min<long long>((h->op.n + WG - 1) / WG, 16384);
+}
+
 static int projection_checks(cup2d_ps *h, double dt) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   if (h->n_exc > 0)
@@ -1584,8 +1590,8 @@ static int pressure_rhs_body(cup2d_ps *h, const double *d_u, const double *d_v, 
     return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (const int rc = projection_checks(h, dt)) return rc;
   HIPCHK(h, hipSetDevice(h->device));
-  k_pressure_rhs<<<h->G, WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_u, d_v, d_ud, d_vd,
-                                             d_chi, dt, d_rhs);
+  k_pressure_rhs<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_u, d_v, d_ud,
+                                                    d_vd, d_chi, dt, d_rhs);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
@@ -1596,7 +1602,8 @@ static int pressure_correct_body(cup2d_ps *h, const double *d_p, double dt, doub
   if (d_u == d_v || d_p == d_u || d_p == d_v) return fail(h, CUP2D_PS_EINVAL, "p, u and v must not alias");
   if (const int rc = projection_checks(h, dt)) return rc;
   HIPCHK(h, hipSetDevice(h->device));
-  k_pressure_correct<<<h->G, WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_p, dt, d_u, d_v);
+  k_pressure_correct<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_p, dt, d_u,
+                                                        d_v);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
