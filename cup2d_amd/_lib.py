@@ -70,6 +70,8 @@ SIGNATURES = {
     "cup2d_psb_set_meshes": (_I, [_P, _I, _P]),
     "cup2d_psb_set_meshes_on_device": (_I, [_P, _I, _P]),
     "cup2d_psb_pressure_step_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
+    "cup2d_psb_pressure_rhs_device": (_I, [_P, _P, _P, _P, _P, _P, _D, _P]),
+    "cup2d_psb_pressure_correct_device": (_I, [_P, _P, _D, _P, _P]),
     "cup2d_psb_offsets": (_I, [_P, _P]),
     "cup2d_psb_solve": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_psb_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),

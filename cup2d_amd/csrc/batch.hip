@@ -861,6 +861,7 @@ struct cup2d_psb {
   int ell_w = 0;
   bf16x8 *d_pfrag = nullptr;
   long long *d_mean_row = nullptr;
+  double *d_h = nullptr;            // [nb] cell size per block
   // device: work units
   int4 *d_item = nullptr;
   int *d_item_begin = nullptr;
@@ -1379,8 +1380,18 @@ static int set_meshes_body(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *mesh
   brecycle(h, h->unit_allocs);
   brecycle(h, h->allocs);
   h->ready = false;
-  const int rc = on_device ? build_device(h, n_inst, meshes) : build_host(h, n_inst, meshes);
-  return rc ? rc : finish_meshes(h, n_inst);
+  int rc = on_device ? build_device(h, n_inst, meshes) : build_host(h, n_inst, meshes);
+  if (rc) return rc;
+  // cell size per block, h0 / 2^level (BlockInfo::h), for the projection operators
+  std::vector<double> hb;
+  hb.reserve((size_t)h->nb);
+  for (int k = 0; k < n_inst; k++) {
+    const cup2d_mesh_desc &m = meshes[k];
+    const double h0 = m.extent / (double(std::max(m.bpdx, m.bpdy)) * BS);
+    for (long long b = 0; b < m.n_blocks; b++) hb.push_back(h0 / double(1LL << m.level[b]));
+  }
+  if ((rc = bupload(h, h->allocs, &h->d_h, hb))) return rc;
+  return finish_meshes(h, n_inst);
 }
 
 // The concatenated operator assembled on host threads.
@@ -1658,6 +1669,53 @@ static int pressure_step_b(cup2d_psb *h, const double *d_rhs, double *d_p, doubl
 int cup2d_psb_pressure_step_device(cup2d_psb *h, const double *d_rhs, double *d_p, double tol_abs,
                                    double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
   return bguarded(h, [&] { return pressure_step_b(h, d_rhs, d_p, tol_abs, tol_rel, max_restarts, stats); });
+}
+
+static int projection_checks_b(cup2d_psb *h, double dt) {
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "no meshes");
+  if (h->eoff[h->K] > 0)
+    return bfail(h, CUP2D_PS_EINVAL,
+                 "a mesh has coarse-fine faces: their ghost values come from the Cubism lab's "
+                 "interpolation, which this operator does not restate");
+  if (!(dt > 0)) return bfail(h, CUP2D_PS_EINVAL, "dt must be positive");
+  return CUP2D_PS_OK;
+}
+
+static int cell_grid_b(const cup2d_psb *h) {
+  return (int)std::min<long long>((h->n + WG - 1) / WG, 16384);
+}
+
+int cup2d_psb_pressure_rhs_device(cup2d_psb *h, const double *d_u, const double *d_v,
+                                  const double *d_udef_u, const double *d_udef_v,
+                                  const double *d_chi, double dt, double *d_rhs) {
+  return bguarded(h, [&]() -> int {
+    if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+    if (!d_u || !d_v || !d_udef_u || !d_udef_v || !d_chi || !d_rhs)
+      return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+    if (const int rc = projection_checks_b(h, dt)) return rc;
+    BCHK(h, hipSetDevice(h->cfg.device));
+    k_pressure_rhs<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_h, d_u, d_v,
+                                                         d_udef_u, d_udef_v, d_chi, dt, d_rhs);
+    BCHK(h, hipGetLastError());
+    BCHK(h, hipStreamSynchronize(h->stream));
+    return CUP2D_PS_OK;
+  });
+}
+
+int cup2d_psb_pressure_correct_device(cup2d_psb *h, const double *d_p, double dt, double *d_u,
+                                      double *d_v) {
+  return bguarded(h, [&]() -> int {
+    if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+    if (!d_p || !d_u || !d_v) return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+    if (d_u == d_v || d_p == d_u || d_p == d_v) return bfail(h, CUP2D_PS_EINVAL, "p, u and v must not alias");
+    if (const int rc = projection_checks_b(h, dt)) return rc;
+    BCHK(h, hipSetDevice(h->cfg.device));
+    k_pressure_correct<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_h, d_p, dt, d_u,
+                                                             d_v);
+    BCHK(h, hipGetLastError());
+    BCHK(h, hipStreamSynchronize(h->stream));
+    return CUP2D_PS_OK;
+  });
 }
 
 int cup2d_psb_spmv_device(cup2d_psb *h, const double *d_x, double *d_y) {

@@ -537,4 +537,76 @@ __device__ __forceinline__ void mf_pupdate(MfIn &in, float *__restrict__ r,
   }
 }
 
+// ----------------------------------------------------------------------------
+// operators either side of the pressure solve (one thread per cell), shared
+// by the single and the batched solver; defined per translation unit
+// ----------------------------------------------------------------------------
+namespace {
+
+// Values of a cell's four neighbours (E, W, N, S) in a mesh without
+// coarse-fine faces: inside the block, in the same-level neighbour block, or
+// the ghost of a domain face -- `wall_sign` times the cell's own value (the
+// reference's labs mirror the boundary cell: -1 for the face-normal velocity
+// component, Definitions.h:84-117; +1 for the zero-gradient pressure).
+struct Nb4 {
+  double e, w, n, s;
+};
+__device__ __forceinline__ Nb4 neighbours(const double *__restrict__ f, long i, int4 nb, double xs,
+                                          double ys) {
+  const int c = (int)(i & 63), ix = c & 7, iy = c >> 3;
+  const double self = f[i];
+  Nb4 r;
+  r.e = ix < 7 ? f[i + 1] : nb.y >= 0 ? f[(long)nb.y * 64 + iy * 8] : xs * self;
+  r.w = ix > 0 ? f[i - 1] : nb.x >= 0 ? f[(long)nb.x * 64 + iy * 8 + 7] : xs * self;
+  r.n = iy < 7 ? f[i + 8] : nb.w >= 0 ? f[(long)nb.w * 64 + ix] : ys * self;
+  r.s = iy > 0 ? f[i - 8] : nb.z >= 0 ? f[(long)nb.z * 64 + 56 + ix] : ys * self;
+  return r;
+}
+
+// rhs = h/(2 dt) [(u_E - u_W) + (v_N - v_S)] - h/(2 dt) chi [(ud_E - ud_W) + (vd_N - vd_S)]
+// (updatePressureRHS, PressureSingle.cpp:336-349), in the reference's order of
+// operations (no contraction).
+__global__ __launch_bounds__(WG) void k_pressure_rhs(int nb, const int4 *__restrict__ nbr,
+                                                     const double *__restrict__ hb,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ v,
+                                                     const double *__restrict__ ud,
+                                                     const double *__restrict__ vd,
+                                                     const double *__restrict__ chi, double dt,
+                                                     double *__restrict__ rhs) {
+#pragma clang fp contract(off)
+  const long n = (long)nb * 64;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const int4 f = nbr[i >> 6];
+    const double facDiv = 0.5 * hb[i >> 6] / dt;
+    const Nb4 U = neighbours(u, i, f, -1.0, 1.0), V = neighbours(v, i, f, 1.0, -1.0);
+    const Nb4 UD = neighbours(ud, i, f, -1.0, 1.0), VD = neighbours(vd, i, f, 1.0, -1.0);
+    double t = facDiv * ((U.e - U.w) + (V.n - V.s));
+    t += -facDiv * chi[i] * ((UD.e - UD.w) + (VD.n - VD.s));
+    rhs[i] = t;
+  }
+}
+
+// u += (-dt/2 h (p_E - p_W)) / h^2, v likewise (pressureCorrectionKernel and
+// PressureSingle::pressureCorrection, PressureSingle.cpp:122-210), zero-gradient
+// pressure ghosts, in the reference's order of operations (no contraction).
+__global__ __launch_bounds__(WG) void k_pressure_correct(int nb, const int4 *__restrict__ nbr,
+                                                         const double *__restrict__ hb,
+                                                         const double *__restrict__ p, double dt,
+                                                         double *__restrict__ u,
+                                                         double *__restrict__ v) {
+#pragma clang fp contract(off)
+  const long n = (long)nb * 64;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double h = hb[i >> 6];
+    const double pFac = -0.5 * dt * h, ih2 = 1.0 / h / h;
+    const Nb4 P = neighbours(p, i, nbr[i >> 6], 1.0, 1.0);
+    const double tu = pFac * (P.e - P.w), tv = pFac * (P.n - P.s);
+    u[i] += tu * ih2;
+    v[i] += tv * ih2;
+  }
+}
+
+}  // namespace
+
 }  // namespace cup2d

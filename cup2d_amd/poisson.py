@@ -366,6 +366,21 @@ class BatchSolver:
             self.h, d_rhs.ptr, d_p.ptr, float(tol), float(rel_tol), int(max_restarts), st))
         return [s.as_dict() for s in st]
 
+    def pressure_rhs(self, u, v, udef_u, udef_v, chi, dt):
+        """The divergence RHS for all instances (host arrays, concatenated)."""
+        d = [self.device_array(np.ascontiguousarray(a, dtype=np.float64)) for a in (u, v, udef_u, udef_v, chi)]
+        d_rhs = self.device_array()
+        _check_b(self.h, _lib.load().cup2d_psb_pressure_rhs_device(self.h, *[a.ptr for a in d], float(dt),
+                                                                   d_rhs.ptr))
+        return d_rhs.numpy()
+
+    def pressure_correct(self, p, u, v, dt):
+        """The velocity correction for all instances; returns (u, v)."""
+        d_p, d_u, d_v = (self.device_array(np.ascontiguousarray(a, dtype=np.float64)) for a in (p, u, v))
+        _check_b(self.h, _lib.load().cup2d_psb_pressure_correct_device(self.h, d_p.ptr, float(dt), d_u.ptr,
+                                                                       d_v.ptr))
+        return d_u.numpy(), d_v.numpy()
+
     def autotune(self, d_b: DeviceArray):
         ib = ctypes.c_int()
         ms = ctypes.c_double()

@@ -246,6 +246,15 @@ int cup2d_psb_set_meshes_on_device(cup2d_psb *h, int n_inst, const cup2d_mesh_de
 int cup2d_psb_pressure_step_device(cup2d_psb *h, const double *d_rhs, double *d_p, double tol_abs,
                                    double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
+/* The divergence RHS and the velocity correction for every instance at once
+ * (as cup2d_ps_pressure_rhs_device / cup2d_ps_pressure_correct_device; all
+ * instances without coarse-fine faces, else CUP2D_PS_EINVAL). */
+int cup2d_psb_pressure_rhs_device(cup2d_psb *h, const double *d_u, const double *d_v,
+                                  const double *d_udef_u, const double *d_udef_v,
+                                  const double *d_chi, double dt, double *d_rhs);
+int cup2d_psb_pressure_correct_device(cup2d_psb *h, const double *d_p, double dt, double *d_u,
+                                      double *d_v);
+
 /* offsets[k] = first unknown of instance k, offsets[n_inst] = total (n_inst+1
  * entries). */
 int cup2d_psb_offsets(const cup2d_psb *h, long long *offsets);

@@ -128,3 +128,9 @@ def test_null_handles_are_einval_without_a_device():
     assert lib.cup2d_ps_pressure_correct_device(None, None, 1.0, None, None) == 1
     assert lib.cup2d_psb_set_meshes_on_device(None, 1, None) == 1
     assert lib.cup2d_psb_pressure_step_device(None, None, None, 0.0, 1e-10, 10, None) == 1
+
+
+def test_batch_projection_null_handle_is_einval():
+    lib = _lib.load()
+    assert lib.cup2d_psb_pressure_rhs_device(None, None, None, None, None, None, 1.0, None) == 1
+    assert lib.cup2d_psb_pressure_correct_device(None, None, 1.0, None, None) == 1

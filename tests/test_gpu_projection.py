@@ -75,3 +75,29 @@ def test_projection_round_trip_reduces_divergence():
     rhs1 = s.pressure_rhs(uo, vo, z, z, z, dt)
     assert st["converged"]
     assert np.linalg.norm(rhs1) < 0.05 * np.linalg.norm(rhs0)
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_batch_projection_operators_match_oracle(on_device):
+    """All instances in one call: each instance's slice equals the oracle's."""
+    meshes = [MESHES[n]() for n in ("uniform16x8_walls", "one_block", "periodic_xy", "periodic_x",
+                                    "level2_uniform")]
+    bs = PS.BatchSolver().set_meshes(meshes, on_device=on_device)
+    rng = np.random.default_rng(7)
+    u, v, ud, vd, p = (rng.normal(size=bs.n) for _ in range(5))
+    chi = rng.uniform(0, 1, size=bs.n)
+    dt = 0.02
+    rhs = bs.pressure_rhs(u, v, ud, vd, chi, dt)
+    uo, vo = bs.pressure_correct(p, u, v, dt)
+    for k, m in enumerate(meshes):
+        sl = slice(bs.offsets[k], bs.offsets[k + 1])
+        assert np.array_equal(rhs[sl], PR.pressure_rhs(m, u[sl], v[sl], ud[sl], vd[sl], chi[sl], dt)), k
+        ru, rv = PR.pressure_correct(m, p[sl], u[sl], v[sl], dt)
+        assert np.array_equal(uo[sl], ru) and np.array_equal(vo[sl], rv), k
+    # a batch with a coarse-fine mesh is rejected
+    cf = M.from_blocks(2, 2, 2, 1.0, [(0, 0, 0), (0, 1, 0), (0, 1, 1), (1, 0, 2), (1, 1, 2),
+                                      (1, 0, 3), (1, 1, 3)])
+    bs.set_meshes([meshes[0], cf], on_device=on_device)
+    z = np.zeros(bs.n)
+    with pytest.raises(_lib.SolverError):
+        bs.pressure_rhs(z, z, z, z, z, dt)
