@@ -837,7 +837,7 @@ struct DevBuf {
 
 struct cup2d_ps {
   cup2d_ps_config cfg{};
-  std::string err;
+  mutable std::string err;  // also set by the const queries
   int device = 0;
   bool mixed = false;
   hipStream_t stream = nullptr;
@@ -1747,7 +1747,10 @@ int cup2d_ps_sizes(const cup2d_ps *h, long long *n, long long *nnz, long long *n
 int cup2d_ps_get_csr(const cup2d_ps *h, long long *row_ptr, int *col, double *val) {
   if (!h || !row_ptr || !col || !val) return CUP2D_PS_EINVAL;
   if (!h->have_mesh) return CUP2D_PS_ENOMESH;
-  if (!h->has_csr) return CUP2D_PS_EINVAL;
+  if (!h->has_csr) {
+    h->err = "no CSR copy: the mesh was assembled on the device";
+    return CUP2D_PS_EINVAL;
+  }
   for (size_t i = 0; i < h->op.rp.size(); i++) row_ptr[i] = h->op.rp[i];
   std::memcpy(col, h->op.col.data(), h->op.col.size() * sizeof(int));
   std::memcpy(val, h->op.val.data(), h->op.val.size() * sizeof(double));

@@ -134,7 +134,7 @@ def test_device_assembled_mesh_has_no_csr():
     x = np.ones(m.n)
     with pytest.raises(_lib.SolverError):
         s.spmv(x, kind=1)
-    with pytest.raises(_lib.SolverError):
+    with pytest.raises(_lib.SolverError, match="no CSR copy"):
         s.csr()
     # switching between the two assemblies on one handle
     s.set_mesh(W.graded_disk_mesh())
