@@ -16,6 +16,10 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
   tangential component, which is where the two differ, is not read by
   either operator); the pressure lab is zero-gradient (ghost = boundary
   cell), as the explicit matrix's wall rows (ExpAMRSolver.cpp:301) assume.
+  That lab's code is Cubism's BlockLabNeumann (Definitions.h:199-222 only
+  calls it; the submodule is absent), so the pressure ghost at
+  non-periodic domain faces is parity unpinned against the reference:
+  it rests on the matrix's convention.
 
 Pinned in tests/test_projection_oracle.py by closed forms: linear fields
 (exact central differences), the ghost rules on a 1-block mesh checked cell
