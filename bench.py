@@ -1,22 +1,31 @@
 """Benchmark of the pressure-Poisson hot path on one MI355X.
 
-One step = one solve A x = b to 1e-10 relative residual (2-norm, x0 = 0) on
-the 3-level graded AMR mesh (16x8 base blocks of 8x8 cells, disk at
-(0.5,0.5) r=0.1, 1448 blocks / 92,672 unknowns), FP64, with b and x resident
-in HBM when the timed region starts.  Prints one JSON line.
+One step = one batched solve of the largest workload that fits one GPU:
+64 independent AMR Poisson instances (block counts log-uniform in
+[512, 8192], seed 4; 216,896 blocks, 13,881,344 unknowns in all), each with
+its own disk refinement and seeded vortex right-hand side, every instance
+solved from x0 = 0 to 1e-10 relative residual (2-norm) by FP64 BiCGSTAB with
+the 64x64 block-Jacobi preconditioner, b and x resident in HBM.  `value` is
+FP64 solves per second over the timed steps.  Prints one JSON line.
 
-Also measured in the same run:
-* the same solve in mixed precision;
-* the batched workload: 64 independent AMR instances (block counts
-  log-uniform in [512, 8192], 216,896 blocks / 13.9M unknowns in total),
-  solved together in FP64 and in mixed precision, against the same 64
-  solves one after another with the single-solve FP64 solver;
-* roofline of the operator kernel (timed live with HIP events on the
-  solver's stream) on the bench mesh and on the DRAM-resident batch,
-  against the 8 TB/s HBM peak;
-* the CPU baseline: the oracle's restatement of the reference CPU solver
+Also measured in the same run (all in `config`):
+* the same batch in mixed precision (FP32 Krylov vectors, BF16 MFMA
+  preconditioner, FP64 residual replacement), and the like-for-like
+  mixed / FP64 batched ratio;
+* the same 64 solves one after another with the single-solve FP64 solver,
+  fused (graph-replayed) and unfused (one operation per launch, the
+  reference GPU loop's structure): the denominators of the batch speedups;
+* every instance's FP64 true residual, and the error of the FP64 and the
+  mixed solutions against the oracle's FP64 solve on a sample of instances;
+* the single-mesh workloads (3-level graded mesh, 1448 blocks; 6-level
+  mesh, 16,229 blocks) in FP64 and mixed precision;
+* `roofline`: the dominant batched FP64 iteration kernel, timed live with
+  HIP events on the solver's stream, with the counted HBM traffic of the
+  committed rocprofv3 summary; the batched SpMV on its minimal bytes;
+* `cpu_baseline`: the oracle's restatement of the reference CPU solver
   (pipelined BiCGSTAB with the block-Cholesky preconditioner,
-  AMRSolver.cpp:149-490) on a bounded sample, OpenMP on the host cores.
+  AMRSolver.cpp:149-490) on a bounded sample of the same instances, OpenMP
+  on the host cores.
 """
 
 from __future__ import annotations
@@ -34,10 +43,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solves/sec at 1e-10 rel residual, FP64 vs mixed; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-
+WORKLOAD = ("64 independent AMR Poisson instances (N_b log-uniform in [512, 8192], seed 4; 5 levels, "
+            "one random disk each; seeded 16-vortex RHS), each solved from x0 = 0 to 1e-10 rel "
+            "2-norm residual, BiCGSTAB + 64x64 block-Jacobi")
 
 KERNELS = ["p update + block preconditioner", "nu = A z1 (+ dots)", "s update + block preconditioner",
            "t = A z2 (+ dots)", "x, r update (+ dots)"]
+KERNEL_NAMES64 = ["kb_pupdate64", "kb_spmv_nu<double>", "kb_supdate64", "kb_spmv_t<double>",
+                  "kb_xrupdate<double>"]
 
 
 def kernel_bytes(which, n, n_blocks, fp64=True):
@@ -78,78 +91,189 @@ def pmc_traffic(mode):
     return (fetch * ff + write * wf) * 1024.0
 
 
-def spmv_bytes_csr(n, nnz):
-    """Algorithmic bytes of one FP64 CSR SpMV with int32 indices:
-    nnz*(8+4) + (n+1)*4 + n*8 (x) + n*8 (y)."""
-    return nnz * 12 + (n + 1) * 4 + n * 16
+def spmv_bytes_min(n, n_blocks, n_explicit, width=9):
+    """Minimal bytes of the stencil-form SpMV: x and y once (16 B per row),
+    32 B of block metadata (neighbour ids, explicit-row mask, h^2) per block,
+    and per explicit (coarse-fine) row its index (4 B) and `width` coded
+    entries (4 B column + 2 B coefficient code each; 7-9 entries is typical)."""
+    return 16 * n + 32 * n_blocks + n_explicit * (4 + 6 * width)
 
 
-def cpu_baseline(mesh, b, budget_s=12.0):
+def host_cpu():
+    """(model name, logical CPUs) of this host, from /proc/cpuinfo."""
+    model, count = "unknown", 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                count += 1
+    except OSError:
+        pass
+    return model, count
+
+
+def c4_sample(meshes):
+    """Instances of the CPU baseline and the error check: four of the batch,
+    evenly spaced in instance order (their block counts span the range)."""
+    return [0, 16, 32, 48]
+
+
+def cpu_baseline(meshes, rhs, budget_s=30.0):
+    """The oracle's restatement of the reference CPU solver (pipelined
+    BiCGSTAB + block Cholesky, AMRSolver.cpp:149-490) on the sample
+    instances, one solve each, OpenMP on every host thread, stopped early
+    when the budget is spent."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     O.set_num_threads(threads)
-    A = O.assemble(mesh)
-    h2 = O.h2(mesh)
-    solves = 0
-    t0 = time.perf_counter()
-    iters = []
-    while True:
-        x, st = O.pipelined_bicgstab(A, mesh, b, rel_tol=1e-10, max_iter=2000, max_restarts=100,
-                                     h2v=h2)
-        solves += 1
+    done, blocks, iters = [], 0, []
+    t_total = 0.0
+    for k in c4_sample(meshes):
+        m, b = meshes[k], rhs[k]
+        A = O.assemble(m)
+        h2 = O.h2(m)
+        t0 = time.perf_counter()
+        _, st = O.pipelined_bicgstab(A, m, b, rel_tol=1e-10, max_iter=4000, max_restarts=100, h2v=h2)
+        t_total += time.perf_counter() - t0
+        done.append(k)
+        blocks += m.n_blocks
         iters.append(st["iterations"])
-        if time.perf_counter() - t0 > budget_s or solves >= 20:
+        if t_total > budget_s:
             break
-    dt = time.perf_counter() - t0
-    return {"value": solves / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{solves} full solves of the bench workload (pipelined BiCGSTAB + block "
-                      f"Cholesky, AMRSolver.cpp:149-490, to 1e-10 rel 2-norm, {iters[0]} iterations "
-                      f"each) in {dt:.1f} s on {threads} OpenMP threads"}
+    model, _ = host_cpu()
+    batch_blocks = sum(m.n_blocks for m in meshes)
+    return {"value": len(done) / t_total, "unit": "solves/s", "cores": threads, "kind": "port",
+            "cpu": model,
+            "sample": f"instances {done} of the 64-instance batch ({blocks} blocks, "
+                      f"{[meshes[k].n_blocks for k in done]} each), one solve each (pipelined "
+                      f"BiCGSTAB + block Cholesky, AMRSolver.cpp:149-490, to 1e-10 rel 2-norm, "
+                      f"{iters} iterations) in {t_total:.1f} s on {threads} OpenMP threads",
+            "batch_equivalent_solves_per_s": len(meshes) / (t_total * batch_blocks / blocks)}
+
+
+def oracle_errors(meshes, rhs, x_by_prec, offsets):
+    """Relative L2 and max-norm error of the GPU solutions against the
+    oracle's FP64 solve (plain BiCGSTAB to 1e-12, h^2-mean removed) on the
+    sample instances, and the oracle's long-double true residuals."""
+    from oracle import oracle as O
+    out = {name: {"rel_l2_max": 0.0, "rel_max_max": 0.0, "true_rel2_max": 0.0} for name in x_by_prec}
+    for k in c4_sample(meshes):
+        m, b = meshes[k], rhs[k]
+        A = O.assemble(m)
+        h2 = O.h2(m)
+        xr, _ = O.bicgstab(A, b, rel_tol=1e-12, norm="l2", max_iter=8000)
+        xr = O.mean_remove(h2, xr)
+        for name, x in x_by_prec.items():
+            xk = x[offsets[k]:offsets[k + 1]]
+            e = out[name]
+            e["rel_l2_max"] = max(e["rel_l2_max"], float(np.linalg.norm(xk - xr) / np.linalg.norm(xr)))
+            e["rel_max_max"] = max(e["rel_max_max"], float(np.abs(xk - xr).max() / np.abs(xr).max()))
+            e["true_rel2_max"] = max(e["true_rel2_max"], O.true_residual(A, b, xk)["rel2"])
+    for e in out.values():
+        e["instances"] = c4_sample(meshes)
+        e["oracle"] = "plain BiCGSTAB (BiCGSTAB.cu:412-620 restated) to 1e-12 rel 2-norm, mean removed"
+    return out
 
 
 def batch_spmv_handle(PS, meshes):
     """FP64 batch handle with 32 blocks per work item (the tuned size of the
     FP64 batch) for the operator kernel alone."""
     cfg = PS.default_config(norm=PS.NORM_L2)
-    cfg.reserved_i[1] = 32
-    return PS.BatchSolver(cfg).set_meshes(meshes)
+    cfg.item_blocks = 32
+    return PS.BatchSolver(cfg).set_meshes(meshes, on_device=True)
 
 
 def batch_spmv_roofline(PS, meshes, reps):
+    """The batched operator kernel alone (kb_spmv, FP64) on its minimal
+    stencil-form bytes, with the counted traffic and the streaming-copy
+    ceiling of the same size."""
     bs = batch_spmv_handle(PS, meshes)
     sz = bs.sizes()
     d_x = bs.device_array(np.random.default_rng(0).normal(size=bs.n))
     d_y = bs.device_array()
     ms = bs.spmv_time(d_x, d_y, reps=reps)
     ms_copy = bs.copy_time(d_x, d_y, reps=reps)
-    byt = spmv_bytes_csr(sz["n"], sz["nnz"])
+    nb = sum(m.n_blocks for m in meshes)
+    byt = spmv_bytes_min(sz["n"], nb, sz["n_explicit"])
     ach = byt / (ms * 1e-3) / 1e9
     traffic = pmc_traffic("batch_spmv")
     return {"bound": "hbm", "kernel": "kb_spmv (batched stencil SpMV, FP64, 13.9M rows)",
             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic, "algorithmic_bytes_csr_equivalent": byt, "ms_per_launch": ms,
-            # measured DRAM bytes (calibrated counters) over the kernel time, and the
-            # streaming-copy ceiling of the same access pattern on this GPU
-            "dram_GB_per_s": traffic / (ms * 1e-3) / 1e9 if traffic else None,
+            "algorithmic_bytes": byt, "ms_per_launch": ms, "traffic": traffic,
+            "traffic_GB_per_s": traffic / (ms * 1e-3) / 1e9 if traffic else None,
+            "traffic_over_algorithmic": traffic / byt if traffic else None,
             "stream_copy_GB_per_s": 16 * sz["n"] / (ms_copy * 1e-3) / 1e9}
 
 
-def bench_amr6(PS, W, steps, use_graph):
-    """Mixed precision at scale: FP64 and mixed single solves on the 6-level
-    AMR mesh around 4 seeded disks (~16k blocks, ~1M unknowns), to 1e-10
-    relative 2-norm residual, and the relative L2 difference of the mixed
-    pressure from the FP64 one (same mean removal)."""
-    mesh = W.four_disk_amr6_mesh()
-    b = W.workload_rhs(mesh, 3)
-    out = {"workload": "6-level AMR grid refined around 4 seeded disks (r in [0.02, 0.08]), "
-                       "seeded vortex RHS, to 1e-10 rel 2-norm",
-           "n_blocks": mesh.n_blocks, "n": mesh.n}
-    xs = {}
-    for name, prec in (("fp64", PS.FP64), ("mixed", PS.MIXED)):
-        s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=6000, use_graph=use_graph,
-                      precision=prec)
+def timed_batch(PS, meshes, b, precision, steps, warmup, use_graph, profile=0):
+    """Batched solves of the whole batch from x0 = 0: `warmup` untimed, then
+    `steps` timed (device-resident b and x; wall clock around each solve,
+    which synchronises on completion).  Returns (result dict, x, stats)."""
+    bs = PS.BatchSolver(PS.default_config(precision=precision, norm=PS.NORM_L2, max_iter=4000,
+                                          use_graph=use_graph, profile=profile))
+    t0 = time.perf_counter()
+    bs.set_meshes(meshes, on_device=True)  # device-side assembly of every instance
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    d_b = bs.device_array(b)
+    d_x = bs.device_array(np.zeros_like(b))
+    ib, ms_it = bs.autotune(d_b)
+    zeros = np.zeros_like(b)
+    for _ in range(warmup):
+        d_x.upload(zeros)
+        bs.solve_device(d_b, d_x, rel_tol=1e-10)
+    t = []
+    for _ in range(steps):
+        d_x.upload(zeros)
         t0 = time.perf_counter()
-        s.set_mesh(mesh)
+        st = bs.solve_device(d_b, d_x, rel_tol=1e-10)
+        t.append(time.perf_counter() - t0)
+    its = [s["iterations"] for s in st]
+    tot = float(np.sum(t))
+    res = {"solves_per_s": len(meshes) * steps / tot, "ms_per_batch": tot * 1e3 / steps,
+           "ms_per_batch_median": float(np.median(t)) * 1e3,
+           "set_meshes_ms_device_assembly": setup_ms, "item_blocks": ib,
+           "ms_per_iteration_tuned": ms_it,
+           "iterations_max": int(max(its)), "iterations_mean": float(np.mean(its)),
+           "iteration_work_ms": float(np.mean(its)) * ms_it,
+           "refinements_max": int(max(s["refinements"] for s in st)),
+           "all_converged": bool(all(s["converged"] for s in st)),
+           "true_rel2_max": float(max(s["true_rel2"] for s in st)),
+           "true_relinf_max": float(max(s["true_relinf"] for s in st))}
+    return res, d_x.numpy(), st, bs
+
+
+def sequential(PS, meshes, rhs, schedule, use_graph):
+    """The batch's solves one after another with the single-solve FP64
+    solver (schedule 0: fused, graph-replayed; 1: one operation per launch,
+    the reference GPU loop's structure).  One warm-up solve per mesh."""
+    s = PS.Solver(norm=PS.NORM_L2, max_iter=4000, use_graph=use_graph, schedule=schedule)
+    tot, iters, conv = 0.0, [], True
+    for m, bb in zip(meshes, rhs):
+        s.set_mesh(m, on_device=True)
+        d_b = PS.DeviceArray.from_host(s, bb)
+        d_x = PS.DeviceArray(s, m.n)
+        d_x.upload(np.zeros(m.n))
+        s.solve_device(d_b, d_x, rel_tol=1e-10)  # warm-up (graph capture)
+        d_x.upload(np.zeros(m.n))
+        t0 = time.perf_counter()
+        st = s.solve_device(d_b, d_x, rel_tol=1e-10)
+        tot += time.perf_counter() - t0
+        iters.append(st["iterations"])
+        conv = conv and st["converged"]
+    return {"solves_per_s": len(meshes) / tot, "ms_total": tot * 1e3,
+            "iterations_mean": float(np.mean(iters)), "all_converged": bool(conv)}
+
+
+def bench_single(PS, W, mesh, b, steps, use_graph):
+    """FP64, mixed and unfused FP64 single solves of one mesh (device-resident
+    b and x, from x0 = 0, median of `steps` after one warm-up)."""
+    out = {"n_blocks": mesh.n_blocks, "n": mesh.n}
+    xs = {}
+    for name, prec, sched in (("fp64", PS.FP64, 0), ("mixed", PS.MIXED, 0), ("fp64_unfused", PS.FP64, 1)):
+        s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=6000, use_graph=use_graph,
+                      precision=prec, schedule=sched)
+        t0 = time.perf_counter()
+        s.set_mesh(mesh, on_device=True)
         setup_ms = (time.perf_counter() - t0) * 1e3
         d_b = PS.DeviceArray.from_host(s, b)
         d_x = PS.DeviceArray(s, mesh.n)
@@ -157,7 +281,7 @@ def bench_amr6(PS, W, steps, use_graph):
         d_x.upload(zeros)
         s.solve_device(d_b, d_x, rel_tol=1e-10)  # warm-up
         t = []
-        for _ in range(steps):
+        for _ in range(steps if sched == 0 else max(1, steps // 3)):
             d_x.upload(zeros)
             t0 = time.perf_counter()
             st = s.solve_device(d_b, d_x, rel_tol=1e-10)
@@ -167,12 +291,14 @@ def bench_amr6(PS, W, steps, use_graph):
         out[name] = {"solves_per_s": 1.0 / sec, "ms_per_solve": sec * 1e3, "set_mesh_ms": setup_ms,
                      "iterations": st["iterations"], "refinements": st["refinements"],
                      "restarts": st["restarts"], "converged": st["converged"],
-                     "true_rel2_residual": s.residual_device(d_b, d_x)["rel2"]}
+                     "true_rel2_residual": st["true_rel2"]}
+        if name == "fp64":
+            out["fp64"]["kernel_ms"] = [s.kernel_time(k, reps=100) for k in range(5)]
         del s, d_b, d_x
     out["mixed_speedup_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["fused_speedup_vs_unfused"] = out["fp64"]["solves_per_s"] / out["fp64_unfused"]["solves_per_s"]
     out["mixed_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
                                         np.linalg.norm(xs["fp64"]))
-    out["assembly"] = bench_assembly(PS, mesh)
     return out
 
 
@@ -216,64 +342,19 @@ def bench_assembly(PS, mesh, reps=5):
     return res
 
 
-def bench_batch(PS, W, steps, use_graph):
-    """The 64-instance batch: batched FP64 and mixed solves/s, the same 64
-    solves one after another with the single-solve FP64 solver, and the
-    roofline of the batched operator kernel (13.9M unknowns, DRAM-resident)."""
-    meshes = W.batch64_meshes()
-    rhs = [W.workload_rhs(m, 400 + k) for k, m in enumerate(meshes)]
-    b = np.concatenate(rhs)
-    out = {"workload": "64 AMR instances, N_b log-uniform in [512, 8192] (seed 4), 5 levels, one "
-                       "random disk each, seeded vortex RHS; all solved to 1e-10 rel 2-norm",
-           "instances": len(meshes), "n_blocks": int(sum(m.n_blocks for m in meshes))}
-    roof = None
-    for name, prec in (("fp64", PS.FP64), ("mixed", PS.MIXED)):
-        bs = PS.BatchSolver(PS.default_config(precision=prec, norm=PS.NORM_L2, max_iter=4000,
-                                              use_graph=use_graph))
-        t0 = time.perf_counter()
-        bs.set_meshes(meshes)  # host assembly (threads) + upload
-        setup_ms = (time.perf_counter() - t0) * 1e3
-        d_b = bs.device_array(b)
-        d_x = bs.device_array(np.zeros_like(b))
-        ib, ms_it = bs.autotune(d_b)
-        zeros = np.zeros_like(b)
-        d_x.upload(zeros)
-        bs.solve_device(d_b, d_x, rel_tol=1e-10)  # warm-up
-        t = []
-        for _ in range(steps):
-            d_x.upload(zeros)
-            t0 = time.perf_counter()
-            st = bs.solve_device(d_b, d_x, rel_tol=1e-10)
-            t.append(time.perf_counter() - t0)
-        its = [s["iterations"] for s in st]
-        sec = float(np.median(t))
-        out[name] = {"solves_per_s": len(meshes) / sec, "ms_per_batch": sec * 1e3,
-                     "set_meshes_ms": setup_ms,
-                     "item_blocks": ib, "ms_per_iteration_tuned": ms_it,
-                     "iterations_max": int(max(its)), "iterations_mean": float(np.mean(its)),
-                     "refinements_max": int(max(s["refinements"] for s in st)),
-                     "all_converged": bool(all(s["converged"] for s in st))}
-        if prec == PS.FP64:
-            roof = batch_spmv_roofline(PS, meshes, reps=20)
-        del bs, d_b, d_x
-    out["assembly"] = bench_batch_assembly(PS, meshes)
-    # the same 64 solves one after another (single-solve FP64 solver)
-    s = PS.Solver(norm=PS.NORM_L2, max_iter=4000, use_graph=use_graph)
-    tot = 0.0
-    for m, bb in zip(meshes, rhs):
-        s.set_mesh(m)
-        d_b = PS.DeviceArray.from_host(s, bb)
-        d_x = PS.DeviceArray(s, m.n)
-        d_x.upload(np.zeros(m.n))
-        s.solve_device(d_b, d_x, rel_tol=1e-10)  # warm-up (graph capture)
-        d_x.upload(np.zeros(m.n))
-        t0 = time.perf_counter()
-        s.solve_device(d_b, d_x, rel_tol=1e-10)
-        tot += time.perf_counter() - t0
-    out["fp64_sequential"] = {"solves_per_s": len(meshes) / tot, "ms_total": tot * 1e3}
-    out["mixed_batched_vs_fp64_sequential"] = out["mixed"]["solves_per_s"] / out["fp64_sequential"]["solves_per_s"]
-    out["fp64_batched_vs_fp64_sequential"] = out["fp64"]["solves_per_s"] / out["fp64_sequential"]["solves_per_s"]
-    return out, roof
+def batch_kernels(PS, bs, n, n_blocks, fp64, reps=20):
+    """Every batched iteration kernel alone (HIP events on the handle's
+    stream, every instance iterating): ms per launch, algorithmic bytes and
+    GB/s; with the counted traffic of the committed summary (FP64 rows
+    biter0..4 of profiles/pmc_traffic.txt)."""
+    out = []
+    for k in range(5):
+        ms = bs.kernel_time(k, reps=reps)
+        byt = kernel_bytes(k, n, n_blocks, fp64)
+        tr = pmc_traffic(f"biter{k}") if fp64 else pmc_traffic(f"miter{k}")
+        out.append({"kernel": KERNELS[k], "ms_per_launch": ms, "algorithmic_bytes": byt,
+                    "GB_per_s": byt / (ms * 1e-3) / 1e9, "traffic": tr})
+    return out
 
 
 def kernel_only(PS, W, which, reps):
@@ -326,13 +407,27 @@ def kernel_only(PS, W, which, reps):
         b = np.concatenate([W.workload_rhs(m, 400 + k) for k, m in enumerate(meshes)])
         cfg = PS.default_config(precision=PS.FP64 if which == "batch_fp64" else PS.MIXED,
                                 max_iter=64, norm=PS.NORM_L2, use_graph=0)
-        cfg.reserved_i[1] = 32
+        cfg.item_blocks = 32
         bs = PS.BatchSolver(cfg).set_meshes(meshes)
         d_b = bs.device_array(b)
         d_x = bs.device_array(np.zeros_like(b))
         st = bs.solve_device(d_b, d_x, rel_tol=1e-14)
         print(json.dumps({"kernel": which, "iterations": max(s["iterations"] for s in st),
                           "ms": st[0]["ms"]}))
+        return
+    if which[:5] in ("biter", "miter"):
+        # batched iteration kernel K alone on the 64-instance batch (FP64:
+        # biterK, mixed: miterK), every instance iterating, 32 blocks per item
+        meshes = W.batch64_meshes()
+        cfg = PS.default_config(precision=PS.FP64 if which[0] == "b" else PS.MIXED, norm=PS.NORM_L2)
+        cfg.item_blocks = 32
+        bs = PS.BatchSolver(cfg).set_meshes(meshes, on_device=True)
+        k = int(which[5:])
+        n, nb = bs.n, sum(m.n_blocks for m in meshes)
+        ms = bs.kernel_time(k, reps=reps)
+        byt = kernel_bytes(k, n, nb, which[0] == "b")
+        print(json.dumps({"kernel": KERNELS[k], "ms_per_launch": ms, "launches": reps + 1,
+                          "algorithmic_bytes": byt, "GB_per_s": byt / (ms * 1e-3) / 1e9}))
         return
     if which.startswith("iter"):
         mesh = W.graded_disk_mesh()
@@ -358,33 +453,35 @@ def kernel_only(PS, W, which, reps):
         d_x = s.device_array(np.random.default_rng(0).normal(size=s.n))
         d_y = s.device_array()
         ms = s.spmv_time(d_x, d_y, reps=reps)
+    nb = sz["n"] // 64
+    byt = spmv_bytes_min(sz["n"], nb, sz["n_explicit"])
     print(json.dumps({"kernel": which, "ms_per_launch": ms, "launches": reps, "n": sz["n"],
-                      "nnz": sz["nnz"], "algorithmic_bytes": spmv_bytes_csr(sz["n"], sz["nnz"])}))
+                      "nnz": sz["nnz"], "algorithmic_bytes": byt, "GB_per_s": byt / (ms * 1e-3) / 1e9}))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=10, help="timed batched solves (each: 64 instances)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle-error legs")
+    ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--graph", type=int, default=1,
                     help="1: replay iteration chunks from a hipGraph; 0: plain stream launches")
-    ap.add_argument("--chunk", type=int, default=0, help="iterations per chunk (0: automatic)")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
-    ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--kernel-only", choices=["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection", "iter0", "iter1",
-                                              "iter2", "iter3", "iter4"], default=None,
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the sequential solves and the single-mesh workloads")
+    modes = ["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection"]
+    modes += [f"iter{k}" for k in range(5)] + [f"biter{k}" for k in range(5)] + [f"miter{k}" for k in range(5)]
+    ap.add_argument("--kernel-only", choices=modes, default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
-                         "(spmv: operator on the bench mesh; batch_spmv: batched operator on the "
+                         "(spmv: operator on the 3-level mesh; batch_spmv: batched operator on the "
                          "64-instance batch; batch_fp64 / batch_mixed: 64 iterations of the "
                          "batched solve, every kernel launched from the stream; iterK: iteration "
-                         "kernel K of the bench solve; copy: the library's streaming copy over "
-                         "13,881,344 doubles, the bandwidth ceiling and the traffic counters' "
-                         "calibration; assembly: host against device-side assembly of the "
-                         "6-level 16k-block mesh)")
+                         "kernel K of the 3-level mesh's solve; biterK / miterK: batched FP64 / "
+                         "mixed iteration kernel K with every instance iterating; copy: the "
+                         "library's streaming copy over 13,881,344 doubles, the bandwidth ceiling "
+                         "and the traffic counters' calibration; assembly: host against device-side "
+                         "assembly of the 6-level 16k-block mesh)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
@@ -398,145 +495,72 @@ def main():
         kernel_only(PS, W, args.kernel_only, args.steps)
         return
 
-    mesh = W.graded_disk_mesh()
-    b = W.workload_rhs(mesh, 2, with_sources=True)
-
-    def run_variant(precision, steps, warmup):
-        s = PS.Solver(device=0, norm=PS.NORM_L2, max_iter=4000, use_graph=args.graph,
-                      chunk=args.chunk, precision=precision)
-        t0 = time.perf_counter()
-        s.set_mesh(mesh)  # host assembly + upload, reported apart from the solves
-        setup_ms = (time.perf_counter() - t0) * 1e3
-        d_b = PS.DeviceArray.from_host(s, b)
-        d_x = PS.DeviceArray(s, mesh.n)
-        zeros = np.zeros(mesh.n)
-        for _ in range(warmup):
-            d_x.upload(zeros)
-            s.solve_device(d_b, d_x, rel_tol=1e-10, max_restarts=100)
-        # timed region: `steps` solves, each from x0 = 0
-        stats = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            d_x.upload(zeros)
-            stats.append(s.solve_device(d_b, d_x, rel_tol=1e-10, max_restarts=100))
-        t1 = time.perf_counter()
-        res = s.residual_device(d_b, d_x)
-        return s, d_x, {
-            "ms_per_step": (t1 - t0) * 1e3 / steps,
-            "solves_per_s": steps / (t1 - t0),
-            "iterations": stats[-1]["iterations"],
-            "refinements": stats[-1]["refinements"],
-            "restarts": stats[-1]["restarts"],
-            "converged": all(st["converged"] for st in stats),
-            "true_rel2_residual": res["rel2"],
-            "solve_ms_device": float(np.median([st["ms"] for st in stats])),
-            "set_mesh_ms": setup_ms,
-        }
-
-    main_prec = PS.MIXED if args.precision == "mixed" else PS.FP64
-    other_prec = PS.FP64 if main_prec == PS.MIXED else PS.MIXED
-    s, d_x, main = run_variant(main_prec, args.steps, args.warmup)
-    _, _, other = run_variant(other_prec, max(3, args.steps // 4), 1)
-    sz = s.sizes()
-    ms_per_step = main["ms_per_step"]
-    # the same solve through the host-buffer entry point (b in and x out over
-    # PCIe every solve): reported beside the device-resident value, not as it
-    nh = max(3, args.steps // 4)
-    t0 = time.perf_counter()
-    for _ in range(nh):
-        s.solve(b, rel_tol=1e-10, max_restarts=100)
-    host_rate = nh / (time.perf_counter() - t0)
-    # the pressure step around the solve, device-resident (b = rhs - L p_old,
-    # the solve from 0, p = p_old + dp - <dp>_h2), p_old = the last solution
-    d_rhs = PS.DeviceArray.from_host(s, b)
-    d_p = PS.DeviceArray(s, mesh.n)
-    p_half = 0.5 * d_x.numpy()
-    t_ps = []
-    for _ in range(nh):
-        d_p.upload(p_half)
-        t0 = time.perf_counter()
-        st_ps = s.pressure_step_device(d_rhs, d_p, rel_tol=1e-10, max_restarts=100)
-        t_ps.append(time.perf_counter() - t0)
-    pressure_step = {"ms": float(np.median(t_ps)) * 1e3, "iterations": st_ps["iterations"],
-                     "converged": st_ps["converged"],
-                     "p_old": "half the solution: the step solves for the other half"}
-
-    # every iteration kernel timed alone (HIP events on the solver's stream);
-    # the roofline object is the dominant one
-    kern = []
-    for k in range(5):
-        ms_k = s.kernel_time(k, reps=100)
-        byt = kernel_bytes(k, sz["n"], mesh.n_blocks, main_prec == PS.FP64)
-        kern.append({"kernel": KERNELS[k], "ms_per_launch": ms_k, "algorithmic_bytes": byt,
-                     "GB_per_s": byt / (ms_k * 1e-3) / 1e9})
+    meshes = W.batch64_meshes()
+    rhs = [W.workload_rhs(m, 400 + k) for k, m in enumerate(meshes)]
+    b = np.concatenate(rhs)
+    n_blocks = int(sum(m.n_blocks for m in meshes))
+    fp64, x64, st64, bs64 = timed_batch(PS, meshes, b, PS.FP64, args.steps, args.warmup, args.graph)
+    offsets = bs64.offsets
+    sz = bs64.sizes()
+    kern = batch_kernels(PS, bs64, bs64.n, n_blocks, True)
+    del bs64
+    mixed, x32, _, bs32 = timed_batch(PS, meshes, b, PS.MIXED, args.steps, args.warmup, args.graph)
+    kern32 = batch_kernels(PS, bs32, bs32.n, n_blocks, False)
+    del bs32
+    cfg = {
+        "workload": WORKLOAD, "instances": len(meshes), "n_blocks": n_blocks, "n": sz["n"],
+        "nnz": sz["nnz"], "explicit_rows": sz["n_explicit"],
+        "block_counts": {"min": int(min(m.n_blocks for m in meshes)),
+                         "max": int(max(m.n_blocks for m in meshes))},
+        "graph": args.graph, "fp64": fp64, "mixed": mixed,
+        "mixed_batched_vs_fp64_batched": mixed["solves_per_s"] / fp64["solves_per_s"],
+        "fp64_iteration_kernels": kern, "mixed_iteration_kernels": kern32,
+    }
+    if not args.no_extra:
+        seq = sequential(PS, meshes, rhs, 0, args.graph)
+        unf = sequential(PS, meshes, rhs, 1, 0)
+        cfg["fp64_sequential_fused"] = seq
+        cfg["fp64_sequential_unfused"] = unf
+        cfg["mixed_batched_vs_fp64_sequential_fused"] = mixed["solves_per_s"] / seq["solves_per_s"]
+        cfg["mixed_batched_vs_fp64_sequential_unfused"] = mixed["solves_per_s"] / unf["solves_per_s"]
+        cfg["fp64_batched_vs_fp64_sequential_fused"] = fp64["solves_per_s"] / seq["solves_per_s"]
+        g = W.graded_disk_mesh()
+        cfg["graded_1448_blocks"] = bench_single(PS, W, g, W.workload_rhs(g, 2, with_sources=True), 10,
+                                                 args.graph)
+        a6 = W.four_disk_amr6_mesh()
+        cfg["amr6_16k_blocks"] = bench_single(PS, W, a6, W.workload_rhs(a6, 3), 3, args.graph)
+        cfg["assembly"] = bench_batch_assembly(PS, meshes)
+    if not args.no_cpu:
+        cfg["error_vs_oracle"] = oracle_errors(meshes, rhs, {"fp64": x64, "mixed": x32}, offsets)
     dom = max(range(5), key=lambda k: kern[k]["ms_per_launch"])
-    # SpMV alone on the bench mesh: stencil kernel and plain CSR kernel
-    d_y = PS.DeviceArray(s, mesh.n)
-    ms_spmv = s.spmv_time(d_x, d_y, kind=0, reps=args.spmv_reps)
-    ms_csr = s.spmv_time(d_x, d_y, kind=1, reps=args.spmv_reps)
-    bytes_spmv = spmv_bytes_csr(sz["n"], sz["nnz"])
-    achieved = bytes_spmv / (ms_spmv * 1e-3) / 1e9
-    fp64 = main if main_prec == PS.FP64 else other
-    mixed = main if main_prec == PS.MIXED else other
-
     out = {
         "metric": METRIC,
-        "value": 1e3 / ms_per_step,
+        "value": fp64["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": fp64["ms_per_batch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_prec == PS.FP64 else "f32/bf16 inner, f64 refinement",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": "3-level graded AMR Poisson solve (16x8 base blocks of 8x8 cells, disk "
-                        "(0.5,0.5) r=0.1, radii scaled x5), seeded 16-vortex RHS + 8 disk sources, "
-                        "FP64 BiCGSTAB + 64x64 block-Jacobi to 1e-10 rel 2-norm residual",
-            "n_blocks": mesh.n_blocks, "n": sz["n"], "nnz": sz["nnz"],
-            "explicit_rows": sz["n_explicit"],
-            "precision": args.precision,
-            "iterations": main["iterations"], "restarts": main["restarts"],
-            "converged": main["converged"], "true_rel2_residual": main["true_rel2_residual"],
-            "solve_ms_device": main["solve_ms_device"],
-            "graph": args.graph,
-            "fp64": fp64,
-            "mixed": mixed,
-            "mixed_speedup_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
-            "host_buffers_solves_per_s": host_rate,
-            "pressure_step_device": pressure_step,
-        },
+        "config": cfg,
         "roofline": {
             "bound": "hbm",
-            "kernel": kern[dom]["kernel"] + (" (FP64)" if main_prec == PS.FP64 else " (mixed)"),
-            "achieved": kern[dom]["GB_per_s"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
+            "kernel": KERNEL_NAMES64[dom] + ": " + kern[dom]["kernel"] + " (FP64, 64-instance batch)",
+            "achieved": kern[dom]["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": kern[dom]["GB_per_s"] / HBM_PEAK_GBS,
-            "traffic": pmc_traffic(f"iter{dom}") if main_prec == PS.FP64 else None,
+            "traffic": kern[dom]["traffic"],
             "algorithmic_bytes": kern[dom]["algorithmic_bytes"],
             "ms_per_launch": kern[dom]["ms_per_launch"],
-            "profile": "profiles/roofline_kernels.md, profiles/pmc_traffic.txt",
+            "profile": "profiles/roofline_batch.md, profiles/pmc_traffic.txt",
         },
-        "iteration_kernels": kern,
-        "roofline_spmv": {
-            "bound": "hbm", "kernel": "k_spmv (stencil SpMV, FP64, bench mesh)",
-            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("spmv"),
-            "algorithmic_bytes_csr_equivalent": bytes_spmv, "ms_per_launch": ms_spmv,
-            "csr_kernel_ms_per_launch": ms_csr,
-        },
+        "roofline_spmv": batch_spmv_roofline(PS, meshes, args.spmv_reps),
     }
-    if not args.no_batch:
-        out["config"]["amr6_1M"] = bench_amr6(PS, W, args.batch_steps, args.graph)
-        batch, roof_dram = bench_batch(PS, W, args.batch_steps, args.graph)
-        out["config"]["batch"] = batch
-        out["roofline_dram"] = roof_dram
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mesh, b)
+        out["cpu_baseline"] = cpu_baseline(meshes, rhs)
     print(json.dumps(out))
 
 

@@ -25,18 +25,25 @@ class MeshDesc(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("device", _I), ("precision", _I), ("mean_constraint", _I), ("norm", _I),
-                ("max_iter", _I), ("chunk", _I), ("use_graph", _I), ("reserved_i", _I * 5),
+                ("max_iter", _I), ("chunk", _I), ("use_graph", _I), ("profile", _I),
+                ("item_blocks", _I), ("schedule", _I), ("reserved_i", _I * 2),
                 ("eps", _D), ("breakdown_tol", _D), ("inner_tol", _D), ("reserved_d", _D * 5)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", _I), ("restarts", _I), ("converged", _I), ("refinements", _I),
-                ("err_init", _D), ("err_final", _D), ("ms", _D), ("reserved", _D * 5)]
+                ("err_init", _D), ("err_final", _D), ("ms", _D), ("true_rel2", _D),
+                ("true_relinf", _D), ("ms_phase", _D * 5), ("reserved", _D * 2)]
 
     def as_dict(self):
         return {"iterations": self.iterations, "restarts": self.restarts,
                 "converged": bool(self.converged), "refinements": self.refinements,
-                "err_init": self.err_init, "err_final": self.err_final, "ms": self.ms}
+                "err_init": self.err_init, "err_final": self.err_final, "ms": self.ms,
+                "true_rel2": self.true_rel2, "true_relinf": self.true_relinf,
+                "ms_phase": dict(zip(PHASES, list(self.ms_phase)))}
+
+
+PHASES = ("spmv", "prec", "update", "scalar", "setup")  # cup2d_ps_phase order
 
 
 # every symbol of include/cup2d_poisson.h with (restype, argtypes)
@@ -79,6 +86,7 @@ SIGNATURES = {
     "cup2d_psb_item_blocks": (_I, [_P]),
     "cup2d_psb_spmv_device": (_I, [_P, _P, _P]),
     "cup2d_psb_spmv_time": (_I, [_P, _P, _P, _I, _P]),
+    "cup2d_psb_kernel_time": (_I, [_P, _I, _I, _P]),
     "cup2d_psb_copy_time": (_I, [_P, _P, _P, _I, _P]),
     "cup2d_psb_sizes": (_I, [_P, _P, _P, _P]),
     "cup2d_psb_last_error": (ctypes.c_char_p, [_P]),

@@ -440,4 +440,15 @@ std::vector<uint16_t> make_pfrag(const std::vector<double> &P) {
   return f;
 }
 
+std::vector<double> make_pfrag64(const std::vector<double> &P) {
+  std::vector<double> f(64 * 64);
+  for (int mt = 0; mt < 4; mt++)
+    for (int ks = 0; ks < 16; ks++)
+      for (int l = 0; l < 64; l++) {
+        const int m = l & 15, row = 16 * mt + 4 * (m & 3) + (m >> 2);
+        f[(mt * 16 + ks) * 64 + l] = P[row * 64 + 16 * (l >> 4) + ks];
+      }
+  return f;
+}
+
 }  // namespace cup2d

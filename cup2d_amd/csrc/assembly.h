@@ -106,4 +106,10 @@ std::vector<double> make_pinv();
 // P[16 mt + (l & 15)][32 ks + 8 (l >> 4) + j].
 std::vector<uint16_t> make_pfrag(const std::vector<double> &P);
 
+// P in the A-operand layout of v_mfma_f64_16x16x4f64 (FP64, 64 fragments of
+// 64 lanes): fragment (mt, ks), lane l = P[16 mt + o(l & 15)][16 (l >> 4) + ks]
+// with the row order o(m) = 4 (m & 3) + (m >> 2), which makes the four output
+// registers of a lane four consecutive cells (see tile_prec64).
+std::vector<double> make_pfrag64(const std::vector<double> &P);
+
 }  // namespace cup2d

@@ -533,117 +533,94 @@ __device__ __forceinline__ void running_tiles(const BView &V, const Scal *st, F 
   }
 }
 
+// Calls body(tm) for the workgroup's tiles (grid-stride over workgroups)
+// whose instance is iterating; the next tile's meta is loaded while the
+// current one is processed.  The skip is uniform over the workgroup, so body
+// may synchronize it.
+template <class F>
+__device__ __forceinline__ void running_tiles_wg(const BView &V, const Scal *st, F &&body) {
+  int tile = blockIdx.x;
+  if (tile >= V.n_tiles) return;
+  int4 tm = V.tile[tile];
+  for (;;) {
+    const int next = tile + gridDim.x;
+    const bool more = next < V.n_tiles;
+    const int4 ntm = V.tile[more ? next : tile];
+    if (!st[tm.x].idle) body(tm);
+    if (!more) break;
+    tile = next;
+    tm = ntm;
+  }
+}
+
+// The thread's four cells of tile tm (see tile_prec64): first index and
+// whether its block exists (a lane past the tile's end addresses the
+// tile's first block).
+__device__ __forceinline__ long btile_cells(int4 tm, int w, int lane, bool &ok) {
+  ok = (lane & 15) < tm.z;
+  return (long)(tm.y + (ok ? (lane & 15) : 0)) * 64 + cell4(w, lane);
+}
+
 // FP64: p = r + beta (p - omega nu), or on restart p = rhat = r; z1 = P p
-// (P in LDS, 4 blocks per step)
-__global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__restrict__ P,
+// on MFMA (a workgroup per 16-block tile)
+__global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__restrict__ pf,
                                                    const double *__restrict__ r,
                                                    double *__restrict__ rhat,
                                                    double *__restrict__ p,
                                                    const double *__restrict__ nu,
                                                    double *__restrict__ z1, const Scal *st) {
-  __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][4 * 64];
-  load_P(Ps, P);
-  const int c = lane_id(), w = wave_id();
-  running_tiles(V, st, [&](int4 tm) {
-    const int k = tm.x, b0 = tm.y, tnb = tm.z;
+  __shared__ double tile[TILE][TROW];
+  const int lane = lane_id(), w = wave_id();
+  PFrag64 F;
+  load_pfrag64(F, pf, w, lane);
+  running_tiles_wg(V, st, [&](int4 tm) {
+    const int k = tm.x;
     const double beta = st[k].beta, omega = st[k].omega;
     const int restart = st[k].restart;
-    long ii[4];
-    bool ok[4];
-    double rv[4], pv[4] = {}, nv[4] = {};
-    auto load = [&](int g) {
+    bool ok;
+    const long i0 = btile_cells(tm, w, lane, ok);
+    double rv[4], pv[4], nv[4], pi[4];
+    ld4(r, i0, rv);
+    ld4(p, i0, pv);
+    ld4(nu, i0, nv);
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        ok[j] = g + j < tnb;
-        ii[j] = (long)(b0 + (ok[j] ? g + j : g)) * 64 + c;
-        rv[j] = r[ii[j]];
-      }
-      if (!restart)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          pv[j] = p[ii[j]];
-          nv[j] = nu[ii[j]];
-        }
-    };
-    load(0);
-    for (int g = 0; g < tnb; g += 4) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const double pi = restart ? rv[j] : rv[j] + beta * (pv[j] - omega * nv[j]);
-        if (ok[j]) {
-          if (restart) rhat[ii[j]] = rv[j];
-          p[ii[j]] = pi;
-        }
-        vs[w][j * 64 + c] = ok[j] ? pi : 0.0;
-      }
-      long io[4];
-      bool oo[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        io[j] = ii[j];
-        oo[j] = ok[j];
-      }
-      if (g + 4 < tnb) load(g + 4);  // the next group's loads, in flight during the product
-      wave_sync();
-      double z[4];
-      prec_group<4>(Ps, vs[w], c, z);
-      wave_sync();
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-        if (oo[j]) z1[io[j]] = z[j];
+    for (int q = 0; q < 4; q++) pi[q] = restart ? rv[q] : rv[q] + beta * (pv[q] - omega * nv[q]);
+    if (ok) {
+      if (restart) st4(rhat, i0, rv);
+      st4(p, i0, pi);
     }
+    tile_put(tile, w, lane, ok, pi);
+    __syncthreads();
+    const f64x4 z = tile_prec64(tile, F, lane);
+    __syncthreads();  // the tile is rewritten by the next tile
+    if (ok) st4(z1, i0, z);
   });
 }
 
-// FP64: s = r - alpha nu (stored in r); z2 = P s
-__global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__restrict__ P,
+// FP64: s = r - alpha nu (stored in r); z2 = P s on MFMA
+__global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__restrict__ pf,
                                                    double *__restrict__ r,
                                                    const double *__restrict__ nu,
                                                    double *__restrict__ z2, const Scal *st) {
-  __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][4 * 64];
-  load_P(Ps, P);
-  const int c = lane_id(), w = wave_id();
-  running_tiles(V, st, [&](int4 tm) {
-    const int k = tm.x, b0 = tm.y, tnb = tm.z;
-    const double alpha = st[k].alpha;
-    long ii[4];
-    bool ok[4];
-    double rv[4], nv[4];
-    auto load = [&](int g) {
+  __shared__ double tile[TILE][TROW];
+  const int lane = lane_id(), w = wave_id();
+  PFrag64 F;
+  load_pfrag64(F, pf, w, lane);
+  running_tiles_wg(V, st, [&](int4 tm) {
+    const double alpha = st[tm.x].alpha;
+    bool ok;
+    const long i0 = btile_cells(tm, w, lane, ok);
+    double rv[4], nv[4], si[4];
+    ld4(r, i0, rv);
+    ld4(nu, i0, nv);
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        ok[j] = g + j < tnb;
-        ii[j] = (long)(b0 + (ok[j] ? g + j : g)) * 64 + c;
-        rv[j] = r[ii[j]];
-        nv[j] = nu[ii[j]];
-      }
-    };
-    load(0);
-    for (int g = 0; g < tnb; g += 4) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const double si = rv[j] - alpha * nv[j];
-        if (ok[j]) r[ii[j]] = si;
-        vs[w][j * 64 + c] = ok[j] ? si : 0.0;
-      }
-      long io[4];
-      bool oo[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        io[j] = ii[j];
-        oo[j] = ok[j];
-      }
-      if (g + 4 < tnb) load(g + 4);  // the next group's loads, in flight during the product
-      wave_sync();
-      double z[4];
-      prec_group<4>(Ps, vs[w], c, z);
-      wave_sync();
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-        if (oo[j]) z2[io[j]] = z[j];
-    }
+    for (int q = 0; q < 4; q++) si[q] = rv[q] - alpha * nv[q];
+    if (ok) st4(r, i0, si);
+    tile_put(tile, w, lane, ok, si);
+    __syncthreads();
+    const f64x4 z = tile_prec64(tile, F, lane);
+    __syncthreads();
+    if (ok) st4(z2, i0, z);
   });
 }
 
@@ -823,6 +800,35 @@ __global__ __launch_bounds__(WG) void kb_mean(BView V, Scal *st, const double *p
   if (threadIdx.x == 0) st[k].mean = o[0] / o[1];
 }
 
+// partials of |b - y|^2, |b - y|_inf, |b|^2, |b|_inf per item (b's mean
+// row taken as 0); y holds A x.  Explicit items hold no cells of their own.
+__global__ __launch_bounds__(WG) void kb_rpart(BView V, const double *__restrict__ b,
+                                               const double *__restrict__ y, double *part) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int4 mt = V.item[item];
+    const long long mr = V.mean_row[mt.x];
+    long i0, i1;
+    item_cells(mt, i0, i1);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) {
+      const double bi = (i == mr) ? 0.0 : b[i];
+      const double ri = bi - y[i];
+      v[0] += ri * ri;
+      v[1] = nmax(v[1], fabs(ri));
+      v[2] += bi * bi;
+      v[3] = nmax(v[3], fabs(bi));
+    }
+    store_item_partials<4, 0xAu>(v, part, V.n_items, item);
+  }
+}
+// out[4k .. 4k+3] = the totals of instance k
+__global__ __launch_bounds__(WG) void kb_rtot(BView V, const double *part, double *out) {
+  const int k = blockIdx.x;
+  double o[4];
+  range_totals<4, 0xAu>(part, V.n_items, V.item_begin[k], V.item_begin[k + 1], o);
+  if (threadIdx.x < 4) out[4 * k + threadIdx.x] = o[threadIdx.x];
+}
+
 thread_local std::string g_berr;  // failures without a handle, per calling thread
 
 }  // namespace
@@ -877,6 +883,14 @@ struct cup2d_psb {
   std::vector<Scal> h_st;
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
+  double *d_rt = nullptr;           // [4K] residual norms of the returned solutions
+  std::vector<double> h_rt;
+  // per-phase timers (cfg.profile): an event after every launch, tagged
+  // with the launch's phase
+  std::vector<hipEvent_t> pev;
+  std::vector<int> pph;
+  size_t pused = 0;
+  double pms[5] = {0, 0, 0, 0, 0};
   std::vector<BBuf> allocs;       // mesh-dependent device memory
   std::vector<BBuf> unit_allocs;  // work-unit lists and partials (depend on ib)
   std::vector<BBuf> spare;        // released buffers, reused by size (mesh changes, retuning)
@@ -979,6 +993,34 @@ void drop_graph(cup2d_psb *h) {
   }
 }
 
+// Profiling (cfg.profile), as in the single-solve handle: an event after
+// the launch just made, tagged with its phase (-1: a start mark).
+void bptick(cup2d_psb *h, int ph) {
+  if (!h->cfg.profile) return;
+  if (h->pused == h->pev.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    h->pev.push_back(e);
+    h->pph.push_back(-1);
+  }
+  if (hipEventRecord(h->pev[h->pused], h->stream) != hipSuccess) return;
+  h->pph[h->pused++] = ph;
+}
+void bpcollect(cup2d_psb *h) {
+  if (!h->cfg.profile) return;
+  for (size_t i = 1; i < h->pused; i++) {
+    float ms = 0;
+    if (h->pph[i] >= 0 && hipEventElapsedTime(&ms, h->pev[i - 1], h->pev[i]) == hipSuccess)
+      h->pms[h->pph[i]] += ms;
+  }
+  if (h->pused > 0) {
+    std::swap(h->pev[0], h->pev[h->pused - 1]);
+    h->pph[0] = -1;
+    h->pused = 1;
+  }
+}
+bool bgraphs_on(const cup2d_psb *h) { return h->cfg.use_graph && !h->cfg.profile; }
+
 BView bview(const cup2d_psb *h) {
   BView V;
   V.A.nb = (int)h->nb;
@@ -1005,6 +1047,10 @@ BView bview(const cup2d_psb *h) {
 
 int grid_items(const cup2d_psb *h) {
   return (int)std::min<size_t>(BGRID, std::max<size_t>(1, h->item_inst.size()));
+}
+// FP64 preconditioner kernels: a workgroup per tile
+int grid_tiles64(const cup2d_psb *h) {
+  return (int)std::min<size_t>(BGRID, std::max<size_t>(1, h->tile_inst.size()));
 }
 int grid_tiles(const cup2d_psb *h) {
   return (int)std::min<size_t>(BGRID, std::max<size_t>(1, (h->tile_inst.size() + WAVES - 1) / WAVES));
@@ -1078,19 +1124,27 @@ void launch_iteration_b(cup2d_psb *h) {
     *z1 = (T *)h->v_z1, *z2 = (T *)h->v_z2, *t = (T *)h->v_t;
   Scal *st = h->d_st;
   if constexpr (sizeof(T) == 8)
-    kb_pupdate64<<<Gt, WG, 0, s>>>(V, h->d_P, r, rhat, p, nu, z1, st);
+    kb_pupdate64<<<grid_tiles64(h), WG, 0, s>>>(V, h->d_P, r, rhat, p, nu, z1, st);
   else
     kb_pupdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, rhat, p, nu, z1, st);
+  bptick(h, CUP2D_PS_PHASE_PREC);
   kb_spmv_nu<T><<<Gi, WG, 0, s>>>(V, z1, nu, rhat, st, h->d_p2);
+  bptick(h, CUP2D_PS_PHASE_SPMV);
   kb_alpha<T><<<K, WG, 0, s>>>(V, st, h->d_p2, nu, rhat);
+  bptick(h, CUP2D_PS_PHASE_SCALAR);
   if constexpr (sizeof(T) == 8)
-    kb_supdate64<<<Gt, WG, 0, s>>>(V, h->d_P, r, nu, z2, st);
+    kb_supdate64<<<grid_tiles64(h), WG, 0, s>>>(V, h->d_P, r, nu, z2, st);
   else
     kb_supdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, nu, z2, st);
+  bptick(h, CUP2D_PS_PHASE_PREC);
   kb_spmv_t<T><<<Gi, WG, 0, s>>>(V, z2, t, r, st, h->d_p4);
+  bptick(h, CUP2D_PS_PHASE_SPMV);
   kb_omega<T><<<K, WG, 0, s>>>(V, st, h->d_p4, t, r);
+  bptick(h, CUP2D_PS_PHASE_SCALAR);
   kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, z1, z2, r, t, rhat, st, h->d_p5);
+  bptick(h, CUP2D_PS_PHASE_UPDATE);
   kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0, 0);
+  bptick(h, CUP2D_PS_PHASE_SCALAR);
 }
 
 int chunk_b(const cup2d_psb *h) { return h->cfg.chunk > 0 ? h->cfg.chunk : 8; }
@@ -1108,6 +1162,7 @@ void launch_replacement_b(cup2d_psb *h) {
   kb_init<float><<<Gi, WG, 0, s>>>(V, h->d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5,
                                    F_REPLACE);
   kb_eval<float><<<h->K, WG, 0, s>>>(V, h->d_st, h->d_p5, r, rhat, 2, 0);
+  bptick(h, CUP2D_PS_PHASE_SETUP);
 }
 
 void launch_chunk_b(cup2d_psb *h, int C) {
@@ -1117,6 +1172,7 @@ void launch_chunk_b(cup2d_psb *h, int C) {
     else launch_iteration_b<double>(h);
   }
   kb_alldone<<<1, WG, 0, h->stream>>>(h->K, h->d_st, h->d_all_done);
+  bptick(h, CUP2D_PS_PHASE_SCALAR);
 }
 
 int ensure_graph_b(cup2d_psb *h) {
@@ -1147,13 +1203,13 @@ int read_all_done(cup2d_psb *h, int &done) {
 // earlier (timing runs).
 int run_chunks_b(cup2d_psb *h, int max_iter, int max_chunks) {
   const int C = chunk_b(h);
-  if (h->cfg.use_graph) {
+  if (bgraphs_on(h)) {
     int rc = ensure_graph_b(h);
     if (rc) return rc;
   }
   int launched = 0, chunks = 0;
   for (;;) {
-    if (h->cfg.use_graph) {
+    if (bgraphs_on(h)) {
       BCHK(h, hipGraphLaunch(h->gexec, h->stream));
     } else {
       launch_chunk_b(h, C);
@@ -1164,6 +1220,7 @@ int run_chunks_b(cup2d_psb *h, int max_iter, int max_chunks) {
     int done = 0;
     int rc = read_all_done(h, done);
     if (rc) return rc;
+    bpcollect(h);
     // every instance stops at its own iteration cap; pauses for residual
     // replacement skip iterations, hence the margin
     if (done || launched > 2 * max_iter + 2 * C) break;
@@ -1210,6 +1267,7 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
                                        F_LIVE);
   kb_eval<T><<<K, WG, 0, h->stream>>>(V, h->d_st, h->d_p5, r, rhat, 1, sizeof(T) == 4 ? 1 : 0);
   kb_alldone<<<1, WG, 0, h->stream>>>(K, h->d_st, h->d_all_done);
+  bptick(h, CUP2D_PS_PHASE_SETUP);
   BCHK(h, hipGetLastError());
   int done = 0;
   int rc = read_all_done(h, done);
@@ -1218,6 +1276,7 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
     if ((rc = run_chunks_b(h, max_iter, max_chunks))) return rc;
   }
   kb_select<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_xopt, h->d_st, F_ALL);
+  bptick(h, CUP2D_PS_PHASE_SETUP);
   BCHK(h, hipGetLastError());
   BCHK(h, hipMemcpyAsync(h->h_st.data(), h->d_st, K * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
   BCHK(h, hipStreamSynchronize(h->stream));
@@ -1233,6 +1292,9 @@ int solve_b_body(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_
   if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "solve before cup2d_psb_set_meshes");
   BCHK(h, hipSetDevice(h->cfg.device));
   const int K = h->K, max_iter = h->cfg.max_iter;
+  for (double &m : h->pms) m = 0.0;
+  h->pused = 0;
+  bptick(h, -1);
   BCHK(h, hipEventRecord(h->ev0, h->stream));
   BCHK(h, hipMemcpyAsync(h->d_x, d_x0, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
   if (d_b != h->d_b)  // the replacement step of the captured graph reads the handle's copy
@@ -1260,7 +1322,18 @@ int solve_b_body(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_
   kb_fwrite<<<Gi, WG, 0, h->stream>>>(V, h->d_x, d_out, h->d_st);
   BCHK(h, hipGetLastError());
   BCHK(h, hipEventRecord(h->ev1, h->stream));
+  // the returned solutions' true residuals, recomputed in FP64 (x_opt is
+  // free after the solve and holds A x)
+  kb_spmv<<<Gi, WG, 0, h->stream>>>(V, d_out, h->d_xopt, h->d_pm);
+  kb_meanrow<<<K, WG, 0, h->stream>>>(V, h->d_pm, h->d_xopt);
+  kb_rpart<<<Gi, WG, 0, h->stream>>>(V, d_b, h->d_xopt, h->d_p5);
+  kb_rtot<<<K, WG, 0, h->stream>>>(V, h->d_p5, h->d_rt);
+  bptick(h, CUP2D_PS_PHASE_SETUP);
+  BCHK(h, hipGetLastError());
+  h->h_rt.assign(4 * (size_t)K, 0.0);
+  BCHK(h, hipMemcpyAsync(h->h_rt.data(), h->d_rt, 4 * K * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   BCHK(h, hipStreamSynchronize(h->stream));
+  bpcollect(h);
   float ms = 0;
   BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   if (stats)
@@ -1273,6 +1346,10 @@ int solve_b_body(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_
       stats[k].err_init = err0[k];
       stats[k].err_final = errf[k];
       stats[k].ms = ms;
+      const double *rt = &h->h_rt[4 * (size_t)k];
+      stats[k].true_rel2 = rt[2] > 0 ? std::sqrt(rt[0]) / std::sqrt(rt[2]) : 0.0;
+      stats[k].true_relinf = rt[3] > 0 ? rt[1] / rt[3] : 0.0;
+      for (int q = 0; q < 5; q++) stats[k].ms_phase[q] = h->cfg.profile ? h->pms[q] : 0.0;
     }
   return CUP2D_PS_OK;
 }
@@ -1312,8 +1389,8 @@ bool config_ok(const cup2d_ps_config &c) {
   return c.max_iter > 0 && c.chunk >= 0 && c.eps >= 0 && c.breakdown_tol >= 0 &&
          c.inner_tol >= 0 && c.inner_tol < 1 &&
          (c.precision == CUP2D_PS_FP64 || c.precision == CUP2D_PS_MIXED) &&
-         (c.norm == CUP2D_PS_NORM_INF || c.norm == CUP2D_PS_NORM_L2) && c.reserved_i[1] >= 0 &&
-         c.reserved_i[1] <= MAX_ITEM_BLOCKS;
+         (c.norm == CUP2D_PS_NORM_INF || c.norm == CUP2D_PS_NORM_L2) && c.item_blocks >= 0 &&
+         c.item_blocks <= MAX_ITEM_BLOCKS && c.schedule == 0 && (c.profile == 0 || c.profile == 1);
 }
 
 }  // namespace
@@ -1359,6 +1436,7 @@ int cup2d_psb_destroy(cup2d_psb *h) {
   bfree_pool(h->allocs);
   bfree_pool(h->spare);
   if (h->h_all_done) (void)hipHostFree(h->h_all_done);
+  for (hipEvent_t e : h->pev) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1546,7 +1624,8 @@ static int finish_meshes(cup2d_psb *h, int n_inst) {
   int rc;
   static const std::vector<double> P = make_pinv();  // the same for every mesh
   static const std::vector<uint16_t> pf = make_pfrag(P);
-  if ((rc = bupload(h, pool, &h->d_P, P)) || (rc = bupload(h, pool, &h->d_mean_row, h->h_mean_row)))
+  static const std::vector<double> pf64 = make_pfrag64(P);
+  if ((rc = bupload(h, pool, &h->d_P, pf64)) || (rc = bupload(h, pool, &h->d_mean_row, h->h_mean_row)))
     return rc;
   if ((rc = balloc(h, pool, &h->d_pfrag, pf.size() / 8))) return rc;
   BCHK(h, hipMemcpy(h->d_pfrag, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -1560,10 +1639,11 @@ static int finish_meshes(cup2d_psb *h, int n_inst) {
     if ((rc = balloc(h, pool, &p, n * es))) return rc;
     *v = p;
   }
-  if ((rc = balloc(h, pool, &h->d_st, (size_t)n_inst)) || (rc = balloc(h, pool, &h->d_all_done, 1)))
+  if ((rc = balloc(h, pool, &h->d_st, (size_t)n_inst)) || (rc = balloc(h, pool, &h->d_all_done, 1)) ||
+      (rc = balloc(h, pool, &h->d_rt, 4 * (size_t)n_inst)))
     return rc;
   h->h_st.assign(n_inst, Scal{});
-  int ib = h->cfg.reserved_i[1];
+  int ib = h->cfg.item_blocks;
   if (ib <= 0) {
     std::lock_guard<std::mutex> lock(g_tuned_mu);
     auto it = g_tuned.find({(int)h->mixed, n_inst, h->nb});
@@ -1641,6 +1721,62 @@ static int autotune_body(cup2d_psb *h, const double *d_b, int *item_blocks, doub
   return CUP2D_PS_OK;
 }
 
+// One batched iteration kernel alone, every instance running: inputs x = 0
+// and a fixed b, one chunk of iterations to fill every partial, then `reps`
+// launches of kernel `which` between HIP events on the handle's stream.
+extern "C++" {
+template <class T>
+static int kernel_time_b(cup2d_psb *h, int which, int reps, double *ms_per_launch) {
+  const size_t n = (size_t)h->n;
+  std::vector<double> bh(n);
+  for (size_t i = 0; i < n; i++) bh[i] = double(int(i % 7) - 3);
+  BCHK(h, hipMemcpyAsync(h->d_b, bh.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  BCHK(h, hipMemsetAsync(h->d_x, 0, n * sizeof(double), h->stream));
+  for (int k = 0; k < h->K; k++) {
+    h->h_st[k] = marker_b(h, k, 0.0, 0.0, 1 << 20, 1 << 20);
+    h->h_st[k].rep_factor = 0.0;  // no pauses: every instance iterates in every launch
+    h->h_st[k].stall = 0;
+  }
+  int rc = run_pass<T>(h, h->d_b, chunk_b(h), 1);
+  if (rc) return rc;
+  BCHK(h, hipStreamSynchronize(h->stream));  // bh is read before it goes out of scope
+  const BView V = bview(h);
+  const int Gi = grid_items(h), Gt = grid_tiles(h), K = h->K;
+  hipStream_t s = h->stream;
+  T *r = (T *)h->v_r, *rhat = (T *)h->v_rhat, *p = (T *)h->v_p, *nu = (T *)h->v_nu,
+    *z1 = (T *)h->v_z1, *z2 = (T *)h->v_z2, *t = (T *)h->v_t;
+  Scal *st = h->d_st;
+  auto launch = [&]() {
+    switch (which) {
+      case 0:
+        if constexpr (sizeof(T) == 8) kb_pupdate64<<<grid_tiles64(h), WG, 0, s>>>(V, h->d_P, r, rhat, p, nu, z1, st);
+        else kb_pupdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, rhat, p, nu, z1, st);
+        break;
+      case 1: kb_spmv_nu<T><<<Gi, WG, 0, s>>>(V, z1, nu, rhat, st, h->d_p2); break;
+      case 2:
+        if constexpr (sizeof(T) == 8) kb_supdate64<<<grid_tiles64(h), WG, 0, s>>>(V, h->d_P, r, nu, z2, st);
+        else kb_supdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, nu, z2, st);
+        break;
+      case 3: kb_spmv_t<T><<<Gi, WG, 0, s>>>(V, z2, t, r, st, h->d_p4); break;
+      case 4: kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, z1, z2, r, t, rhat, st, h->d_p5); break;
+      case 5: kb_alpha<T><<<K, WG, 0, s>>>(V, st, h->d_p2, nu, rhat); break;
+      case 6: kb_omega<T><<<K, WG, 0, s>>>(V, st, h->d_p4, t, r); break;
+      default: kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0, 0);
+    }
+  };
+  launch();  // warm-up, untimed
+  BCHK(h, hipEventRecord(h->ev0, s));
+  for (int k = 0; k < reps; k++) launch();
+  BCHK(h, hipEventRecord(h->ev1, s));
+  BCHK(h, hipEventSynchronize(h->ev1));
+  BCHK(h, hipGetLastError());
+  float ms = 0;
+  BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *ms_per_launch = ms / reps;
+  return CUP2D_PS_OK;
+}
+}  // extern "C++"
+
 static int pressure_step_b(cup2d_psb *h, const double *d_rhs, double *d_p, double tol_abs,
                            double tol_rel, int max_restarts, cup2d_ps_stats *stats) {
   if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
@@ -1692,6 +1828,9 @@ int cup2d_psb_pressure_rhs_device(cup2d_psb *h, const double *d_u, const double 
     if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
     if (!d_u || !d_v || !d_udef_u || !d_udef_v || !d_chi || !d_rhs)
       return bfail(h, CUP2D_PS_EINVAL, "NULL vector");
+    // the kernel reads neighbours of every input while writing rhs
+    if (d_rhs == d_u || d_rhs == d_v || d_rhs == d_udef_u || d_rhs == d_udef_v || d_rhs == d_chi)
+      return bfail(h, CUP2D_PS_EINVAL, "rhs must not alias an input");
     if (const int rc = projection_checks_b(h, dt)) return rc;
     BCHK(h, hipSetDevice(h->cfg.device));
     k_pressure_rhs<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_h, d_u, d_v,
@@ -1749,6 +1888,17 @@ int cup2d_psb_spmv_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
   BCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   *ms_per_launch = ms / reps;
   return CUP2D_PS_OK;
+}
+
+int cup2d_psb_kernel_time(cup2d_psb *h, int which, int reps, double *ms_per_launch) {
+  if (!h) return bfail(nullptr, CUP2D_PS_EINVAL, "handle is NULL");
+  if (!ms_per_launch || reps <= 0 || which < 0 || which > 7) return bfail(h, CUP2D_PS_EINVAL, "bad argument");
+  if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "kernel timing before cup2d_psb_set_meshes");
+  return bguarded(h, [&]() -> int {
+    BCHK(h, hipSetDevice(h->cfg.device));
+    return h->mixed ? kernel_time_b<float>(h, which, reps, ms_per_launch)
+                    : kernel_time_b<double>(h, which, reps, ms_per_launch);
+  });
 }
 
 int cup2d_psb_copy_time(cup2d_psb *h, const double *d_x, double *d_y, int reps, double *ms_per_launch) {

@@ -272,18 +272,77 @@ struct BlockRows {
   }
 };
 
-// z = P v for NB blocks at once (FP64): lane c owns output row c; P (64x64,
-// symmetric) sits in LDS, v holds the NB input vectors (64 each).
-template <int NB>
-__device__ __forceinline__ void prec_group(const double *Ps, const double *v, int c, double (&z)[NB]) {
+// ---------------------------------------------------------------------------
+// FP64 MFMA block preconditioner: Z(64 x 16 blocks) = P(64x64) R(64 x 16)
+// with v_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16: lane l holds
+// A[l & 15][l >> 4], B[l >> 4][l & 15] and, in register v,
+// D[(l >> 4) + 4 v][l & 15] -- checked on the device).
+// A workgroup takes a tile of TILE = 16 blocks; wave w computes output rows
+// 16 w .. 16 w + 15 of every block of the tile in 16 K steps.  K step ks
+// covers input cells 16 q + ks (q = 0..3 is the lane's k slot), so lane
+// (j = l & 15, q = l >> 4) supplies cells 16 q .. 16 q + 15 of block j.  The
+// A rows are ordered o(m) = 4 (m & 3) + (m >> 2), so that D row q + 4 v is
+// output cell 16 w + 4 q + v:
+//   A operand of lane l, wave w, step ks: P[16 w + o(l & 15)][16 (l >> 4) + ks]
+//   D register v of lane l, wave w:       Z[16 w + 4 (l >> 4) + v][block l & 15]
+// The cells a thread updates, loads and stores are its output cells
+// 16 w + 4 q + v (v = 0..3); the inputs of the product go through an LDS
+// tile (tile_put / tile_get) between the update and the product.
+// ---------------------------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int TROW = 65;                 // LDS tile row stride (doubles): conflict-free reads
+
+struct PFrag64 {
+  double a[16];
+};
+
+// the wave's 16 A-operand fragments (P rows 16 w .. 16 w + 15)
+__device__ __forceinline__ void load_pfrag64(PFrag64 &F, const double *__restrict__ pf, int w,
+                                             int lane) {
 #pragma unroll
-  for (int j = 0; j < NB; j++) z[j] = 0.0;
-#pragma unroll 8
-  for (int k = 0; k < 64; k++) {
-    const double pk = Ps[k * 64 + c];
+  for (int ks = 0; ks < 16; ks++) F.a[ks] = pf[(w * 16 + ks) * 64 + lane];
+}
+
+// first of the thread's four output cells (relative to its block)
+__device__ __forceinline__ int cell4(int w, int lane) { return 16 * w + 4 * (lane >> 4); }
+
+__device__ __forceinline__ void ld4(const double *__restrict__ a, long i, double (&v)[4]) {
+  const double2 *p2 = reinterpret_cast<const double2 *>(a + i);
+  const double2 u0 = p2[0], u1 = p2[1];
+  v[0] = u0.x; v[1] = u0.y; v[2] = u1.x; v[3] = u1.y;
+}
+__device__ __forceinline__ void st4(double *__restrict__ a, long i, const double (&v)[4]) {
+  double2 *p2 = reinterpret_cast<double2 *>(a + i);
+  p2[0] = make_double2(v[0], v[1]);
+  p2[1] = make_double2(v[2], v[3]);
+}
+
+// the thread's four values (cells cell4 + v of block j of the tile) into the
+// LDS tile; zeros for a lane past the tile's last block
+__device__ __forceinline__ void tile_put(double (*tile)[TROW], int w, int lane, bool ok,
+                                         const double (&v)[4]) {
+  const int j = lane & 15, c0 = cell4(w, lane);
 #pragma unroll
-    for (int j = 0; j < NB; j++) z[j] = fma(pk, v[j * 64 + k], z[j]);
-  }
+  for (int k = 0; k < 4; k++) tile[j][c0 + k] = ok ? v[k] : 0.0;
+}
+
+// z (the thread's four output cells) = rows 16 w + 4 q + v of P times block
+// j of the tile, after a workgroup barrier that published the tile
+__device__ __forceinline__ f64x4 tile_prec64(const double (*tile)[TROW], const PFrag64 &F,
+                                             int lane) {
+  const int j = lane & 15, q = lane >> 4;
+  double in[16];
+#pragma unroll
+  for (int ks = 0; ks < 16; ks++) in[ks] = tile[j][16 * q + ks];
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 16; ks++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(F.a[ks], in[ks], acc, 0, 0, 0);
+  return acc;
+}
+__device__ __forceinline__ void st4(double *__restrict__ a, long i, const f64x4 &v) {
+  double2 *p2 = reinterpret_cast<double2 *>(a + i);
+  p2[0] = make_double2(v[0], v[1]);
+  p2[1] = make_double2(v[2], v[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -464,10 +523,6 @@ __device__ inline Scal evaluate(const Scal &prev, const double (&o)[NQ5], bool r
 }
 
 
-__device__ __forceinline__ void load_P(double *Ps, const double *__restrict__ P) {
-  for (int i = threadIdx.x; i < 64 * 64; i += WG) Ps[i] = P[i];
-  __syncthreads();
-}
 
 
 // 16 values of the lane's input cells (32 ks + 8 q + j) of one block

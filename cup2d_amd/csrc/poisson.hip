@@ -43,6 +43,7 @@ namespace {
 
 constexpr int MAXG = 512;  // workgroups of the block part of the vector kernels (grid-stride beyond)
 constexpr int MAXGE = 64;  // workgroups of the explicit-row part
+constexpr int MAXGP = 2048;  // workgroups of the FP64 preconditioner kernels (a tile each)
 constexpr int SPMV_U = 2;  // blocks per wavefront in flight in the operator kernels
 
 // ----------------------------------------------------------------------------
@@ -150,129 +151,95 @@ __global__ __launch_bounds__(WG) void k_init(OpView A, int Gb, const double *__r
 }
 
 // ----------------------------------------------------------------------------
-// iteration kernels, FP64 (vector-FMA preconditioner, P in LDS)
+// iteration kernels, FP64 (MFMA preconditioner: a workgroup per tile of 16
+// blocks, a wave per 16 output rows, see tile_prec64)
 // ----------------------------------------------------------------------------
 
-// P (64x64 FP64) into registers, 16 values per thread of the workgroup, so
-// the load is in flight while the scalars are reduced; stored to LDS later.
-__device__ __forceinline__ void prefetch_P(const double *__restrict__ P, double (&pr)[16]) {
-#pragma unroll
-  for (int k = 0; k < 16; k++) pr[k] = P[threadIdx.x + k * WG];
-}
-__device__ __forceinline__ void store_P(double *Ps, const double (&pr)[16]) {
-#pragma unroll
-  for (int k = 0; k < 16; k++) Ps[threadIdx.x + k * WG] = pr[k];
-  __syncthreads();
-}
-
-// The wave's group g of NB blocks (lane = cell): indices and validity.
-template <int NB>
-__device__ __forceinline__ void group_index(int g, int nb, int c, long (&ii)[NB], bool (&ok)[NB]) {
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    ok[j] = g * NB + j < nb;
-    ii[j] = (long)(ok[j] ? g * NB + j : g * NB) * 64 + c;
-  }
+// The thread's four cells of tile t: first index and whether its block exists
+// (a lane past the last block addresses the tile's first block).
+__device__ __forceinline__ long tile_cells(int t, int nb, int w, int lane, bool &ok) {
+  const int b = t * TILE + (lane & 15);
+  ok = b < nb;
+  return (long)(ok ? b : t * TILE) * 64 + cell4(w, lane);
 }
 
 // Top of iteration: p = r + beta (p - omega nu) (or p = rhat = r after a
 // restart), z1 = P p (BiCGSTAB.cu:471-520).  The x_opt copy of the
-// iteration's state is made by k_xrupdate, which reads x anyway.  The loads
-// of P and of the wave's first group are issued before the state is
+// iteration's state is made by k_xrupdate, which reads x anyway.  The P
+// fragments and the first tile's loads are issued before the state is
 // evaluated.
-template <int NB>
-__global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict__ P,
+__global__ __launch_bounds__(WG) void k_pupdate(int nb, const double *__restrict__ pf,
                                                 const double *__restrict__ r,
                                                 double *__restrict__ rhat, double *__restrict__ p,
                                                 const double *__restrict__ nu,
                                                 double *__restrict__ z1, Scal *st, int par,
                                                 const double *p5, int G5) {
   __shared__ Scal cur;
-  __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][NB * 64];
-  const int c = lane_id(), w = wave_id();
-  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
-  double pr[16];
-  prefetch_P(P, pr);
-  int g = blockIdx.x * WAVES + w;
-  long ii[NB];
-  bool ok[NB];
-  double rv[NB], pv[NB], nv[NB];
-  auto load_group = [&](int gg) {
-    group_index<NB>(gg, nb, c, ii, ok);
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      rv[j] = r[ii[j]];
-      pv[j] = p[ii[j]];
-      nv[j] = nu[ii[j]];
-    }
+  __shared__ double tile[TILE][TROW];
+  const int lane = lane_id(), w = wave_id();
+  const int ntiles = (nb + TILE - 1) / TILE;
+  PFrag64 F;
+  load_pfrag64(F, pf, w, lane);
+  int t = blockIdx.x;
+  bool ok = false;
+  long i0 = 0;
+  double rv[4], pv[4], nv[4];
+  auto load = [&](int tt) {
+    i0 = tile_cells(tt, nb, w, lane, ok);
+    ld4(r, i0, rv);
+    ld4(p, i0, pv);
+    ld4(nu, i0, nv);
   };
-  if (g < ng) load_group(g);
-  store_P(Ps, pr);  // published by the barriers of the state evaluation
+  if (t < ntiles) load(t);
   top_of_iteration(st, par, p5, G5, cur);
   if (cur.done) return;
   const double beta = cur.beta, omega = cur.omega;
   const int restart = cur.restart;
-  for (; g < ng; g += nw) {
+  for (; t < ntiles; t += gridDim.x) {
+    double pi[4];
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const double pi = restart ? rv[j] : rv[j] + beta * (pv[j] - omega * nv[j]);
-      if (ok[j]) {
-        if (restart) rhat[ii[j]] = rv[j];
-        p[ii[j]] = pi;
-      }
-      vs[w][j * 64 + c] = ok[j] ? pi : 0.0;
+    for (int k = 0; k < 4; k++) pi[k] = restart ? rv[k] : rv[k] + beta * (pv[k] - omega * nv[k]);
+    if (ok) {
+      if (restart) st4(rhat, i0, rv);
+      st4(p, i0, pi);
     }
-    long io[NB];
-    bool oo[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      io[j] = ii[j];
-      oo[j] = ok[j];
-    }
-    if (g + nw < ng) load_group(g + nw);  // in flight during the product
-    wave_sync();
-    double z[NB];
-    prec_group<NB>(Ps, vs[w], c, z);
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-      if (oo[j]) z1[io[j]] = z[j];
+    tile_put(tile, w, lane, ok, pi);
+    const bool oo = ok;
+    const long io = i0;
+    if (t + (int)gridDim.x < ntiles) load(t + gridDim.x);  // in flight during the product
+    __syncthreads();
+    const f64x4 z = tile_prec64(tile, F, lane);
+    __syncthreads();  // the tile is rewritten by the next pass
+    if (oo) st4(z1, io, z);
   }
 }
 
 // alpha = rho / ((rhat, nu) + eps); s = r - alpha nu (stored in r); z2 = P s
-// (BiCGSTAB.cu:527-545).  Loads of P and of the first group first.
-template <int NB>
-__global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict__ P,
+// (BiCGSTAB.cu:527-545).  Loads of P and of the first tile first.
+__global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict__ pf,
                                                 double *__restrict__ r, double *__restrict__ nu,
                                                 const double *__restrict__ rhat,
                                                 double *__restrict__ z2, Scal *st, int par,
                                                 const double *p2, int G2, long mean_row) {
-  __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][NB * 64];
-  const int c = lane_id(), w = wave_id();
-  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
-  double pr[16];
-  prefetch_P(P, pr);
-  int g = blockIdx.x * WAVES + w;
-  long ii[NB];
-  bool ok[NB];
-  double rv[NB], nv[NB];
-  auto load_group = [&](int gg) {
-    group_index<NB>(gg, nb, c, ii, ok);
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      rv[j] = r[ii[j]];
-      nv[j] = nu[ii[j]];
-    }
+  __shared__ double tile[TILE][TROW];
+  const int lane = lane_id(), w = wave_id();
+  const int ntiles = (nb + TILE - 1) / TILE;
+  PFrag64 F;
+  load_pfrag64(F, pf, w, lane);
+  int t = blockIdx.x;
+  bool ok = false;
+  long i0 = 0;
+  double rv[4], nv[4];
+  auto load = [&](int tt) {
+    i0 = tile_cells(tt, nb, w, lane, ok);
+    ld4(r, i0, rv);
+    ld4(nu, i0, nv);
   };
-  if (g < ng) load_group(g);
+  if (t < ntiles) load(t);
   const double rh_mr = mean_row >= 0 ? rhat[mean_row] : 0.0;
   const int done = st[par].done;  // state read before the reduction's barrier
   const double rho = st[par].rho, eps = st[par].eps;
   double o[2];
-  store_P(Ps, pr);  // published by the reduction's barrier
   load_totals<2, 0u>(p2, G2, o);
   if (done) return;
   const double rn = o[0] + (mean_row >= 0 ? rh_mr * o[1] : 0.0);
@@ -281,32 +248,25 @@ __global__ __launch_bounds__(WG) void k_supdate(int nb, const double *__restrict
     st[par].alpha = alpha;
     st[par].red[2] = rn;
   }
-  for (; g < ng; g += nw) {
+  for (; t < ntiles; t += gridDim.x) {
+    double si[4];
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      if (ii[j] == mean_row) nv[j] = o[1];
-      const double si = rv[j] - alpha * nv[j];
-      if (ok[j]) {
-        if (ii[j] == mean_row) nu[ii[j]] = nv[j];
-        r[ii[j]] = si;
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k == mean_row) {
+        nv[k] = o[1];
+        if (ok) nu[mean_row] = nv[k];
       }
-      vs[w][j * 64 + c] = ok[j] ? si : 0.0;
+      si[k] = rv[k] - alpha * nv[k];
     }
-    long io[NB];
-    bool oo[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      io[j] = ii[j];
-      oo[j] = ok[j];
-    }
-    if (g + nw < ng) load_group(g + nw);  // in flight during the product
-    wave_sync();
-    double z[NB];
-    prec_group<NB>(Ps, vs[w], c, z);
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-      if (oo[j]) z2[io[j]] = z[j];
+    if (ok) st4(r, i0, si);
+    tile_put(tile, w, lane, ok, si);
+    const bool oo = ok;
+    const long io = i0;
+    if (t + (int)gridDim.x < ntiles) load(t + gridDim.x);  // in flight during the product
+    __syncthreads();
+    const f64x4 z = tile_prec64(tile, F, lane);
+    __syncthreads();
+    if (oo) st4(z2, io, z);
   }
 }
 
@@ -753,31 +713,25 @@ __global__ __launch_bounds__(WG) void k_meanrow_fix(const double *pm, int G, lon
   if (threadIdx.x == 0) y[mean_row] = (T)o[0];
 }
 
-// z = P r on every block, FP64
-template <int NB>
-__global__ __launch_bounds__(WG) void k_prec(int nb, const double *__restrict__ P,
+// z = P r on every block, FP64 on MFMA (a workgroup per tile of 16 blocks)
+__global__ __launch_bounds__(WG) void k_prec(int nb, const double *__restrict__ pf,
                                              const double *__restrict__ r,
                                              double *__restrict__ z) {
-  __shared__ double Ps[64 * 64];
-  __shared__ double vs[WAVES][NB * 64];
-  load_P(Ps, P);
-  const int c = lane_id(), w = wave_id();
-  const int ng = (nb + NB - 1) / NB, nw = gridDim.x * WAVES;
-  for (int g = blockIdx.x * WAVES + w; g < ng; g += nw) {
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const int bb = g * NB + j;
-      vs[w][j * 64 + c] = bb < nb ? r[(long)bb * 64 + c] : 0.0;
-    }
-    wave_sync();
-    double zz[NB];
-    prec_group<NB>(Ps, vs[w], c, zz);
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const int bb = g * NB + j;
-      if (bb < nb) z[(long)bb * 64 + c] = zz[j];
-    }
+  __shared__ double tile[TILE][TROW];
+  const int lane = lane_id(), w = wave_id();
+  const int ntiles = (nb + TILE - 1) / TILE;
+  PFrag64 F;
+  load_pfrag64(F, pf, w, lane);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    bool ok;
+    const long i0 = tile_cells(t, nb, w, lane, ok);
+    double v[4];
+    ld4(r, i0, v);
+    tile_put(tile, w, lane, ok, v);
+    __syncthreads();
+    const f64x4 zz = tile_prec64(tile, F, lane);
+    __syncthreads();
+    if (ok) st4(z, i0, zz);
   }
 }
 
@@ -823,6 +777,68 @@ __global__ __launch_bounds__(WG) void k_residual(long n, const double *__restric
 }
 
 // ----------------------------------------------------------------------------
+// the unfused comparator: one operation per launch, the structure of the
+// reference GPU loop (BiCGSTAB.cu:412-620), whose BLAS-1 calls, scalar
+// kernels (BiCGSTAB.cu:255-294) and host synchronisations it keeps
+// ----------------------------------------------------------------------------
+
+// device scalars, in the order of BiCGSTABScalars (BiCGSTAB.cuh:12-22);
+// `one` is the constant 1.0 the reference passes as d_eye_
+struct UScal {
+  double alpha, beta, omega, eps, rho_prev, rho_curr, buff_1, buff_2, one;
+};
+
+// partial sums of a.b (ddot) or of a.a (dnrm2, b == a), per workgroup
+__global__ __launch_bounds__(WG) void ku_dot(long n, const double *__restrict__ a,
+                                             const double *__restrict__ b, double *part) {
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) v[0] += a[i] * b[i];
+  store_partials<1, 0u>(v, part);
+}
+// partial maxima of |a| (idamax)
+__global__ __launch_bounds__(WG) void ku_amax(long n, const double *__restrict__ a, double *part) {
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    v[0] = nmax(v[0], fabs(a[i]));
+  store_partials<1, 1u>(v, part);
+}
+// second stage of a reduction: *dst = total (mode 0: sum, 1: sqrt of the
+// sum -- dnrm2 --, 2: max)
+__global__ __launch_bounds__(WG) void ku_total(const double *part, int G, int mode, double *dst) {
+  double o[1];
+  if (mode == 2) load_totals<1, 1u>(part, G, o);
+  else load_totals<1, 0u>(part, G, o);
+  if (threadIdx.x == 0) *dst = mode == 1 ? sqrt(o[0]) : o[0];
+}
+// y += a x, a read from device memory (daxpy with a device scalar)
+__global__ __launch_bounds__(WG) void ku_axpy(long n, const double *a, const double *__restrict__ x,
+                                              double *__restrict__ y) {
+  const double av = *a;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) y[i] += av * x[i];
+}
+// x *= a (dscal)
+__global__ __launch_bounds__(WG) void ku_scal(long n, const double *a, double *__restrict__ x) {
+  const double av = *a;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) x[i] *= av;
+}
+// the reference's one-thread scalar kernels
+__global__ void ku_set_negative(double *dst, const double *src) { *dst = -*src; }
+__global__ void ku_set_squared(double *v) { *v *= *v; }
+__global__ void ku_set_zero(double *v) { *v = 0.0; }
+__global__ void ku_set_beta(UScal *c) {
+  c->beta = (c->rho_curr / (c->rho_prev + c->eps)) * (c->alpha / (c->omega + c->eps));
+}
+__global__ void ku_breakdown_update(UScal *c) {
+  c->rho_prev = 1.;
+  c->alpha = 1.;
+  c->omega = 1.;
+  c->beta = (c->rho_curr / (c->rho_prev + c->eps)) * (c->alpha / (c->omega + c->eps));
+}
+__global__ void ku_set_alpha(UScal *c) { c->alpha = c->rho_curr / (c->buff_1 + c->eps); }
+__global__ void ku_set_omega(UScal *c) { c->omega = c->buff_1 / (c->buff_2 + c->eps); }
+__global__ void ku_set_rho(UScal *c) { c->rho_prev = c->rho_curr; }
+
+// ----------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------
 
@@ -857,7 +873,7 @@ struct cup2d_ps {
   bool has_csr = false;                  // plain CSR copy present (host-assembled meshes)
   int *d_csr_rp = nullptr, *d_csr_col = nullptr;
   double *d_csr_val = nullptr;
-  double *d_P = nullptr;
+  double *d_P = nullptr;                 // FP64 MFMA fragments of P_inv (make_pfrag64)
   bf16x8 *d_pf = nullptr;                // BF16 MFMA fragments of P_inv
   // FP64 work vectors (x is the solution in both precisions)
   double *d_x = nullptr, *d_xopt = nullptr, *d_r = nullptr, *d_rhat = nullptr, *d_p = nullptr,
@@ -867,19 +883,26 @@ struct cup2d_ps {
   float *f_r = nullptr, *f_rhat = nullptr, *f_p = nullptr, *f_nu = nullptr, *f_z1 = nullptr,
         *f_z2 = nullptr, *f_t = nullptr;
   Scal *d_s = nullptr;                   // two state slots (iteration parity)
+  double *d_u = nullptr;                 // scalars of the unfused comparator (UScal + constants)
   double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr, *d_pfin = nullptr;
   Scal *h_s = nullptr;                   // pinned copy of the state slots
   int G = 1;                             // workgroups of the SpMV / vector kernels
   int Gb = 1;                            // of which the block part of the operator kernels
   int Gp = 1;                            // workgroups of the FP64 preconditioner kernels
-  int Gm = 1;                            // workgroups of the MFMA preconditioner kernels
-  int nbp = 2;                           // blocks per wave in the FP64 preconditioner kernels
+  int Gm = 1;                            // workgroups of the BF16 preconditioner kernels
   // graph of one chunk of iterations
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
   // device buffers of the current mesh, and those of earlier meshes kept
   // for reuse: a mesh change of similar size allocates (and frees) nothing
   std::vector<DevBuf> allocs, spare;
+  // per-phase timers (cfg.profile): an event after every launch, tagged
+  // with the launch's phase; the interval since the previous event is the
+  // launch's time
+  std::vector<hipEvent_t> pev;
+  std::vector<int> pph;
+  size_t pused = 0;
+  double pms[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -995,20 +1018,60 @@ OpView view(const cup2d_ps *h) {
   return A;
 }
 
-template <int NB>
+// Profiling (cfg.profile): records an event after the launch just made,
+// tagged with its phase (-1: a start mark).  Errors of the timing calls
+// leave the phase times incomplete, never the solve.
+void ptick(cup2d_ps *h, int ph) {
+  if (!h->cfg.profile) return;
+  if (h->pused == h->pev.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    h->pev.push_back(e);
+    h->pph.push_back(-1);
+  }
+  if (hipEventRecord(h->pev[h->pused], h->stream) != hipSuccess) return;
+  h->pph[h->pused++] = ph;
+}
+// After a synchronisation of the stream: adds every interval to its phase
+// and keeps the last event as the start of the next.
+void pcollect(cup2d_ps *h) {
+  if (!h->cfg.profile) return;
+  for (size_t i = 1; i < h->pused; i++) {
+    float ms = 0;
+    if (h->pph[i] >= 0 && hipEventElapsedTime(&ms, h->pev[i - 1], h->pev[i]) == hipSuccess)
+      h->pms[h->pph[i]] += ms;
+  }
+  if (h->pused > 0) {
+    std::swap(h->pev[0], h->pev[h->pused - 1]);
+    h->pph[0] = -1;
+    h->pused = 1;
+  }
+}
+void pstart(cup2d_ps *h) {
+  for (double &m : h->pms) m = 0.0;
+  h->pused = 0;
+  ptick(h, -1);
+}
+bool graphs_on(const cup2d_ps *h) { return h->cfg.use_graph && !h->cfg.profile; }
+
 void launch_iter_fp64(cup2d_ps *h, int par) {
   const OpView A = view(h);
   const int G = h->G, Gp = h->Gp, nb = (int)h->op.nb;
   const long mr = h->op.mean_row;
   hipStream_t st = h->stream;
-  k_pupdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1,
+  k_pupdate<<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1,
                                    h->d_s, par, h->d_p5, G);
+  ptick(h, CUP2D_PS_PHASE_PREC);
   k_spmv_nu<double><<<G, WG, 0, st>>>(A, h->Gb, h->d_z1, h->d_nu, h->d_rhat, h->d_s, par, h->d_p2);
-  k_supdate<NB><<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
+  ptick(h, CUP2D_PS_PHASE_SPMV);
+  k_supdate<<<Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, par,
                                    h->d_p2, G, mr);
+  ptick(h, CUP2D_PS_PHASE_PREC);
   k_spmv_t<double><<<G, WG, 0, st>>>(A, h->Gb, h->d_z2, h->d_t, h->d_r, h->d_s, par, h->d_p4);
+  ptick(h, CUP2D_PS_PHASE_SPMV);
   k_xrupdate<double><<<G, WG, 0, st>>>(nb, h->d_x, h->d_xopt, h->d_z1, h->d_z2, h->d_r, h->d_t, h->d_rhat,
                                        h->d_s, par, h->d_p4, G, h->d_p5, mr);
+  ptick(h, CUP2D_PS_PHASE_UPDATE);
 }
 
 void launch_iter_mixed(cup2d_ps *h, int par) {
@@ -1018,20 +1081,23 @@ void launch_iter_mixed(cup2d_ps *h, int par) {
   hipStream_t st = h->stream;
   k_pupdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_rhat, h->f_p, h->f_nu, h->f_z1,
                                   h->d_s, par, h->d_p5, G);
+  ptick(h, CUP2D_PS_PHASE_PREC);
   k_spmv_nu<float><<<G, WG, 0, st>>>(A, h->Gb, h->f_z1, h->f_nu, h->f_rhat, h->d_s, par, h->d_p2);
+  ptick(h, CUP2D_PS_PHASE_SPMV);
   k_supdate_mf<<<Gm, WG, 0, st>>>(nb, h->d_pf, h->f_r, h->f_nu, h->f_rhat, h->f_z2, h->d_s, par,
                                   h->d_p2, G, mr);
+  ptick(h, CUP2D_PS_PHASE_PREC);
   k_spmv_t<float><<<G, WG, 0, st>>>(A, h->Gb, h->f_z2, h->f_t, h->f_r, h->d_s, par, h->d_p4);
+  ptick(h, CUP2D_PS_PHASE_SPMV);
   k_xrupdate<float><<<G, WG, 0, st>>>(nb, h->d_x, h->d_xopt, h->f_z1, h->f_z2, h->f_r, h->f_t, h->f_rhat,
                                       h->d_s, par, h->d_p4, G, h->d_p5, mr);
+  ptick(h, CUP2D_PS_PHASE_UPDATE);
 }
 
 // Launches one BiCGSTAB iteration (parity `par` selects the state slot).
 void launch_iteration(cup2d_ps *h, int par) {
   if (h->mixed) launch_iter_mixed(h, par);
-  else if (h->nbp == 4) launch_iter_fp64<4>(h, par);
-  else if (h->nbp == 2) launch_iter_fp64<2>(h, par);
-  else launch_iter_fp64<1>(h, par);
+  else launch_iter_fp64(h, par);
 }
 
 // Iterations per chunk: even, so that every chunk starts on state slot 0.
@@ -1045,6 +1111,7 @@ void launch_chunk(cup2d_ps *h) {
   const int C = chunk_of(h);
   for (int k = 0; k < C; k++) launch_iteration(h, k & 1);
   k_eval<<<1, WG, 0, h->stream>>>(h->d_s, 0, h->d_p5, h->G);
+  ptick(h, CUP2D_PS_PHASE_SCALAR);
 }
 
 int ensure_graph(cup2d_ps *h) {
@@ -1070,13 +1137,13 @@ int ensure_graph(cup2d_ps *h) {
 // one chunk have been launched).
 int run_iterations(cup2d_ps *h, int max_iter) {
   const int C = chunk_of(h);
-  if (h->cfg.use_graph) {
+  if (graphs_on(h)) {
     int rc = ensure_graph(h);
     if (rc) return rc;
   }
   int launched = 0;
   for (;;) {
-    if (h->cfg.use_graph) {
+    if (graphs_on(h)) {
       HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
     } else {
       launch_chunk(h);
@@ -1086,6 +1153,7 @@ int run_iterations(cup2d_ps *h, int max_iter) {
     HIPCHK(h, hipMemcpyAsync(&h->h_s[0].done, &h->d_s[0].done, sizeof(int),
                              hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    pcollect(h);
     if (h->h_s[0].done || launched > max_iter + C) break;
   }
   return CUP2D_PS_OK;
@@ -1131,9 +1199,11 @@ int solve_fp64(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
   k_init<double><<<G, WG, 0, h->stream>>>(view(h), h->Gb, d_b, d_x0, h->d_x, h->d_xopt, h->d_r,
                                           h->d_rhat, h->d_p, h->d_nu, h->d_pm, G, h->d_p5);
   HIPCHK(h, hipGetLastError());
+  ptick(h, CUP2D_PS_PHASE_SETUP);
   int rc = run_iterations(h, h->cfg.max_iter);
   if (rc) return rc;
   if ((rc = finish(h, d_out, 0))) return rc;
+  ptick(h, CUP2D_PS_PHASE_SETUP);
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, 2 * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1193,7 +1263,9 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
     k_init<float><<<G, WG, 0, h->stream>>>(A, h->Gb, d_b, h->d_x, h->d_x, h->d_xopt, h->f_r, h->f_rhat,
                                            h->f_p, h->f_nu, h->d_pm, G, h->d_p5);
     HIPCHK(h, hipGetLastError());
+    ptick(h, CUP2D_PS_PHASE_SETUP);
     int rc = outer_residual(h, err);
+    pcollect(h);
     if (rc) return rc;
     if (err != err) return fail(h, CUP2D_PS_EINVAL, "non-finite residual (NaN/Inf in b or x0)");
     if (err0 < 0) err0 = err;
@@ -1214,6 +1286,7 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
     HIPCHK(h, hipMemcpyAsync(h->d_s, h->h_s, 2 * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
     if ((rc = run_iterations(h, h->cfg.max_iter - total_iters))) return rc;
     k_select_best<<<G, WG, 0, h->stream>>>(nb, h->d_x, h->d_xopt, h->d_s, 0);
+    ptick(h, CUP2D_PS_PHASE_SETUP);
     HIPCHK(h, hipMemcpyAsync(h->h_s, h->d_s, sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     total_iters += h->h_s[0].iter;
@@ -1223,6 +1296,7 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
   }
   int rc = finish(h, d_out, 1);
   if (rc) return rc;
+  ptick(h, CUP2D_PS_PHASE_SETUP);
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (st) {
@@ -1240,13 +1314,210 @@ int solve_mixed(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_ou
   return CUP2D_PS_OK;
 }
 
+// The unfused FP64 comparator (cfg.schedule 1): BiCGSTAB.cu:412-620 step by
+// step, one operation per launch (two for a reduction, as cuBLAS), the
+// scalars on the device, and the reference's blocking host reads: the three
+// reductions of the breakdown test, (rhat, nu), (t, s) with |t|^2, and the
+// stopping norm, every iteration.  Same operator and preconditioner kernels
+// as the fused solver; stopping norm, eps, breakdown tolerance and the
+// iteration cap from the configuration (the reference: infinity norm,
+// 1e-21, 1e-16, 1000).
+int solve_unfused(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out,
+                  double tol_abs, double tol_rel, int max_restarts, cup2d_ps_stats *st) {
+  const long n = (long)h->op.n;
+  const long mr = h->op.mean_row;
+  const size_t bytes = n * sizeof(double);
+  const int Gv = (int)std::min<long>(MAXG, (n + WG - 1) / WG), G = h->G;
+  const OpView A = view(h);
+  hipStream_t s = h->stream;
+  UScal *c = reinterpret_cast<UScal *>(h->d_u);
+  double *hr = reinterpret_cast<double *>(h->h_s);  // pinned staging of scalar reads
+  double *x = h->d_x, *xopt = h->d_xopt, *r = h->d_r, *rhat = h->d_rhat, *p = h->d_p,
+         *nu = h->d_nu, *z = h->d_z1, *t = h->d_t, *part = h->d_pm;
+  auto spmv = [&](const double *in, double *out) {  // hd_cusparseSpMV, BiCGSTAB.cu:315-410
+    k_spmv<double><<<G, WG, 0, s>>>(A, h->Gb, in, out, h->d_pm);
+    if (mr >= 0) k_meanrow_fix<double><<<1, WG, 0, s>>>(h->d_pm, G, mr, out);
+    ptick(h, CUP2D_PS_PHASE_SPMV);
+  };
+  auto prec = [&](const double *in, double *out) {  // cublasDgemm, BiCGSTAB.cu:520, :545
+    k_prec<<<h->Gp, WG, 0, s>>>((int)h->op.nb, h->d_P, in, out);
+    ptick(h, CUP2D_PS_PHASE_PREC);
+  };
+  auto reduce = [&](const double *a, const double *b, int mode, double *dst) {
+    if (mode == 2) ku_amax<<<Gv, WG, 0, s>>>(n, a, part);
+    else ku_dot<<<Gv, WG, 0, s>>>(n, a, b, part);
+    ku_total<<<1, WG, 0, s>>>(part, Gv, mode, dst);
+    ptick(h, CUP2D_PS_PHASE_UPDATE);
+  };
+  auto axpy = [&](const double *a, const double *xv, double *yv) {
+    ku_axpy<<<Gv, WG, 0, s>>>(n, a, xv, yv);
+    ptick(h, CUP2D_PS_PHASE_UPDATE);
+  };
+  auto scalar = [&]() { ptick(h, CUP2D_PS_PHASE_SCALAR); };
+  auto read = [&](const double *src, int count) -> int {  // blocking read of device scalars
+    HIPCHK(h, hipMemcpyAsync(hr, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    return CUP2D_PS_OK;
+  };
+  const int norm_mode = h->cfg.norm ? 1 : 2;  // dnrm2 or idamax of r
+  HIPCHK(h, hipEventRecord(h->ev0, s));
+  const UScal c0{1., 1., 1., h->cfg.eps, 1., 1., 0., 0., 1.};
+  HIPCHK(h, hipMemcpyAsync(c, &c0, sizeof(UScal), hipMemcpyHostToDevice, s));
+  // 1. r = b - A x0 (b's mean row is 0, ExpAMRSolver.cpp:340-345)
+  HIPCHK(h, hipMemcpyAsync(x, d_x0, bytes, hipMemcpyDeviceToDevice, s));
+  spmv(x, nu);
+  HIPCHK(h, hipMemcpyAsync(r, d_b, bytes, hipMemcpyDeviceToDevice, s));
+  if (mr >= 0) ku_set_zero<<<1, 1, 0, s>>>(r + mr);
+  ku_set_negative<<<1, 1, 0, s>>>(&c->buff_2, &c->one);  // the reference's d_nye_ = -1
+  axpy(&c->buff_2, nu, r);
+  reduce(r, r, norm_mode, &c->buff_1);
+  int rc = read(&c->buff_1, 1);
+  if (rc) return rc;
+  const double error_init = hr[0];
+  double error = error_init, error_opt = error_init;
+  if (error != error) return fail(h, CUP2D_PS_EINVAL, "non-finite residual (NaN/Inf in b or x0)");
+  HIPCHK(h, hipMemcpyAsync(xopt, x, bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(rhat, r, bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(h, hipMemsetAsync(nu, 0, bytes, s));
+  HIPCHK(h, hipMemsetAsync(p, 0, bytes, s));
+  ptick(h, CUP2D_PS_PHASE_SETUP);
+  bool converged = error == 0.0;
+  int restarts = 0, iters = 0;
+  for (int k = 0; k < h->cfg.max_iter && !converged; k++) {
+    // rho = (rhat, r); |r|^2 and |rhat|^2 for the breakdown test
+    reduce(rhat, r, 0, &c->rho_curr);
+    reduce(r, r, 1, &c->buff_1);
+    reduce(rhat, rhat, 1, &c->buff_2);
+    if ((rc = read(&c->rho_curr, 3))) return rc;  // rho_curr, buff_1, buff_2 are contiguous
+    const double rho = hr[0], r2 = hr[1] * hr[1], rh2 = hr[2] * hr[2];
+    const bool breakdown = rho * rho < h->cfg.breakdown_tol * r2 * rh2;
+    ku_set_beta<<<1, 1, 0, s>>>(c);
+    scalar();
+    if (breakdown && max_restarts > 0) {
+      restarts++;
+      if (restarts >= max_restarts) break;
+      HIPCHK(h, hipMemcpyAsync(rhat, r, bytes, hipMemcpyDeviceToDevice, s));
+      reduce(rhat, rhat, 1, &c->rho_curr);
+      ku_set_squared<<<1, 1, 0, s>>>(&c->rho_curr);
+      HIPCHK(h, hipMemsetAsync(nu, 0, bytes, s));
+      HIPCHK(h, hipMemsetAsync(p, 0, bytes, s));
+      ku_breakdown_update<<<1, 1, 0, s>>>(c);
+      scalar();
+    }
+    // p = r + beta (p - omega nu)
+    ku_set_negative<<<1, 1, 0, s>>>(&c->buff_1, &c->omega);
+    scalar();
+    axpy(&c->buff_1, nu, p);
+    ku_scal<<<Gv, WG, 0, s>>>(n, &c->beta, p);
+    ptick(h, CUP2D_PS_PHASE_UPDATE);
+    axpy(&c->one, r, p);
+    prec(p, z);
+    spmv(z, nu);
+    // alpha = rho / (rhat, nu)
+    reduce(rhat, nu, 0, &c->buff_1);
+    if ((rc = read(&c->buff_1, 1))) return rc;
+    ku_set_alpha<<<1, 1, 0, s>>>(c);
+    scalar();
+    axpy(&c->alpha, z, x);
+    ku_set_negative<<<1, 1, 0, s>>>(&c->buff_1, &c->alpha);
+    scalar();
+    axpy(&c->buff_1, nu, r);  // s = r - alpha nu, in r
+    prec(r, z);
+    spmv(z, t);
+    // omega = (t, s) / (t, t)
+    reduce(t, r, 0, &c->buff_1);
+    reduce(t, t, 1, &c->buff_2);
+    ku_set_squared<<<1, 1, 0, s>>>(&c->buff_2);
+    scalar();
+    if ((rc = read(&c->buff_1, 2))) return rc;
+    ku_set_omega<<<1, 1, 0, s>>>(c);
+    scalar();
+    axpy(&c->omega, z, x);
+    ku_set_negative<<<1, 1, 0, s>>>(&c->buff_1, &c->omega);
+    scalar();
+    axpy(&c->buff_1, t, r);
+    reduce(r, r, norm_mode, &c->buff_1);
+    if ((rc = read(&c->buff_1, 1))) return rc;
+    pcollect(h);
+    error = hr[0];
+    iters = k + 1;
+    if (error < error_opt) {
+      error_opt = error;
+      HIPCHK(h, hipMemcpyAsync(xopt, x, bytes, hipMemcpyDeviceToDevice, s));
+      if (error <= tol_abs || error / error_init <= tol_rel) converged = true;
+    }
+    ku_set_rho<<<1, 1, 0, s>>>(c);
+    scalar();
+  }
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(x, xopt, bytes, hipMemcpyDeviceToDevice, s));
+  if ((rc = finish(h, d_out, 1))) return rc;
+  ptick(h, CUP2D_PS_PHASE_SETUP);
+  HIPCHK(h, hipEventRecord(h->ev1, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->iterations = iters;
+    st->restarts = restarts;
+    st->converged = converged;
+    st->err_init = error_init;
+    st->err_final = error_opt;
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    st->ms = ms;
+  }
+  return CUP2D_PS_OK;
+}
+
+// |b - A x|_2, |b - A x|_inf, |b|_2, |b|_inf in FP64 on the device (b's
+// mean row taken as 0); uses the work vector t and the final partials.
+int residual_norms(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]) {
+  k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), h->Gb, d_x, h->d_t, h->d_pm);
+  if (h->op.mean_row >= 0)
+    k_meanrow_fix<double><<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, h->d_t);
+  k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pfin);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> part(4 * (size_t)h->G);
+  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_pfin, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double r2 = 0, rinf = 0, b2 = 0, binf = 0;
+  for (int i = 0; i < h->G; i++) {
+    r2 += part[i];
+    rinf = std::max(rinf, part[h->G + i]);
+    b2 += part[2 * h->G + i];
+    binf = std::max(binf, part[3 * h->G + i]);
+  }
+  out[0] = std::sqrt(r2);
+  out[1] = rinf;
+  out[2] = std::sqrt(b2);
+  out[3] = binf;
+  return CUP2D_PS_OK;
+}
+
 int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out, double tol_abs,
                double tol_rel, int max_restarts, cup2d_ps_stats *st) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "solve before cup2d_ps_set_mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  return guarded(h, [&] {
-    return h->mixed ? solve_mixed(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st)
-                    : solve_fp64(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st);
+  return guarded(h, [&]() -> int {
+    pstart(h);
+    int rc = h->cfg.schedule == 1
+                 ? solve_unfused(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st)
+                 : h->mixed ? solve_mixed(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st)
+                            : solve_fp64(h, d_b, d_x0, d_out, tol_abs, tol_rel, max_restarts, st);
+    if (rc) return rc;
+    // the returned solution's true residual, recomputed in FP64
+    double nr[4];
+    if ((rc = residual_norms(h, d_b, d_out, nr))) return rc;
+    ptick(h, CUP2D_PS_PHASE_SETUP);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    pcollect(h);
+    if (st) {
+      st->true_rel2 = nr[2] > 0 ? nr[0] / nr[2] : 0.0;
+      st->true_relinf = nr[3] > 0 ? nr[1] / nr[3] : 0.0;
+      for (int k = 0; k < 5; k++) st->ms_phase[k] = h->cfg.profile ? h->pms[k] : 0.0;
+    }
+    return CUP2D_PS_OK;
   });
 }
 
@@ -1276,7 +1547,9 @@ static int create_body(const cup2d_ps_config *cfg, cup2d_ps **out) {
   else cup2d_ps_config_default(&c);
   if (c.max_iter <= 0 || c.chunk < 0 || c.eps < 0 || c.breakdown_tol < 0 || c.inner_tol < 0 ||
       c.inner_tol >= 1 || (c.precision != CUP2D_PS_FP64 && c.precision != CUP2D_PS_MIXED) ||
-      (c.norm != CUP2D_PS_NORM_INF && c.norm != CUP2D_PS_NORM_L2))
+      (c.norm != CUP2D_PS_NORM_INF && c.norm != CUP2D_PS_NORM_L2) || (c.profile != 0 && c.profile != 1) ||
+      c.item_blocks < 0 || (c.schedule != 0 && c.schedule != 1) ||
+      (c.schedule == 1 && c.precision != CUP2D_PS_FP64))
     return fail(nullptr, CUP2D_PS_EINVAL, "invalid configuration");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -1310,6 +1583,7 @@ int cup2d_ps_destroy(cup2d_ps *h) {
   for (const DevBuf &b : h->spare) (void)hipFree(b.p);
   h->spare.clear();
   if (h->h_s) (void)hipHostFree(h->h_s);
+  for (hipEvent_t e : h->pev) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1375,7 +1649,8 @@ static int finish_mesh(cup2d_ps *h) {
   int rc = 0;
   static const std::vector<double> P = make_pinv();  // the same for every mesh
   static const std::vector<uint16_t> pf = make_pfrag(P);
-  if ((rc = upload(h, &h->d_P, P)) || (rc = dalloc(h, &h->d_pf, 8 * 64))) return rc;
+  static const std::vector<double> pf64 = make_pfrag64(P);
+  if ((rc = upload(h, &h->d_P, pf64)) || (rc = dalloc(h, &h->d_pf, 8 * 64))) return rc;
   HIPCHK(h, hipMemcpy(h->d_pf, pf.data(), pf.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   const size_t n = op.n;
   double **vecs[] = {&h->d_x, &h->d_xopt, &h->d_r, &h->d_rhat, &h->d_p, &h->d_nu,
@@ -1386,7 +1661,7 @@ static int finish_mesh(cup2d_ps *h) {
   for (float **v : fvecs)
     if ((rc = dalloc(h, v, h->mixed ? n : 64))) return rc;
   const size_t GT = MAXG + MAXGE;
-  if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_p2, 2 * GT)) ||
+  if ((rc = dalloc(h, &h->d_s, 2)) || (rc = dalloc(h, &h->d_u, 16)) || (rc = dalloc(h, &h->d_p2, 2 * GT)) ||
       (rc = dalloc(h, &h->d_p4, 4 * GT)) || (rc = dalloc(h, &h->d_p5, NQ5 * GT)) ||
       (rc = dalloc(h, &h->d_pm, GT)) || (rc = dalloc(h, &h->d_pfin, 4 * GT)))
     return rc;
@@ -1395,11 +1670,8 @@ static int finish_mesh(cup2d_ps *h) {
   h->Gb = (int)std::max<long long>(1, std::min<long long>(MAXG, (waves + WAVES - 1) / WAVES));
   const int Ge = h->n_exc > 0 ? (int)std::min<long long>(MAXGE, (h->n_exc + WG - 1) / WG) : 0;
   h->G = h->Gb + Ge;
-  h->nbp = h->cfg.reserved_i[0] > 0 ? h->cfg.reserved_i[0] : (op.nb >= 8192 ? 4 : 2);
-  if (h->nbp != 1 && h->nbp != 2 && h->nbp != 4) h->nbp = 2;
-  const long long groups = (op.nb + h->nbp - 1) / h->nbp;
-  h->Gp = (int)std::max<long long>(1, std::min<long long>(MAXG, (groups + WAVES - 1) / WAVES));
   const long long tiles = (op.nb + TILE - 1) / TILE;
+  h->Gp = (int)std::max<long long>(1, std::min<long long>(MAXGP, tiles));
   h->Gm = (int)std::max<long long>(1, std::min<long long>(MAXG, (tiles + WAVES - 1) / WAVES));
   trim_spare(h);
   h->have_mesh = true;
@@ -1525,6 +1797,9 @@ static int pressure_rhs_body(cup2d_ps *h, const double *d_u, const double *d_v, 
                              const double *d_vd, const double *d_chi, double dt, double *d_rhs) {
   if (!h || !d_u || !d_v || !d_ud || !d_vd || !d_chi || !d_rhs)
     return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  // the kernel reads neighbours of every input while writing rhs
+  if (d_rhs == d_u || d_rhs == d_v || d_rhs == d_ud || d_rhs == d_vd || d_rhs == d_chi)
+    return fail(h, CUP2D_PS_EINVAL, "rhs must not alias an input");
   if (const int rc = projection_checks(h, dt)) return rc;
   HIPCHK(h, hipSetDevice(h->device));
   k_pressure_rhs<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_u, d_v, d_ud,
@@ -1652,16 +1927,12 @@ static int kernel_time_body(cup2d_ps *h, int which, int reps, double *ms_per_lau
     } else {
       switch (which) {
         case 0:
-          if (h->nbp == 4) k_pupdate<4><<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1, h->d_s, 1, h->d_p5, G);
-          else if (h->nbp == 2) k_pupdate<2><<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1, h->d_s, 1, h->d_p5, G);
-          else k_pupdate<1><<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1, h->d_s, 1, h->d_p5, G);
+          k_pupdate<<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1, h->d_s, 1, h->d_p5, G);
           break;
         case 1: k_spmv_nu<double><<<G, WG, 0, st>>>(A, h->Gb, h->d_z1, h->d_nu, h->d_rhat, h->d_s, 1,
                                                     h->d_p2); break;
         case 2:
-          if (h->nbp == 4) k_supdate<4><<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, 1, h->d_p2, G, mr);
-          else if (h->nbp == 2) k_supdate<2><<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, 1, h->d_p2, G, mr);
-          else k_supdate<1><<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, 1, h->d_p2, G, mr);
+          k_supdate<<<h->Gp, WG, 0, st>>>(nb, h->d_P, h->d_r, h->d_nu, h->d_rhat, h->d_z2, h->d_s, 1, h->d_p2, G, mr);
           break;
         case 3: k_spmv_t<double><<<G, WG, 0, st>>>(A, h->Gb, h->d_z2, h->d_t, h->d_r, h->d_s, 1,
                                                    h->d_p4); break;
@@ -1688,9 +1959,7 @@ int cup2d_ps_precond_device(cup2d_ps *h, const double *d_r, double *d_z) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
   const int nb = (int)h->op.nb;
-  if (h->nbp == 4) k_prec<4><<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
-  else if (h->nbp == 2) k_prec<2><<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
-  else k_prec<1><<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
+  k_prec<<<h->Gp, WG, 0, h->stream>>>(nb, h->d_P, d_r, d_z);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
@@ -1710,27 +1979,7 @@ static int residual_body(cup2d_ps *h, const double *d_b, const double *d_x, doub
   if (!h || !d_b || !d_x || !out) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
   HIPCHK(h, hipSetDevice(h->device));
-  k_spmv<double><<<h->G, WG, 0, h->stream>>>(view(h), h->Gb, d_x, h->d_t, h->d_pm);
-  if (h->op.mean_row >= 0)
-    k_meanrow_fix<double><<<1, WG, 0, h->stream>>>(h->d_pm, h->G, h->op.mean_row, h->d_t);
-  k_residual<<<h->G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pfin);
-  HIPCHK(h, hipGetLastError());
-  std::vector<double> part(4 * (size_t)h->G);
-  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_pfin, part.size() * sizeof(double),
-                           hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  double r2 = 0, rinf = 0, b2 = 0, binf = 0;
-  for (int i = 0; i < h->G; i++) {
-    r2 += part[i];
-    rinf = std::max(rinf, part[h->G + i]);
-    b2 += part[2 * h->G + i];
-    binf = std::max(binf, part[3 * h->G + i]);
-  }
-  out[0] = std::sqrt(r2);
-  out[1] = rinf;
-  out[2] = std::sqrt(b2);
-  out[3] = binf;
-  return CUP2D_PS_OK;
+  return residual_norms(h, d_b, d_x, out);
 }
 
 int cup2d_ps_sizes(const cup2d_ps *h, long long *n, long long *nnz, long long *n_explicit,

@@ -408,6 +408,14 @@ class BatchSolver:
                                                          ctypes.byref(ms)))
         return ms.value
 
+    def kernel_time(self, which, reps=20):
+        """ms per launch of batched iteration kernel `which` with every
+        instance iterating (0 p update + prec, 1 nu = A z1, 2 s update + prec,
+        3 t = A z2, 4 x/r update, 5 alpha, 6 omega, 7 state evaluation)."""
+        ms = ctypes.c_double()
+        _check_b(self.h, _lib.load().cup2d_psb_kernel_time(self.h, int(which), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def copy_time(self, d_x: DeviceArray, d_y: DeviceArray, reps=50):
         """ms per launch of the library's streaming copy d_y = d_x over n."""
         ms = ctypes.c_double()
@@ -472,7 +480,8 @@ class HipPoissonSolver(PoissonSolver):
                                             mean_constraint=int(sim.bMeanConstraint != 0), norm=norm,
                                             max_iter=int(sim.maxPoissonIterations), **cfg))
         self.last_stats = None
-        self._mesh_id = None
+        self._mesh = None          # the mesh whose operator the handle holds
+        self.assemblies = 0        # operator assemblies so far
 
     def solve(self, input: ScalarGrid, output: ScalarGrid) -> None:  # noqa: A002
         sim = self.sim
@@ -480,12 +489,15 @@ class HipPoissonSolver(PoissonSolver):
         tol = 0.0 if sim.step < 10 else sim.PoissonTol
         rel = 0.0 if sim.step < 10 else sim.PoissonTolRel
         restarts = 100 if sim.step < 10 else sim.maxPoissonRestarts
-        if output.UpdateFluxCorrection or self._mesh_id is not id(output.mesh):
+        # the mesh is held by reference: an object identity test cannot be
+        # fooled by an id() reused after garbage collection
+        if output.UpdateFluxCorrection or self._mesh is not output.mesh:
             output.UpdateFluxCorrection = False
             # the operator is assembled by GPU kernels (the same matrix as the
             # host assembly, bit for bit); only the block list crosses PCIe
             self.solver.set_mesh(output.mesh, on_device=True)
-            self._mesh_id = id(output.mesh)
+            self._mesh = output.mesh
+            self.assemblies += 1
         if input.data.shape != output.data.shape:
             raise ValueError("input and output grids differ")
         b = input.data.reshape(-1)

@@ -64,7 +64,17 @@ typedef struct {
   int max_iter;           /* iteration cap (reference GPU: 1000, BiCGSTAB.cu:468) */
   int chunk;              /* iterations per captured graph launch; 0 = automatic */
   int use_graph;          /* 1 = replay iteration chunks as a hipGraph */
-  int reserved_i[5];
+  int profile;            /* 1 = per-phase device timers (the reference's DeviceProfiler,
+                             DeviceProfiler.cuh:26-107, used at BiCGSTAB.cu:191-619): kernels are
+                             launched one by one between HIP events and the phase times are
+                             returned in cup2d_ps_stats.ms_phase; 0 = off (graph replay) */
+  int item_blocks;        /* batch handles: blocks per work item, 1..256; 0 = the value tuned in
+                             this process for the batch's shape, else 8 */
+  int schedule;           /* 0 = fused kernels (default); 1 = one operation per launch with a
+                             host check of the stopping norm every iteration, the structure of the
+                             reference GPU loop (BiCGSTAB.cu:412-620), FP64 single solves only: the
+                             unfused comparator */
+  int reserved_i[2];
   double eps;             /* denominator guard; reference GPU 1e-21 (BiCGSTAB.cu:427),
                              reference CPU 1e-100 (AMRSolver.cpp:167) */
   double breakdown_tol;   /* restart when rho^2 < breakdown_tol |r|^2 |rhat|^2 (BiCGSTAB.cu:483) */
@@ -82,8 +92,24 @@ typedef struct {
   double err_init;        /* stopping norm of r0 = b - A x0 */
   double err_final;       /* stopping norm of the returned iterate (recurrence residual) */
   double ms;              /* wall time of the solve on the handle's stream, ms */
-  double reserved[5];
+  double true_rel2;       /* |b - A x|_2 / |b|_2 of the returned x, recomputed in FP64 on the device
+                             after the solve (b's mean row taken as 0 with mean_constraint) */
+  double true_relinf;     /* |b - A x|_inf / |b|_inf, likewise */
+  double ms_phase[5];     /* with cfg.profile: device ms per phase, indexed by cup2d_ps_phase;
+                             0 otherwise */
+  double reserved[2];
 } cup2d_ps_stats;
+
+/* Phases of cup2d_ps_stats.ms_phase (the reference profiler's tags "KerSpMV",
+ * "Prec", ... at BiCGSTAB.cu:191-619).  A fused kernel counts in the phase of
+ * its main operation: the preconditioner kernels also update p or s. */
+typedef enum {
+  CUP2D_PS_PHASE_SPMV = 0,    /* A z (with the dot products fused into it) */
+  CUP2D_PS_PHASE_PREC = 1,    /* P_inv r (with the p / s update fused into it) */
+  CUP2D_PS_PHASE_UPDATE = 2,  /* x and r updates, other vector passes and dot products */
+  CUP2D_PS_PHASE_SCALAR = 3,  /* scalar recurrences and stopping tests */
+  CUP2D_PS_PHASE_SETUP = 4    /* r0 = b - A x0, residual replacements, mean removal, true residual */
+} cup2d_ps_phase;
 
 /* Fills the defaults: FP64, mean_constraint 0, NORM_INF, max_iter 1000,
  * eps 1e-100, breakdown_tol 1e-16, inner_tol 0, graph replay on. */
@@ -221,9 +247,8 @@ const char *cup2d_ps_last_error(const cup2d_ps *h);
  * instance at once, each instance with its own scalars, stopping test,
  * restarts and (mixed precision) FP64 refinement passes.  Vectors are the
  * instances' vectors concatenated in instance order (offsets from
- * cup2d_psb_offsets).  Config fields as for cup2d_ps_create; reserved_i[1]
- * sets the blocks per work item, 1..256 (0: tuned value of this process,
- * else 8).
+ * cup2d_psb_offsets).  Config fields as for cup2d_ps_create (item_blocks
+ * sets the blocks per work item; schedule must be 0).
  * ------------------------------------------------------------------------ */
 typedef struct cup2d_psb cup2d_psb;
 
@@ -284,6 +309,15 @@ int cup2d_psb_item_blocks(const cup2d_psb *h);
 int cup2d_psb_spmv_device(cup2d_psb *h, const double *d_x, double *d_y);
 int cup2d_psb_spmv_time(cup2d_psb *h, const double *d_x, double *d_y, int reps,
                         double *ms_per_launch);
+/* Times one of the batched iteration kernels alone with every instance
+ * iterating: `reps` launches back to back on the handle's stream (HIP events,
+ * mean ms per launch), each redoing one iteration's work from the same state
+ * (set up from x = 0 and a fixed b; the handle's work vectors are
+ * overwritten, the next solve starts afresh).  which: 0 p update +
+ * preconditioner, 1 nu = A z1, 2 s update + preconditioner, 3 t = A z2,
+ * 4 x and r update, 5 alpha, 6 omega, 7 state evaluation.  No counterpart in
+ * the reference (DeviceProfiler.cuh:26-107 times the reference's phases). */
+int cup2d_psb_kernel_time(cup2d_psb *h, int which, int reps, double *ms_per_launch);
 /* d_y = d_x over the batch's unknowns with the library's streaming copy
  * kernel, timed over `reps` launches (HIP events, ms per launch): the
  * bandwidth ceiling of the solver's vector access pattern, and the kernel

@@ -99,7 +99,7 @@ def test_batch_create_validates_config_before_device():
             "def rc(**kw):\n"
             "    c = PS.default_config()\n"
             "    for k, v in kw.items():\n"
-            "        if k == 'ib': c.reserved_i[1] = v\n"
+            "        if k == 'ib': c.item_blocks = v\n"
             "        else: setattr(c, k, v)\n"
             "    h = ctypes.c_void_p()\n"
             "    return lib.cup2d_psb_create(ctypes.byref(c), ctypes.byref(h))\n"
@@ -134,3 +134,28 @@ def test_batch_projection_null_handle_is_einval():
     lib = _lib.load()
     assert lib.cup2d_psb_pressure_rhs_device(None, None, None, None, None, None, 1.0, None) == 1
     assert lib.cup2d_psb_pressure_correct_device(None, None, 1.0, None, None) == 1
+
+
+@pytest.mark.parametrize("lang", ["c99", "c++17"])
+def test_header_compiles_and_links_from_c(lang, tmp_path):
+    """A translation unit that includes include/cup2d_poisson.h, compiled as
+    C and as C++ with warnings as errors, links every declared entry point
+    against the library and runs its host-only calls (config defaults, the
+    host assembly, error reporting) without a GPU."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "c_abi", "abi_check.c")
+    exe = str(tmp_path / "abi_check")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    if lang == "c99":
+        cmd = ["gcc", "-std=c99", "-x", "c"]
+    else:
+        cmd = ["g++", "-std=c++17", "-x", "c++"]
+    cmd += ["-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(ROOT, "include"), src,
+            "-L", libdir, "-lcup2d_poisson", "-Wl,-rpath," + libdir, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().startswith("ABI OK 592"), r.stdout + r.stderr
+    # every entry point of the header is named in the C file
+    names = set(re.findall(r"&(cup2d_\w+)", open(src).read()))
+    assert names == set(_declared("cup2d_poisson.h"))

@@ -45,7 +45,7 @@ def oracle_solutions(oracle, batch_meshes, batch_rhs):
 
 def make_batch(meshes, precision=PS.FP64, item_blocks=0, **cfg):
     c = PS.default_config(precision=precision, max_iter=4000, **cfg)
-    c.reserved_i[1] = item_blocks
+    c.item_blocks = item_blocks
     return PS.BatchSolver(c).set_meshes(meshes)
 
 
@@ -212,3 +212,30 @@ def test_batch_copy_time_copies(batch_meshes):
     assert np.array_equal(d_y.numpy(), x)
     with pytest.raises(PS._lib.SolverError):
         bs.copy_time(d_x, d_y, reps=0)
+
+
+@pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
+def test_batch_kernel_time_and_stats(oracle, batch_meshes, batch_rhs, precision):
+    """Each batched iteration kernel can be timed alone; the handle solves
+    correctly afterwards, and every instance's stats carry its FP64 true
+    residual (as the oracle recomputes it) and, with cfg.profile, the phase
+    times of the batch."""
+    bs = make_batch(batch_meshes, precision=precision, norm=PS.NORM_L2)
+    for k in range(8):
+        ms = bs.kernel_time(k, reps=3)
+        assert 0.0 < ms < 100.0, (k, ms)
+    with pytest.raises(PS._lib.SolverError):
+        bs.kernel_time(8)
+    b = np.concatenate(batch_rhs)
+    x, st = bs.solve(b, rel_tol=1e-10)
+    for k, (m, bk) in enumerate(zip(batch_meshes, batch_rhs)):
+        tr = oracle.true_residual(oracle.assemble(m), bk, bs.split(x)[k])
+        assert st[k]["converged"] and tr["rel2"] <= 1.05e-10
+        assert abs(st[k]["true_rel2"] - tr["rel2"]) <= 1e-3 * tr["rel2"] + 1e-16
+        assert abs(st[k]["true_relinf"] - tr["relinf"]) <= 1e-3 * tr["relinf"] + 1e-16
+    bp = make_batch(batch_meshes, precision=precision, norm=PS.NORM_L2, profile=1)
+    xp, stp = bp.solve(b, rel_tol=1e-10)
+    assert np.array_equal(xp, x)
+    ph = stp[0]["ms_phase"]
+    assert ph["spmv"] > 0 and ph["prec"] > 0 and ph["scalar"] > 0
+    assert 0.5 * stp[0]["ms"] <= sum(ph.values()) <= 1.5 * stp[0]["ms"] + 1.0, (ph, stp[0]["ms"])

@@ -55,6 +55,16 @@ def test_projection_operators_reject_coarse_fine_and_bad_dt():
         s.pressure_rhs(z, z, z, z, z, 0.0)
     with pytest.raises(_lib.SolverError):
         s.pressure_correct(z, z, z, -1.0)
+    # rhs written in place over an input is rejected (single and batch)
+    d = [PS.DeviceArray.from_host(s, z) for _ in range(5)]
+    for k in range(5):
+        with pytest.raises(_lib.SolverError, match="alias"):
+            s.pressure_rhs_device(*d, 1.0, d[k])
+    bs = PS.BatchSolver().set_meshes([u, u])
+    db = [bs.device_array(np.zeros(bs.n)) for _ in range(5)]
+    lib = _lib.load()
+    for k in range(5):
+        assert lib.cup2d_psb_pressure_rhs_device(bs.h, *[a.ptr for a in db], 1.0, db[k].ptr) == 1
 
 
 def test_projection_round_trip_reduces_divergence():

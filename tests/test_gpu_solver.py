@@ -219,6 +219,20 @@ def test_poisson_solver_interface(oracle):
     assert tr["relinf"] <= 1.5e-10
     # input untouched
     assert np.array_equal(sim.tmp.data.reshape(-1), b)
+    # the operator is reused while the mesh is unchanged (ExpAMRSolver.cpp:370-380)
+    assert solver.assemblies == 1
+    solver.solve(sim.tmp, sim.pres)
+    assert solver.assemblies == 1 and solver.last_stats["converged"]
+    # a new mesh object (the same blocks) and the mesh-changed flag re-assemble
+    m2 = W.graded_disk_mesh()
+    sim.pres = PS.ScalarGrid(m2, UpdateFluxCorrection=False)
+    solver.solve(sim.tmp, sim.pres)
+    assert solver.assemblies == 2
+    sim.pres.UpdateFluxCorrection = True
+    solver.solve(sim.tmp, sim.pres)
+    assert solver.assemblies == 3 and not sim.pres.UpdateFluxCorrection
+    tr = oracle.true_residual(A, b, sim.pres.data.reshape(-1))
+    assert tr["relinf"] <= 1.5e-10
 
 
 @pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
@@ -237,3 +251,62 @@ def test_kernel_time_then_solve(oracle, precision):
     assert tr["rel2"] <= 1.05e-10
     with pytest.raises(PS._lib.SolverError):
         s.kernel_time(5)
+
+
+def test_unfused_comparator_matches_oracle(oracle):
+    """schedule=1: the one-operation-per-launch FP64 loop of the reference
+    (BiCGSTAB.cu:412-620) reaches the same solution as the oracle; in the
+    infinity norm with the reference's own stopping test."""
+    for name in ("graded", "tiny_cf"):
+        m = MESHES[name]()
+        A = oracle.assemble(m)
+        b = W.vortex_rhs(m, 17)
+        xr, _ = oracle.bicgstab(A, b, rel_tol=1e-13, norm="l2", max_iter=6000)
+        xr = oracle.mean_remove(oracle.h2(m), xr)
+        for norm in (PS.NORM_L2, PS.NORM_INF):
+            s = PS.Solver(schedule=1, norm=norm, max_iter=4000).set_mesh(m)
+            x, st = s.solve(b, rel_tol=1e-12)
+            tr = oracle.true_residual(A, b, x)
+            assert st["converged"], (name, st)
+            assert (tr["rel2"] if norm == PS.NORM_L2 else tr["relinf"]) <= 1.05e-12, tr
+            assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+            # the stats' own true residual agrees with the oracle's
+            assert abs(st["true_rel2"] - tr["rel2"]) <= 1e-3 * tr["rel2"] + 1e-16
+    # the mean-row form
+    m = MESHES["graded"]()
+    s1 = PS.Solver(schedule=1, mean_constraint=1, norm=PS.NORM_L2, max_iter=4000).set_mesh(m)
+    b = W.vortex_rhs(m, 18)
+    x, st = s1.solve(b, rel_tol=1e-10)
+    A1 = oracle.assemble(m, 1)
+    bm = b.copy()
+    bm[A1.mean_row] = 0.0
+    tr = oracle.true_residual(A1, bm, x, mean_mode=1, h2v=oracle.h2(m))
+    assert st["converged"] and tr["rel2"] <= 1.05e-10
+    with pytest.raises(PS._lib.SolverError):
+        PS.Solver(schedule=1, precision=PS.MIXED)
+
+
+@pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
+def test_profile_phases_and_true_residual(oracle, precision):
+    """cfg.profile: per-phase device times that add up to the solve, and the
+    same solution as the graph-replayed solve; every solve reports its FP64
+    true residual."""
+    m = MESHES["graded"]()
+    A = oracle.assemble(m)
+    b = W.vortex_rhs(m, 19)
+    cfg = dict(precision=precision, norm=PS.NORM_L2, max_iter=4000)
+    x0, st0 = PS.Solver(**cfg).set_mesh(m).solve(b, rel_tol=1e-10)
+    x1, st1 = PS.Solver(profile=1, **cfg).set_mesh(m).solve(b, rel_tol=1e-10)
+    assert np.array_equal(x0, x1)
+    assert all(v == 0.0 for v in st0["ms_phase"].values())
+    ph = st1["ms_phase"]
+    assert ph["spmv"] > 0 and ph["prec"] > 0 and ph["update"] > 0 and ph["setup"] > 0
+    assert 0.5 * st1["ms"] <= sum(ph.values()) <= 1.5 * st1["ms"] + 1.0, (ph, st1["ms"])
+    tr = oracle.true_residual(A, b, x0)
+    for st in (st0, st1):
+        assert abs(st["true_rel2"] - tr["rel2"]) <= 1e-3 * tr["rel2"] + 1e-16
+        assert abs(st["true_relinf"] - tr["relinf"]) <= 1e-3 * tr["relinf"] + 1e-16
+    # the unfused comparator's phases
+    if precision == PS.FP64:
+        _, st2 = PS.Solver(profile=1, schedule=1, **cfg).set_mesh(m).solve(b, rel_tol=1e-10)
+        assert st2["converged"] and st2["ms_phase"]["spmv"] > 0 and st2["ms_phase"]["scalar"] > 0

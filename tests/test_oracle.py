@@ -250,3 +250,57 @@ def test_mean_remove_closed_form(oracle):
     xm = oracle.mean_remove(h2, x)
     assert abs(np.dot(np.repeat(h2, 64), xm)) < 1e-13
     assert np.allclose(x - xm, (x - xm)[0])
+
+
+@pytest.mark.parametrize("bp", [(2, 1), (16, 8)])
+def test_spmv_equals_textbook_product(oracle, bp):
+    """orc_spmv (mode 0) is the product with the independently built textbook
+    Laplacian on uniform meshes."""
+    m = M.uniform(*bp, extent=2.0)
+    A = oracle.assemble(m)
+    x = np.random.default_rng(12).normal(size=m.n)
+    y = oracle.spmv(A, x)
+    ref = textbook_neumann(m) @ x
+    assert np.abs(y - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("which", ["tiny_cf", "graded", "periodic"])
+def test_spmv_mean_modes_equal_scipy(oracle, which):
+    """orc_spmv in every mean mode against scipy's CSR product of the same
+    matrix plus the mean terms built here from the mesh's cell sizes:
+    mode 0: y = A x; mode 1 (BiCGSTAB.cu:393-409): the mean row's value is
+    sum h^2 x, every other row as mode 0; mode 2 (AMRSolver.h:110-118):
+    y_i = (A x)_i + h_i^2 sum h^2 x."""
+    if which == "tiny_cf":
+        m = _tiny_cf_mesh()
+    elif which == "graded":
+        m = W.graded_disk_mesh()
+    else:
+        m = M.graded(8, 4, 4, 2.0, [(0.1, 0.1, 0.05), (1.9, 0.9, 0.04)], [0.3, 0.15, 0.08], True, True)
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=m.n)
+    w = m.cell_h() ** 2                      # h^2 of every unknown's block
+    h2b = oracle.h2(m)
+    assert np.array_equal(np.repeat(h2b, 64), w)
+    A0 = oracle.assemble(m, 0)
+    ax = A0.to_scipy() @ x
+    scale = np.abs(ax).max()
+    y0 = oracle.spmv(A0, x)
+    assert np.abs(y0 - ax).max() <= 1e-14 * scale
+    A1 = oracle.assemble(m, 1)
+    assert A1.mean_row >= 0
+    y1 = oracle.spmv(A1, x, mean_mode=1, h2v=h2b)
+    ref1 = A1.to_scipy() @ x
+    ref1[A1.mean_row] = np.dot(w, x)
+    assert abs(y1[A1.mean_row] - ref1[A1.mean_row]) <= 1e-14 * np.abs(w * x).sum()
+    keep = np.arange(m.n) != A1.mean_row
+    assert np.abs(y1[keep] - ax[keep]).max() <= 1e-14 * scale
+    y2 = oracle.spmv(A0, x, mean_mode=2, h2v=h2b)
+    ref2 = ax + w * np.dot(w, x)
+    assert np.abs(y2 - ref2).max() <= 1e-13 * np.abs(ref2).max()
+    # the residual routine agrees with b - spmv in each mode
+    b = rng.normal(size=m.n)
+    for A, mode, y in ((A0, 0, y0), (A1, 1, y1), (A0, 2, y2)):
+        tr = oracle.true_residual(A, b, x, mean_mode=mode, h2v=h2b)
+        assert abs(tr["r2"] - np.linalg.norm(b - y)) <= 1e-12 * tr["r2"]
+        assert abs(tr["rinf"] - np.abs(b - y).max()) <= 1e-12 * tr["rinf"]
