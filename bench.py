@@ -99,6 +99,12 @@ def spmv_bytes_min(n, n_blocks, n_explicit, width=9):
     return 16 * n + 32 * n_blocks + n_explicit * (4 + 6 * width)
 
 
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON result)."""
+    sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.stderr.flush()
+
+
 def host_cpu():
     """(model name, logical CPUs) of this host, from /proc/cpuinfo."""
     model, count = "unknown", 0
@@ -499,12 +505,15 @@ def main():
     rhs = [W.workload_rhs(m, 400 + k) for k, m in enumerate(meshes)]
     b = np.concatenate(rhs)
     n_blocks = int(sum(m.n_blocks for m in meshes))
+    progress(f"batch: {len(meshes)} instances, {n_blocks} blocks")
     fp64, x64, st64, bs64 = timed_batch(PS, meshes, b, PS.FP64, args.steps, args.warmup, args.graph)
+    progress(f"batched FP64: {fp64['solves_per_s']:.1f} solves/s")
     offsets = bs64.offsets
     sz = bs64.sizes()
     kern = batch_kernels(PS, bs64, bs64.n, n_blocks, True)
     del bs64
     mixed, x32, _, bs32 = timed_batch(PS, meshes, b, PS.MIXED, args.steps, args.warmup, args.graph)
+    progress(f"batched mixed: {mixed['solves_per_s']:.1f} solves/s")
     kern32 = batch_kernels(PS, bs32, bs32.n, n_blocks, False)
     del bs32
     cfg = {
@@ -518,7 +527,9 @@ def main():
     }
     if not args.no_extra:
         seq = sequential(PS, meshes, rhs, 0, args.graph)
+        progress(f"sequential fused FP64: {seq['solves_per_s']:.1f} solves/s")
         unf = sequential(PS, meshes, rhs, 1, 0)
+        progress(f"sequential unfused FP64: {unf['solves_per_s']:.1f} solves/s")
         cfg["fp64_sequential_fused"] = seq
         cfg["fp64_sequential_unfused"] = unf
         cfg["mixed_batched_vs_fp64_sequential_fused"] = mixed["solves_per_s"] / seq["solves_per_s"]
@@ -527,11 +538,14 @@ def main():
         g = W.graded_disk_mesh()
         cfg["graded_1448_blocks"] = bench_single(PS, W, g, W.workload_rhs(g, 2, with_sources=True), 10,
                                                  args.graph)
+        progress("graded 1448-block mesh done")
         a6 = W.four_disk_amr6_mesh()
         cfg["amr6_16k_blocks"] = bench_single(PS, W, a6, W.workload_rhs(a6, 3), 3, args.graph)
+        progress("6-level 16k-block mesh done")
         cfg["assembly"] = bench_batch_assembly(PS, meshes)
     if not args.no_cpu:
         cfg["error_vs_oracle"] = oracle_errors(meshes, rhs, {"fp64": x64, "mixed": x32}, offsets)
+        progress("errors against the oracle done")
     dom = max(range(5), key=lambda k: kern[k]["ms_per_launch"])
     out = {
         "metric": METRIC,
@@ -561,6 +575,7 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(meshes, rhs)
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 

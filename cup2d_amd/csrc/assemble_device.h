@@ -33,6 +33,7 @@ struct DeviceOperator {
   int n_codes = 0;      // distinct coefficients (etab entries)
   // device arrays, allocated through the caller's allocator
   int4 *nbr = nullptr;                // [nb] faces Xm, Xp, Ym, Yp
+  int2 *fid = nullptr;                // [4 nb] coarse-fine neighbours per face (Operator::fid)
   unsigned long long *mask = nullptr; // [nb] explicit cells
   double *h2 = nullptr;               // [nb]
   int *erow = nullptr;                // [n_exc]

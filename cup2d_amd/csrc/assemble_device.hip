@@ -227,6 +227,7 @@ __global__ __launch_bounds__(AWG) void ka_faces(Blocks B, Hash H, int4 *nbr, int
           } else if ((s = lookup(H, in, l - 1, I >> 1, J >> 1)) >= 0) {
             v[f] = -2;
             fd.x = (int)s;
+            fd.y = -1 - (int)((f < 2 ? j : i) & 1);  // the half of the coarse face
           } else {
             long long c[2];
             for (int hh = 0; hh < 2; hh++) {
@@ -654,13 +655,12 @@ int assemble_on_device(const MeshDesc *meshes, int K, int mean_constraint, hipSt
   ACHK(hipGetLastError());
 
   // ---- faces, explicit cells, h^2 -----------------------------------------------
-  if (!give(&out.nbr, nb) || !give(&out.mask, nb) || !give(&out.h2, nb)) {
+  if (!give(&out.nbr, nb) || !give(&out.mask, nb) || !give(&out.h2, nb) || !give(&out.fid, 4 * nb)) {
     err = "device allocation failed";
     return CUP2D_PS_ENOMEM;
   }
-  int2 *d_fid;
+  int2 *const d_fid = out.fid;
   int *d_cnt, *d_bptr;
-  ACHK(T.get(&d_fid, 4 * nb));
   ACHK(T.get(&d_cnt, nb + 1));
   ACHK(T.get(&d_bptr, nb + 1));
   ka_faces<<<gb, AWG, 0, st>>>(B, H, out.nbr, d_fid, out.mask, d_cnt, out.h2, d_E);

@@ -150,6 +150,7 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
   // ---- face classification (ExpAMRSolver.cpp:221-249, :141-177) -------------
   std::vector<Face> F(size_t(nb) * 4);
   op.nbr.assign(size_t(nb) * 4, -1);
+  op.fid.assign(size_t(nb) * 8, -1);
   static const int di[4] = {-1, 1, 0, 0};
   static const int dj[4] = {0, 0, -1, 1};
   for (long long b = 0; b < nb; b++) {
@@ -171,6 +172,8 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
       } else if ((fc.coarse = find(l - 1, I >> 1, J >> 1)) >= 0) {
         fc.kind = COARSER;
         op.nbr[4 * b + f] = -2;
+        op.fid[8 * b + 2 * f] = (int)fc.coarse;
+        op.fid[8 * b + 2 * f + 1] = -1 - (int)((f < 2 ? B[b].j : B[b].i) & 1);
       } else {
         // the two children of the neighbour position that touch this face
         for (int h = 0; h < 2; h++) {
@@ -191,6 +194,8 @@ bool build_operator(const MeshDesc &m, int mean_constraint, Operator &op, std::s
         }
         fc.kind = FINER;
         op.nbr[4 * b + f] = -2;
+        op.fid[8 * b + 2 * f] = (int)fc.fine[0];
+        op.fid[8 * b + 2 * f + 1] = (int)fc.fine[1];
       }
     }
   }

@@ -36,6 +36,12 @@ struct Operator {
   long long n = 0;   // unknowns = 64 nb
   // stencil form
   std::vector<int> nbr;          // [4 nb] faces Xm, Xp, Ym, Yp
+  // [8 nb] coarse-fine neighbours of each face, a pair per face: one level
+  // coarser (coarse block, -1 - half), half = which half of the coarse
+  // block's face this block touches (its index parity along the face);
+  // one level finer (the two fine blocks, low then high along the face);
+  // (-1, -1) otherwise
+  std::vector<int> fid;
   std::vector<uint64_t> exc_mask;  // [nb] bit c set when cell c is an explicit row
   std::vector<int> exc_bptr;     // [nb+1] explicit rows of block b: exc_bptr[b]..exc_bptr[b+1]
   std::vector<int> exc_cell;     // [n_exc] cell index (0..63) of each explicit row

@@ -229,6 +229,138 @@ __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict
   }
 }
 
+// ----------------------------------------------------------------------------
+// Row kernels of the operator: a thread per block row.  Thread t of the
+// workgroup owns row iy = t & 7 of block t >> 3 of a pass of RB = 32 blocks
+// of the item: its eight values come in with 16-byte loads (contiguous over
+// the workgroup), the horizontal neighbours inside the row stay in
+// registers, the two values across the block's x faces are single loads,
+// and the rows above and below come from an LDS tile holding, per block, its
+// eight rows plus the halo rows across its y faces.  A wall face's halo is
+// the cell itself (its term x_nb - x_c vanishes), so the stencil is the same
+// five terms for every plain cell; explicit (coarse-fine) cells are left to
+// the explicit items.
+// ----------------------------------------------------------------------------
+constexpr int RB = 32;            // blocks per pass: WG threads = RB blocks x 8 rows
+constexpr int TROWS = RB * 10;    // LDS rows per chunk column: row below, 8 rows, row above
+
+template <class T>
+struct Chunk;
+template <>
+struct Chunk<double> {
+  typedef double2 type;
+  static constexpr int n = 4;     // 16-byte chunks per row of 8
+};
+template <>
+struct Chunk<float> {
+  typedef float4 type;
+  static constexpr int n = 2;
+};
+
+__device__ __forceinline__ void split(double2 c, double *v) { v[0] = c.x; v[1] = c.y; }
+__device__ __forceinline__ void split(float4 c, float *v) { v[0] = c.x; v[1] = c.y; v[2] = c.z; v[3] = c.w; }
+__device__ __forceinline__ double2 join(const double *v) { return make_double2(v[0], v[1]); }
+__device__ __forceinline__ float4 join(const float *v) { return make_float4(v[0], v[1], v[2], v[3]); }
+
+template <class T>
+__device__ __forceinline__ void ld_row(const T *__restrict__ x, long base, T (&v)[8]) {
+  typedef typename Chunk<T>::type C;
+  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+  const C *p = reinterpret_cast<const C *>(x + base);
+  C c[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) c[k] = p[k];
+#pragma unroll
+  for (int k = 0; k < NC; k++) split(c[k], v + W * k);
+}
+template <class T>
+__device__ __forceinline__ void st_row(T *__restrict__ x, long base, const T (&v)[8]) {
+  typedef typename Chunk<T>::type C;
+  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+  C *p = reinterpret_cast<C *>(x + base);
+#pragma unroll
+  for (int k = 0; k < NC; k++) p[k] = join(v + W * k);
+}
+// LDS tile: chunk k of row r at tile[k * TROWS + r] (consecutive threads
+// touch consecutive 16-byte slots: no bank conflicts)
+template <class T>
+__device__ __forceinline__ void st_tile(typename Chunk<T>::type *tile, int r, const T (&v)[8]) {
+  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+#pragma unroll
+  for (int k = 0; k < NC; k++) tile[k * TROWS + r] = join(v + W * k);
+}
+template <class T>
+__device__ __forceinline__ void ld_tile(const typename Chunk<T>::type *tile, int r, T (&v)[8]) {
+  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+#pragma unroll
+  for (int k = 0; k < NC; k++) split(tile[k * TROWS + r], v + W * k);
+}
+
+// LDS of the row kernels' tile
+template <class T>
+constexpr int row_tile_bytes() { return Chunk<T>::n * TROWS * 16; }
+
+// Calls f(base, y, x, a, h2, emask) for each row of the item's blocks
+// (b0 .. b0 + nbk - 1): base = first global index of the row, y = the plain
+// stencil of its eight cells (FP64), x / a = the row of x / aux, h2 = h^2
+// of the block, emask = the row's explicit-cell bits.  Every thread of the
+// workgroup must call it (it synchronizes the workgroup).
+template <class T, bool AUX = true, class F>
+__device__ __forceinline__ void row_blocks(const BView &V, const T *__restrict__ x,
+                                           const T *__restrict__ aux, int b0, int nbk,
+                                           unsigned char *tile_raw, F &&f) {
+  typedef typename Chunk<T>::type C;
+  C *tile = reinterpret_cast<C *>(tile_raw);
+  const int t = threadIdx.x, jl = t >> 3, iy = t & 7;
+  const bool edge = iy == 0 || iy == 7;
+  for (int s0 = 0; s0 < nbk; s0 += RB) {
+    const int j = s0 + jl;
+    const bool ok = j < nbk;
+    const int b = b0 + (ok ? j : s0);
+    const int4 nb = V.A.nbr[b];
+    const unsigned long long m = V.A.mask[b];
+    const double h2 = V.A.h2[b];
+    const long base = (long)b * 64 + iy * 8;
+    T xr[8], ar[8], hr[8];
+    ld_row(x, base, xr);
+    if constexpr (AUX) ld_row(aux, base, ar);
+    const T xl = x[nb.x >= 0 ? (long)nb.x * 64 + iy * 8 + 7 : base];
+    const T xe = x[nb.y >= 0 ? (long)nb.y * 64 + iy * 8 : base + 7];
+    if (edge) {
+      const int nv = iy == 0 ? nb.z : nb.w;
+      ld_row(x, nv >= 0 ? (long)nv * 64 + (iy == 0 ? 56 : 0) : base, hr);
+    }
+    st_tile(tile, 10 * jl + iy + 1, xr);
+    if (edge) st_tile(tile, 10 * jl + (iy == 0 ? 0 : 9), hr);
+    __syncthreads();
+    T dn[8], up[8];
+    ld_tile(tile, 10 * jl + iy, dn);
+    ld_tile(tile, 10 * jl + iy + 2, up);
+    __syncthreads();  // the tile is rewritten by the next pass
+    double y[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double l = k > 0 ? (double)xr[k - 1] : (double)xl;
+      const double r = k < 7 ? (double)xr[k + 1] : (double)xe;
+      y[k] = (((double)dn[k] + l) + (r + (double)up[k])) - 4.0 * (double)xr[k];
+    }
+    if (ok) f(base, y, xr, ar, h2, (unsigned)(m >> (8 * iy)) & 0xffu);
+  }
+}
+
+// Stores the row's plain cells (all eight with one set of 16-byte stores
+// when the row has no explicit cell).
+template <class T>
+__device__ __forceinline__ void st_plain(T *__restrict__ y, long base, const T (&v)[8], unsigned em) {
+  if (em == 0) {
+    st_row(y, base, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (!((em >> k) & 1u)) y[base + k] = v[k];
+  }
+}
+
 // r = b - A x (FP64), stored as T to r and rhat; p = nu = 0; x_opt = x;
 // partials of the update totals and of sum h^2 x, for the instances that
 // pass `filter` (the start of a solve, or a residual replacement).  The
@@ -281,26 +413,35 @@ template <class T>
 __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ z1,
                                                  T *__restrict__ nu, const T *__restrict__ rhat,
                                                  const Scal *st, double *p2) {
-  const int c = lane_id();
-  running_items<ub_of<T>()>(V, st, [&](int item, int4 mt, const Heads &hd) {
+  __shared__ __align__(16) unsigned char tile[row_tile_bytes<T>()];
+  running_items(V, st, [&](int item, int4 mt, const Heads &) {
     const long long mr = V.mean_row[mt.x];
     double v[2] = {0.0, 0.0};
-    auto store_y = [&](long i, double y, double rh) {
-      if (i == mr) y = 0.0;  // completed by kb_alpha
-      const T yt = (T)y;
-      nu[i] = yt;
-      v[0] += rh * (double)yt;
-    };
     if (mt.z > 0) {
-      item_blocks<T>(V, z1, rhat, mt.y, mt.z, hd, [&](const auto &R, int u, int) {
-        bool plain;
-        const double y = R.y(u, c, plain);
-        v[1] += R.hh[u] * R.xc(u);
-        if (plain) store_y(R.i(u, c), y, R.a(u));
-      });
+      row_blocks<T>(V, z1, rhat, mt.y, mt.z, tile,
+                    [&](long base, const double (&y)[8], const T (&xr)[8], const T (&ar)[8], double h2,
+                        unsigned em) {
+                      double hs = 0.0;
+                      T yt[8];
+#pragma unroll
+                      for (int k = 0; k < 8; k++) {
+                        hs += (double)xr[k];
+                        yt[k] = (T)y[k];
+                      }
+                      if (base == mr) yt[0] = T(0);  // the mean row (cell 0 of its block): kb_alpha
+#pragma unroll
+                      for (int k = 0; k < 8; k++)
+                        v[0] += ((em >> k) & 1u) ? 0.0 : (double)ar[k] * (double)yt[k];
+                      v[1] += h2 * hs;
+                      st_plain(nu, base, yt, em);
+                    });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
-      store_y(row, explicit_row<T>(V.A, z1, e, mt.w), (double)rhat[row]);
+      double y = explicit_row<T>(V.A, z1, e, mt.w);
+      if (row == mr) y = 0.0;  // completed by kb_alpha
+      const T yt = (T)y;
+      nu[row] = yt;
+      v[0] += (double)rhat[row] * (double)yt;
     }
     store_item_partials<2, 0u>(v, p2, V.n_items, item);
   });
@@ -311,28 +452,40 @@ template <class T>
 __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z2,
                                                 T *__restrict__ t, const T *__restrict__ sv,
                                                 const Scal *st, double *p4) {
-  const int c = lane_id();
-  running_items<ub_of<T>()>(V, st, [&](int item, int4 mt, const Heads &hd) {
+  __shared__ __align__(16) unsigned char tile[row_tile_bytes<T>()];
+  running_items(V, st, [&](int item, int4 mt, const Heads &) {
     const long long mr = V.mean_row[mt.x];
     double v[3] = {0.0, 0.0, 0.0};
-    auto store_y = [&](long i, double y, double si) {
-      if (i == mr) y = 0.0;  // completed by kb_omega
-      const T yt = (T)y;
-      t[i] = yt;
-      const double yd = (double)yt;
-      v[0] += yd * si;
-      v[1] += yd * yd;
-    };
     if (mt.z > 0) {
-      item_blocks<T>(V, z2, sv, mt.y, mt.z, hd, [&](const auto &R, int u, int) {
-        bool plain;
-        const double y = R.y(u, c, plain);
-        v[2] += R.hh[u] * R.xc(u);
-        if (plain) store_y(R.i(u, c), y, R.a(u));
-      });
+      row_blocks<T>(V, z2, sv, mt.y, mt.z, tile,
+                    [&](long base, const double (&y)[8], const T (&xr)[8], const T (&ar)[8], double h2,
+                        unsigned em) {
+                      double hs = 0.0;
+                      T yt[8];
+#pragma unroll
+                      for (int k = 0; k < 8; k++) {
+                        hs += (double)xr[k];
+                        yt[k] = (T)y[k];
+                      }
+                      if (base == mr) yt[0] = T(0);  // the mean row: kb_omega
+#pragma unroll
+                      for (int k = 0; k < 8; k++) {
+                        const double yd = ((em >> k) & 1u) ? 0.0 : (double)yt[k];
+                        v[0] += yd * (double)ar[k];
+                        v[1] += yd * yd;
+                      }
+                      v[2] += h2 * hs;
+                      st_plain(t, base, yt, em);
+                    });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x, row = V.A.erow[e];
-      store_y(row, explicit_row<T>(V.A, z2, e, mt.w), (double)sv[row]);
+      double y = explicit_row<T>(V.A, z2, e, mt.w);
+      if (row == mr) y = 0.0;  // completed by kb_omega
+      const T yt = (T)y;
+      t[row] = yt;
+      const double yd = (double)yt;
+      v[0] += yd * (double)sv[row];
+      v[1] += yd * yd;
     }
     store_item_partials<3, 0u>(v, p4, V.n_items, item);
   });
@@ -342,17 +495,19 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
 // sum h^2 x for the mean rows
 __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict__ x,
                                               double *__restrict__ y, double *pm) {
-  const int c = lane_id();
-  running_items<UB64>(V, nullptr, [&](int item, int4 mt, const Heads &hd) {
+  __shared__ __align__(16) unsigned char tile[row_tile_bytes<double>()];
+  running_items(V, nullptr, [&](int item, int4 mt, const Heads &) {
     double v[1] = {0.0};
     if (mt.z > 0) {
-      item_blocks<double>(V, x, x, mt.y, mt.z, hd,
-                          [&](const auto &R, int u, int) {
-                            bool plain;
-                            const double yi = R.y(u, c, plain);
-                            v[0] += R.hh[u] * R.xc(u);
-                            if (plain) y[R.i(u, c)] = yi;
-                          });
+      row_blocks<double, false>(V, x, x, mt.y, mt.z, tile,
+                         [&](long base, const double (&yr)[8], const double (&xr)[8], const double (&)[8],
+                             double h2, unsigned em) {
+                           double hs = 0.0;
+#pragma unroll
+                           for (int k = 0; k < 8; k++) hs += xr[k];
+                           v[0] += h2 * hs;
+                           st_plain(y, base, yr, em);
+                         });
     } else if ((int)threadIdx.x < -mt.z) {
       const int e = mt.y + threadIdx.x;
       y[V.A.erow[e]] = explicit_row<double>(V.A, x, e, mt.w);
@@ -859,6 +1014,7 @@ struct cup2d_psb {
   std::vector<int> tile_inst, tile_b0, tile_nb;
   // device: operator
   int4 *d_nbr = nullptr;
+  int2 *d_fid = nullptr;            // [4 nb] coarse-fine neighbours of each face
   unsigned long long *d_mask = nullptr;
   int *d_erow = nullptr, *d_ecol = nullptr;
   uint16_t *d_ecode = nullptr;
@@ -1527,12 +1683,14 @@ static int build_host(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
   if (W > ELL_MAX)
     return bfail(h, CUP2D_PS_EINVAL, "explicit row with more than " + std::to_string(ELL_MAX) + " entries");
   std::vector<int4> nbr;
+  std::vector<int2> fid;
   std::vector<unsigned long long> mask;
   std::vector<double> h2;
   EllRows ell;
   ell.W = W;
   std::string eerr;
   nbr.reserve(h->nb);
+  fid.reserve(4 * h->nb);
   mask.reserve(h->nb);
   h2.reserve(h->nb);
   h->h_mean_row.assign(n_inst, -1);
@@ -1549,6 +1707,8 @@ static int build_host(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
       v.z = shift(op.nbr[4 * b + 2]);
       v.w = shift(op.nbr[4 * b + 3]);
       nbr.push_back(v);
+      for (int f = 0; f < 4; f++)  // block ids shifted; a coarse face's half code (< 0) kept
+        fid.push_back(make_int2(shift(op.fid[8 * b + 2 * f]), shift(op.fid[8 * b + 2 * f + 1])));
       mask.push_back(op.exc_mask[b]);
       h2.push_back(op.h2[b]);
     }
@@ -1563,7 +1723,7 @@ static int build_host(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes) {
   auto &pool = h->allocs;
   int rc;
   if ((rc = bupload(h, pool, &h->d_nbr, nbr)) || (rc = bupload(h, pool, &h->d_mask, mask)) ||
-      (rc = bupload(h, pool, &h->d_erow, ell.erow)) ||
+      (rc = bupload(h, pool, &h->d_fid, fid)) || (rc = bupload(h, pool, &h->d_erow, ell.erow)) ||
       (rc = bupload(h, pool, &h->d_ecol, ell_transpose(ell.ecol_rm, W))) ||
       (rc = bupload(h, pool, &h->d_ecode, ell_transpose(ell.ecode_rm, W))) ||
       (rc = bupload(h, pool, &h->d_etab, ell.values)) || (rc = bupload(h, pool, &h->d_h2, h2)))
@@ -1608,6 +1768,7 @@ static int build_device(cup2d_psb *h, int n_inst, const cup2d_mesh_desc *meshes)
   h->elen = std::move(D.elen);
   h->h_mean_row = D.mean_rows;
   h->d_nbr = D.nbr;
+  h->d_fid = D.fid;
   h->d_mask = D.mask;
   h->d_h2 = D.h2;
   h->d_erow = D.erow;
@@ -1809,10 +1970,6 @@ int cup2d_psb_pressure_step_device(cup2d_psb *h, const double *d_rhs, double *d_
 
 static int projection_checks_b(cup2d_psb *h, double dt) {
   if (!h->ready) return bfail(h, CUP2D_PS_ENOMESH, "no meshes");
-  if (h->eoff[h->K] > 0)
-    return bfail(h, CUP2D_PS_EINVAL,
-                 "a mesh has coarse-fine faces: their ghost values come from the Cubism lab's "
-                 "interpolation, which this operator does not restate");
   if (!(dt > 0)) return bfail(h, CUP2D_PS_EINVAL, "dt must be positive");
   return CUP2D_PS_OK;
 }
@@ -1833,8 +1990,13 @@ int cup2d_psb_pressure_rhs_device(cup2d_psb *h, const double *d_u, const double 
       return bfail(h, CUP2D_PS_EINVAL, "rhs must not alias an input");
     if (const int rc = projection_checks_b(h, dt)) return rc;
     BCHK(h, hipSetDevice(h->cfg.device));
-    k_pressure_rhs<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_h, d_u, d_v,
-                                                         d_udef_u, d_udef_v, d_chi, dt, d_rhs);
+    k_pressure_rhs<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_mask, h->d_h, d_u,
+                                                         d_v, d_udef_u, d_udef_v, d_chi, dt, d_rhs);
+    const int n_exc = (int)h->eoff[h->K];
+    if (n_exc > 0)
+      k_pressure_rhs_cf<<<exc_grid(n_exc), WG, 0, h->stream>>>(n_exc, h->d_erow, h->d_nbr, h->d_fid,
+                                                               h->d_h, d_u, d_v, d_udef_u, d_udef_v,
+                                                               d_chi, dt, d_rhs);
     BCHK(h, hipGetLastError());
     BCHK(h, hipStreamSynchronize(h->stream));
     return CUP2D_PS_OK;
@@ -1849,8 +2011,12 @@ int cup2d_psb_pressure_correct_device(cup2d_psb *h, const double *d_p, double dt
     if (d_u == d_v || d_p == d_u || d_p == d_v) return bfail(h, CUP2D_PS_EINVAL, "p, u and v must not alias");
     if (const int rc = projection_checks_b(h, dt)) return rc;
     BCHK(h, hipSetDevice(h->cfg.device));
-    k_pressure_correct<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_h, d_p, dt, d_u,
-                                                             d_v);
+    k_pressure_correct<<<cell_grid_b(h), WG, 0, h->stream>>>((int)h->nb, h->d_nbr, h->d_mask, h->d_h, d_p,
+                                                             dt, d_u, d_v);
+    const int n_exc = (int)h->eoff[h->K];
+    if (n_exc > 0)
+      k_pressure_correct_cf<<<exc_grid(n_exc), WG, 0, h->stream>>>(n_exc, h->d_erow, h->d_nbr, h->d_fid,
+                                                                   h->d_h, d_p, dt, d_u, d_v);
     BCHK(h, hipGetLastError());
     BCHK(h, hipStreamSynchronize(h->stream));
     return CUP2D_PS_OK;

@@ -598,11 +598,12 @@ __device__ __forceinline__ void mf_pupdate(MfIn &in, float *__restrict__ r,
 // ----------------------------------------------------------------------------
 namespace {
 
-// Values of a cell's four neighbours (E, W, N, S) in a mesh without
-// coarse-fine faces: inside the block, in the same-level neighbour block, or
-// the ghost of a domain face -- `wall_sign` times the cell's own value (the
-// reference's labs mirror the boundary cell: -1 for the face-normal velocity
-// component, Definitions.h:84-117; +1 for the zero-gradient pressure).
+// Values of a cell's four neighbours (E, W, N, S) across plain faces:
+// inside the block, in the same-level neighbour block, or the ghost of a
+// domain face -- `wall_sign` times the cell's own value (the reference's
+// labs mirror the boundary cell: -1 for the face-normal velocity component,
+// Definitions.h:84-117; +1 for the zero-gradient pressure).  Cells on a
+// coarse-fine face are computed by the *_cf kernels below instead.
 struct Nb4 {
   double e, w, n, s;
 };
@@ -620,8 +621,10 @@ __device__ __forceinline__ Nb4 neighbours(const double *__restrict__ f, long i, 
 
 // rhs = h/(2 dt) [(u_E - u_W) + (v_N - v_S)] - h/(2 dt) chi [(ud_E - ud_W) + (vd_N - vd_S)]
 // (updatePressureRHS, PressureSingle.cpp:336-349), in the reference's order of
-// operations (no contraction).
+// operations (no contraction).  Cells on a coarse-fine face (set in mask)
+// are left to k_pressure_rhs_cf.
 __global__ __launch_bounds__(WG) void k_pressure_rhs(int nb, const int4 *__restrict__ nbr,
+                                                     const unsigned long long *__restrict__ mask,
                                                      const double *__restrict__ hb,
                                                      const double *__restrict__ u,
                                                      const double *__restrict__ v,
@@ -632,6 +635,7 @@ __global__ __launch_bounds__(WG) void k_pressure_rhs(int nb, const int4 *__restr
 #pragma clang fp contract(off)
   const long n = (long)nb * 64;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    if ((mask[i >> 6] >> (i & 63)) & 1ull) continue;
     const int4 f = nbr[i >> 6];
     const double facDiv = 0.5 * hb[i >> 6] / dt;
     const Nb4 U = neighbours(u, i, f, -1.0, 1.0), V = neighbours(v, i, f, 1.0, -1.0);
@@ -645,7 +649,9 @@ __global__ __launch_bounds__(WG) void k_pressure_rhs(int nb, const int4 *__restr
 // u += (-dt/2 h (p_E - p_W)) / h^2, v likewise (pressureCorrectionKernel and
 // PressureSingle::pressureCorrection, PressureSingle.cpp:122-210), zero-gradient
 // pressure ghosts, in the reference's order of operations (no contraction).
+// Cells on a coarse-fine face are left to k_pressure_correct_cf.
 __global__ __launch_bounds__(WG) void k_pressure_correct(int nb, const int4 *__restrict__ nbr,
+                                                         const unsigned long long *__restrict__ mask,
                                                          const double *__restrict__ hb,
                                                          const double *__restrict__ p, double dt,
                                                          double *__restrict__ u,
@@ -653,12 +659,211 @@ __global__ __launch_bounds__(WG) void k_pressure_correct(int nb, const int4 *__r
 #pragma clang fp contract(off)
   const long n = (long)nb * 64;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    if ((mask[i >> 6] >> (i & 63)) & 1ull) continue;
     const double h = hb[i >> 6];
     const double pFac = -0.5 * dt * h, ih2 = 1.0 / h / h;
     const Nb4 P = neighbours(p, i, nbr[i >> 6], 1.0, 1.0);
     const double tu = pFac * (P.e - P.w), tv = pFac * (P.n - P.s);
     u[i] += tu * ih2;
     v[i] += tv * ih2;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Cells on a coarse-fine face, in face form: the operator is the sum over the
+// faces Xm, Xp, Ym, Yp of s_f flux_f (s = -1 low, +1 high), flux_f being the
+// face value the reference's kernels store for the flux correction
+// (PressureSingle.cpp:350-397, :139-185).  Across a coarser face the
+// neighbour value is the explicit operator's coarse-fine ghost
+// (ExpAMRSolver.cpp:100-128, ExpAMRSolver.h:268-300); across a finer face
+// flux_f is the sum of the two fine cells' fluxes (the flux correction:
+// coarse and fine fluxes equal and opposite).  fid holds the coarse-fine
+// neighbours of each face: (coarse block, -1 - half) or the two fine blocks.
+// ----------------------------------------------------------------------------
+
+__device__ __forceinline__ long gcell(long b, int ix, int iy) { return b * 64 + iy * 8 + ix; }
+
+// grid of the *_cf kernels: a thread per coarse-fine cell, up to 1024 workgroups
+inline int exc_grid(int n_exc) { return n_exc > 0 ? (n_exc + WG - 1) / WG < 1024 ? (n_exc + WG - 1) / WG : 1024 : 1; }
+
+// 2/3 x_close - 1/5 x_far + 8/15 (x_c + sT D1 + D2), D1/D2 along the face
+// around coarse cell c (tangential index t in its block; stride `st`
+// between coarse cells along the face), one-sided at t = 3, 7 (backward) and
+// 0, 4 (forward)
+__device__ __forceinline__ double cf_ghost(const double *__restrict__ x, long close, long far_,
+                                           long c, int t, int st, double sT) {
+#pragma clang fp contract(off)
+  const double xc = x[c];
+  double d1, d2;
+  if (t == 3 || t == 7) {
+    const double a = x[c - 2 * st], bb = x[c - st];
+    d1 = ((1.0 / 8.0) * a + (-1.0 / 2.0) * bb) + (3.0 / 8.0) * xc;
+    d2 = ((1.0 / 32.0) * a + (-1.0 / 16.0) * bb) + (1.0 / 32.0) * xc;
+  } else if (t == 0 || t == 4) {
+    const double a = x[c + 2 * st], bb = x[c + st];
+    d1 = ((-1.0 / 8.0) * a + (1.0 / 2.0) * bb) + (-3.0 / 8.0) * xc;
+    d2 = ((1.0 / 32.0) * a + (-1.0 / 16.0) * bb) + (1.0 / 32.0) * xc;
+  } else {
+    const double a = x[c - st], bb = x[c + st];
+    d1 = (-1.0 / 8.0) * a + (1.0 / 8.0) * bb;
+    d2 = ((1.0 / 32.0) * a + (1.0 / 32.0) * bb) + (-1.0 / 16.0) * xc;
+  }
+  return ((2.0 / 3.0) * x[close] + (-1.0 / 5.0) * x[far_]) + (8.0 / 15.0) * ((xc + sT * d1) + d2);
+}
+
+// Where the ghost of fine cell (ix, iy) of block b across face f (coarser
+// neighbour cb, `half` of its face) comes from.
+struct CfSrc {
+  long close, far_, c;
+  int t, st;
+  double sT;
+};
+__device__ __forceinline__ CfSrc fine_side(long b, int ix, int iy, int f, long cb, int half) {
+  CfSrc r;
+  const bool xf = f < 2;
+  r.close = gcell(b, ix, iy);
+  r.sT = ((xf ? iy : ix) % 2 == 0) ? -1.0 : 1.0;
+  if (xf) {
+    const int ixc = f == 0 ? 7 : 0, iyc = iy / 2 + 4 * half;
+    r.far_ = gcell(b, f == 0 ? ix + 1 : ix - 1, iy);
+    r.c = gcell(cb, ixc, iyc);
+    r.t = iyc;
+    r.st = 8;
+  } else {
+    const int iyc = f == 2 ? 7 : 0, ixc = ix / 2 + 4 * half;
+    r.far_ = gcell(b, ix, f == 2 ? iy + 1 : iy - 1);
+    r.c = gcell(cb, ixc, iyc);
+    r.t = ixc;
+    r.st = 1;
+  }
+  return r;
+}
+// ... and of fine cell hh (0: low, 1: high) behind face f of coarse cell
+// (ix, iy) of block b, whose finer neighbours are f0, f1.
+__device__ __forceinline__ CfSrc coarse_side(long b, int ix, int iy, int f, long f0, long f1, int hh) {
+  CfSrc r;
+  const bool xf = f < 2;
+  const int tc = xf ? iy : ix;
+  const long fb = tc >= 4 ? f1 : f0;
+  const int t = (tc % 4) * 2 + hh;
+  switch (f) {
+    case 0: r.close = gcell(fb, 7, t); r.far_ = gcell(fb, 6, t); break;
+    case 1: r.close = gcell(fb, 0, t); r.far_ = gcell(fb, 1, t); break;
+    case 2: r.close = gcell(fb, t, 7); r.far_ = gcell(fb, t, 6); break;
+    default: r.close = gcell(fb, t, 0); r.far_ = gcell(fb, t, 1); break;
+  }
+  r.c = gcell(b, ix, iy);
+  r.t = tc;
+  r.st = xf ? 8 : 1;
+  r.sT = hh == 0 ? -1.0 : 1.0;
+  return r;
+}
+
+// Face values of one cell: coef (nb + self) - (coef chi_self) (dnb + dself)
+// (with_d) or coef (nb + self); coef(h) of the cell that owns the face.
+struct CfField {
+  const double *c, *d;  // the component the face reads, its deformation part (or null)
+  double wall;          // the wall ghost is wall * self
+};
+template <class Coef>
+__device__ __forceinline__ double cf_flux(const CfField &F, const double *__restrict__ chi,
+                                          const double *__restrict__ hb, long i, int ix, int iy,
+                                          int f, int4 nb4, int2 fd, Coef coef) {
+#pragma clang fp contract(off)
+  const bool with_d = F.d != nullptr;
+  auto one = [&](double self, double nbv, double dself, double dnb, double cf, long k) {
+    double t = cf * (nbv + self);
+    if (with_d) t = t - (cf * chi[k]) * (dnb + dself);
+    return t;
+  };
+  const long b = i >> 6;
+  const double cself = coef(hb[b]);
+  const bool edge = f == 0 ? ix == 0 : f == 1 ? ix == 7 : f == 2 ? iy == 0 : iy == 7;
+  const double ds = with_d ? F.d[i] : 0.0;
+  if (!edge) {
+    const long j = f == 0 ? i - 1 : f == 1 ? i + 1 : f == 2 ? i - 8 : i + 8;
+    return one(F.c[i], F.c[j], ds, with_d ? F.d[j] : 0.0, cself, i);
+  }
+  const int nv = f == 0 ? nb4.x : f == 1 ? nb4.y : f == 2 ? nb4.z : nb4.w;
+  if (nv == -1) return one(F.c[i], F.wall * F.c[i], ds, F.wall * ds, cself, i);
+  if (nv >= 0) {
+    const long j = f == 0 ? gcell(nv, 7, iy) : f == 1 ? gcell(nv, 0, iy) : f == 2 ? gcell(nv, ix, 7)
+                                                                        : gcell(nv, ix, 0);
+    return one(F.c[i], F.c[j], ds, with_d ? F.d[j] : 0.0, cself, i);
+  }
+  if (fd.y < 0) {  // coarser neighbour
+    const CfSrc g = fine_side(b, ix, iy, f, fd.x, -1 - fd.y);
+    const double gv = cf_ghost(F.c, g.close, g.far_, g.c, g.t, g.st, g.sT);
+    const double gd = with_d ? cf_ghost(F.d, g.close, g.far_, g.c, g.t, g.st, g.sT) : 0.0;
+    return one(F.c[i], gv, ds, gd, cself, i);
+  }
+  double tot = 0.0;  // finer neighbours: the two fine cells' fluxes
+#pragma unroll
+  for (int hh = 0; hh < 2; hh++) {
+    const CfSrc g = coarse_side(b, ix, iy, f, fd.x, fd.y, hh);
+    const double gv = cf_ghost(F.c, g.close, g.far_, g.c, g.t, g.st, g.sT);
+    const double gd = with_d ? cf_ghost(F.d, g.close, g.far_, g.c, g.t, g.st, g.sT) : 0.0;
+    const double fl = one(F.c[g.close], gv, with_d ? F.d[g.close] : 0.0, gd, coef(hb[g.close >> 6]),
+                          g.close);
+    tot = hh == 0 ? fl : tot + fl;
+  }
+  return tot;
+}
+
+// The RHS at the cells on a coarse-fine face (explicit rows erow[0..n_exc)),
+// one thread per cell.
+__global__ __launch_bounds__(WG) void k_pressure_rhs_cf(int n_exc, const int *__restrict__ erow,
+                                                        const int4 *__restrict__ nbr,
+                                                        const int2 *__restrict__ fid,
+                                                        const double *__restrict__ hb,
+                                                        const double *__restrict__ u,
+                                                        const double *__restrict__ v,
+                                                        const double *__restrict__ ud,
+                                                        const double *__restrict__ vd,
+                                                        const double *__restrict__ chi, double dt,
+                                                        double *__restrict__ rhs) {
+#pragma clang fp contract(off)
+  for (int e = blockIdx.x * WG + threadIdx.x; e < n_exc; e += gridDim.x * WG) {
+    const long i = erow[e], b = i >> 6;
+    const int ix = (int)(i & 7), iy = (int)((i >> 3) & 7);
+    const int4 nb4 = nbr[b];
+    auto facDiv = [dt](double h) { return 0.5 * h / dt; };
+    const CfField FU{u, ud, -1.0}, FV{v, vd, -1.0};
+    double acc = 0.0;
+#pragma unroll 1
+    for (int f = 0; f < 4; f++) {
+      const double fl = cf_flux(f < 2 ? FU : FV, chi, hb, i, ix, iy, f, nb4, fid[4 * b + f], facDiv);
+      const double term = (f == 0 || f == 2) ? -fl : fl;
+      acc = f == 0 ? term : acc + term;
+    }
+    rhs[i] = acc;
+  }
+}
+
+// The correction at the cells on a coarse-fine face, one thread per cell.
+__global__ __launch_bounds__(WG) void k_pressure_correct_cf(int n_exc, const int *__restrict__ erow,
+                                                            const int4 *__restrict__ nbr,
+                                                            const int2 *__restrict__ fid,
+                                                            const double *__restrict__ hb,
+                                                            const double *__restrict__ p, double dt,
+                                                            double *__restrict__ u,
+                                                            double *__restrict__ v) {
+#pragma clang fp contract(off)
+  for (int e = blockIdx.x * WG + threadIdx.x; e < n_exc; e += gridDim.x * WG) {
+    const long i = erow[e], b = i >> 6;
+    const int ix = (int)(i & 7), iy = (int)((i >> 3) & 7);
+    const int4 nb4 = nbr[b];
+    auto pFac = [dt](double h) { return -0.5 * dt * h; };
+    const CfField FP{p, nullptr, 1.0};
+    double t[4];
+#pragma unroll 1
+    for (int f = 0; f < 4; f++) {
+      const double fl = cf_flux(FP, nullptr, hb, i, ix, iy, f, nb4, fid[4 * b + f], pFac);
+      t[f] = (f == 0 || f == 2) ? -fl : fl;
+    }
+    const double h = hb[b], ih2 = 1.0 / h / h;
+    u[i] = u[i] + (t[0] + t[1]) * ih2;
+    v[i] = v[i] + (t[2] + t[3]) * ih2;
   }
 }
 

@@ -862,6 +862,7 @@ struct cup2d_ps {
   Operator op;
   // device operator
   int4 *d_nbr = nullptr;
+  int2 *d_fid = nullptr;                 // [4 nb] coarse-fine neighbours of each face
   unsigned long long *d_mask = nullptr;
   int *d_erow = nullptr, *d_ecol = nullptr;
   uint16_t *d_ecode = nullptr;
@@ -1616,6 +1617,8 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   std::vector<int4> nbr(op.nb);
   for (long long b = 0; b < op.nb; b++)
     nbr[b] = make_int4(op.nbr[4 * b], op.nbr[4 * b + 1], op.nbr[4 * b + 2], op.nbr[4 * b + 3]);
+  std::vector<int2> fid(4 * op.nb);
+  for (long long k = 0; k < 4 * op.nb; k++) fid[k] = make_int2(op.fid[2 * k], op.fid[2 * k + 1]);
   std::vector<unsigned long long> mask(op.exc_mask.begin(), op.exc_mask.end());
   h->ell_w = ell_width(op);
   if (h->ell_w > ELL_MAX)
@@ -1629,7 +1632,7 @@ static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   for (size_t r = 0; r < ewid.size(); r++)
     ewid[r] = (uint8_t)ell.width((long long)(64 * r), (long long)std::min(n_exc, 64 * r + 64));
   if ((rc = upload(h, &h->d_nbr, nbr)) || (rc = upload(h, &h->d_mask, mask)) ||
-      (rc = upload(h, &h->d_erow, ell.erow)) ||
+      (rc = upload(h, &h->d_fid, fid)) || (rc = upload(h, &h->d_erow, ell.erow)) ||
       (rc = upload(h, &h->d_ecol, ell_transpose(ell.ecol_rm, h->ell_w))) ||
       (rc = upload(h, &h->d_ecode, ell_transpose(ell.ecode_rm, h->ell_w))) ||
       (rc = upload(h, &h->d_etab, ell.values)) || (rc = upload(h, &h->d_ewid, ewid)) ||
@@ -1715,6 +1718,7 @@ static int set_mesh_on_device_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
   h->op.mean_row = D.mean_row;
   h->op.corner_block = D.corner_block;
   h->d_nbr = D.nbr;
+  h->d_fid = D.fid;
   h->d_mask = D.mask;
   h->d_h2 = D.h2;
   h->d_erow = D.erow;
@@ -1785,10 +1789,6 @@ static int cell_grid(const cup2d_ps *h) {
 
 static int projection_checks(cup2d_ps *h, double dt) {
   if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
-  if (h->n_exc > 0)
-    return fail(h, CUP2D_PS_EINVAL,
-                "mesh has coarse-fine faces: their ghost values come from the Cubism lab's "
-                "interpolation, which this operator does not restate");
   if (!(dt > 0)) return fail(h, CUP2D_PS_EINVAL, "dt must be positive");
   return CUP2D_PS_OK;
 }
@@ -1802,8 +1802,12 @@ static int pressure_rhs_body(cup2d_ps *h, const double *d_u, const double *d_v, 
     return fail(h, CUP2D_PS_EINVAL, "rhs must not alias an input");
   if (const int rc = projection_checks(h, dt)) return rc;
   HIPCHK(h, hipSetDevice(h->device));
-  k_pressure_rhs<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_u, d_v, d_ud,
-                                                    d_vd, d_chi, dt, d_rhs);
+  k_pressure_rhs<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_mask, h->d_h, d_u,
+                                                    d_v, d_ud, d_vd, d_chi, dt, d_rhs);
+  if (h->n_exc > 0)
+    k_pressure_rhs_cf<<<exc_grid((int)h->n_exc), WG, 0, h->stream>>>((int)h->n_exc, h->d_erow, h->d_nbr,
+                                                                    h->d_fid, h->d_h, d_u, d_v, d_ud,
+                                                                    d_vd, d_chi, dt, d_rhs);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;
@@ -1814,8 +1818,12 @@ static int pressure_correct_body(cup2d_ps *h, const double *d_p, double dt, doub
   if (d_u == d_v || d_p == d_u || d_p == d_v) return fail(h, CUP2D_PS_EINVAL, "p, u and v must not alias");
   if (const int rc = projection_checks(h, dt)) return rc;
   HIPCHK(h, hipSetDevice(h->device));
-  k_pressure_correct<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_h, d_p, dt, d_u,
-                                                        d_v);
+  k_pressure_correct<<<cell_grid(h), WG, 0, h->stream>>>((int)h->op.nb, h->d_nbr, h->d_mask, h->d_h, d_p,
+                                                        dt, d_u, d_v);
+  if (h->n_exc > 0)
+    k_pressure_correct_cf<<<exc_grid((int)h->n_exc), WG, 0, h->stream>>>((int)h->n_exc, h->d_erow,
+                                                                        h->d_nbr, h->d_fid, h->d_h, d_p,
+                                                                        dt, d_u, d_v);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CUP2D_PS_OK;

@@ -205,7 +205,8 @@ class Solver:
 
     def pressure_rhs(self, u, v, udef_u, udef_v, chi, dt):
         """The divergence RHS of the pressure equation (PressureSingle.cpp:324-349)
-        for host arrays; meshes without coarse-fine faces."""
+        for host arrays; cells on coarse-fine faces with the flux correction
+        (PressureSingle.cpp:350-397)."""
         d = [DeviceArray.from_host(self, np.ascontiguousarray(a, dtype=np.float64))
              for a in (u, v, udef_u, udef_v, chi)]
         d_rhs = DeviceArray(self, d[0].n)

@@ -150,7 +150,12 @@ int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double to
 /* The pressure step around the solve, device-resident (reference
  * PressureSingle::operator(), PressureSingle.cpp:842-879):
  *   b  = rhs - L p_old   (updatePressureRHS1, :401-456, applied at :848-849;
- *                         L = the mesh's operator without the mean-row replacement)
+ *                         L = the mesh's operator without the mean-row replacement.
+ *                         At cells on a coarse-fine face the reference takes L p_old
+ *                         from the Cubism lab's interpolated ghosts and flux
+ *                         correction; here it is the explicit operator's row, the
+ *                         same ghost interpolation in face form -- equality with
+ *                         Cubism's is parity unpinned, Cubism being absent)
  *   dp = solve A dp = b from dp = 0 (:845, :851; as cup2d_ps_solve_device)
  *   p  = dp + (p_old - <dp>_h2), <.>_h2 the h^2-weighted mean (:853-879).
  * d_rhs: the divergence part of the RHS (updatePressureRHS's output, PressureSingle.cpp:324-398),
@@ -159,12 +164,19 @@ int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double to
 int cup2d_ps_pressure_step_device(cup2d_ps *h, const double *d_rhs, double *d_p, double tol_abs,
                                   double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
-/* The two operators either side of the pressure step on device arrays, for
- * meshes without coarse-fine faces (CUP2D_PS_EINVAL otherwise: there the
- * reference's values come from the Cubism lab's interpolation and flux
- * correction).  Ghost values at non-periodic domain faces mirror the boundary
- * cell: the face-normal velocity component negated (Definitions.h:84-117, both
- * wall and free-slip), the pressure copied (zero gradient).
+/* The two operators either side of the pressure step on device arrays.
+ * Ghost values at non-periodic domain faces mirror the boundary cell: the
+ * face-normal velocity component negated (Definitions.h:84-117, both wall and
+ * free-slip), the pressure copied (zero gradient).  Cells on a coarse-fine
+ * block face are computed in face form, as the sum over their faces of the
+ * face values the reference stores for the flux correction
+ * (PressureSingle.cpp:350-397, :139-185): across a coarser face with the
+ * explicit operator's coarse-fine ghost (ExpAMRSolver.cpp:100-128), across a
+ * finer face as the sum of the two fine cells' face values, so coarse and
+ * fine fluxes are equal and opposite.  The reference takes these ghosts
+ * from Cubism's lab interpolation (absent here): equality with it is parity
+ * unpinned; linear fields are differentiated exactly there and the RHS sums
+ * to the boundary flux.
  *
  * Divergence RHS (updatePressureRHS, PressureSingle.cpp:324-349):
  *   rhs = h/(2 dt) [(u_E - u_W) + (v_N - v_S)] - h/(2 dt) chi [(ud_E - ud_W) + (vd_N - vd_S)]
@@ -272,8 +284,7 @@ int cup2d_psb_pressure_step_device(cup2d_psb *h, const double *d_rhs, double *d_
                                    double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
 /* The divergence RHS and the velocity correction for every instance at once
- * (as cup2d_ps_pressure_rhs_device / cup2d_ps_pressure_correct_device; all
- * instances without coarse-fine faces, else CUP2D_PS_EINVAL). */
+ * (as cup2d_ps_pressure_rhs_device / cup2d_ps_pressure_correct_device). */
 int cup2d_psb_pressure_rhs_device(cup2d_psb *h, const double *d_u, const double *d_v,
                                   const double *d_udef_u, const double *d_udef_v,
                                   const double *d_chi, double dt, double *d_rhs);

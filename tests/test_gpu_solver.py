@@ -221,6 +221,7 @@ def test_poisson_solver_interface(oracle):
     assert np.array_equal(sim.tmp.data.reshape(-1), b)
     # the operator is reused while the mesh is unchanged (ExpAMRSolver.cpp:370-380)
     assert solver.assemblies == 1
+    sim.pres.data[...] = 0.0  # x0 = 0 for every solve (PressureSingle.cpp:845)
     solver.solve(sim.tmp, sim.pres)
     assert solver.assemblies == 1 and solver.last_stats["converged"]
     # a new mesh object (the same blocks) and the mesh-changed flag re-assemble
@@ -229,6 +230,7 @@ def test_poisson_solver_interface(oracle):
     solver.solve(sim.tmp, sim.pres)
     assert solver.assemblies == 2
     sim.pres.UpdateFluxCorrection = True
+    sim.pres.data[...] = 0.0
     solver.solve(sim.tmp, sim.pres)
     assert solver.assemblies == 3 and not sim.pres.UpdateFluxCorrection
     tr = oracle.true_residual(A, b, sim.pres.data.reshape(-1))
@@ -265,10 +267,18 @@ def test_unfused_comparator_matches_oracle(oracle):
         xr = oracle.mean_remove(oracle.h2(m), xr)
         for norm in (PS.NORM_L2, PS.NORM_INF):
             s = PS.Solver(schedule=1, norm=norm, max_iter=4000).set_mesh(m)
+            # the stopping test reads the recurrence residual, as the
+            # reference's; at 1e-10 the true residual meets the tolerance
+            x, st = s.solve(b, rel_tol=1e-10)
+            tr = oracle.true_residual(A, b, x)
+            assert st["converged"], (name, st)
+            assert (tr["rel2"] if norm == PS.NORM_L2 else tr["relinf"]) <= 1.05e-10, tr
+            # solved to 1e-12 (where the recurrence and the true residual
+            # part by rounding: within 1e-11), the solution is the oracle's
             x, st = s.solve(b, rel_tol=1e-12)
             tr = oracle.true_residual(A, b, x)
             assert st["converged"], (name, st)
-            assert (tr["rel2"] if norm == PS.NORM_L2 else tr["relinf"]) <= 1.05e-12, tr
+            assert (tr["rel2"] if norm == PS.NORM_L2 else tr["relinf"]) <= 1e-11, tr
             assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
             # the stats' own true residual agrees with the oracle's
             assert abs(st["true_rel2"] - tr["rel2"]) <= 1e-3 * tr["rel2"] + 1e-16

@@ -1,11 +1,15 @@
 """CPU checks that pin the oracle's restatement of the operators either side
 of the pressure solve (oracle/projection.py): closed forms of linear fields,
-the ghost rules cell by cell, and periodic meshes against numpy.roll."""
+the ghost rules cell by cell, periodic meshes against numpy.roll, and at
+coarse-fine faces the face-form Laplacian against the assembled operator,
+exact differences of linear fields, the discrete divergence theorem and
+momentum conservation of the correction."""
 
 import numpy as np
 import pytest
 
 from cup2d_amd import mesh as M
+from cup2d_amd import workloads as W
 from oracle import projection as PR
 
 
@@ -78,9 +82,100 @@ def test_periodic_against_roll():
     assert np.array_equal(G(vo), G(v) + pf * dy(G(p)) * ih2)
 
 
-def test_coarse_fine_mesh_rejected():
-    m = M.from_blocks(2, 2, 2, 1.0, [(0, 0, 0), (0, 1, 0), (0, 1, 1), (1, 0, 2), (1, 1, 2),
-                                     (1, 0, 3), (1, 1, 3)])
+def _tiny_cf():
+    return M.from_blocks(2, 2, 2, 1.0, [(0, 0, 0), (0, 1, 0), (0, 1, 1), (1, 0, 2), (1, 1, 2),
+                                        (1, 0, 3), (1, 1, 3)])
+
+
+def _periodic_graded():
+    return M.graded(8, 4, 4, 2.0, [(0.1, 0.1, 0.05), (1.9, 0.9, 0.04), (1.0, 0.5, 0.1)],
+                    [0.3, 0.15, 0.08], True, True)
+
+
+CF_MESHES = {"tiny_cf": _tiny_cf, "graded": W.graded_disk_mesh, "periodic": _periodic_graded}
+
+
+@pytest.mark.parametrize("which", sorted(CF_MESHES))
+def test_face_form_laplacian_is_the_assembled_operator(oracle, which):
+    """The coarse-fine ghosts of the restatement, put into the face-form
+    Laplacian, give the assembled operator's rows (pinned by the known-answer
+    row and conservation) at every explicit cell."""
+    m = CF_MESHES[which]()
+    x = np.random.default_rng(2).normal(size=m.n)
+    ax = oracle.assemble(m).to_scipy() @ x
+    lf = PR.laplacian_face_form(m, x)
+    assert len(lf) == len(PR.explicit_cells(m)) > 0
+    err = max(abs(ax[i] - v) for i, v in lf.items())
+    assert err <= 1e-14 * np.abs(ax).max()
+
+
+def _away_from_walls(m, x, y):
+    h = m.cell_h()
+    Lx = m.extent * m.bpdx / max(m.bpdx, m.bpdy)
+    Ly = m.extent * m.bpdy / max(m.bpdx, m.bpdy)
+    return (x > h) & (x < Lx - h) & (y > h) & (y < Ly - h)
+
+
+def test_linear_fields_coarse_fine():
+    """Linear velocity and pressure are differentiated exactly at the
+    explicit cells too (the ghost interpolation is exact for linear fields
+    and the flux correction preserves it), away from the walls."""
+    m = W.graded_disk_mesh()
+    x, y = m.cell_centers()
+    a, b, dt = 0.7, -1.3, 0.01
     z = np.zeros(m.n)
-    with pytest.raises(ValueError):
-        PR.pressure_rhs(m, z, z, z, z, z, 1.0)
+    h = m.cell_h()
+    rhs = PR.pressure_rhs(m, a * x + 0.2, b * y - 0.1, z, z, z, dt)
+    ex = np.array([PR._gi(*c) for c in PR.explicit_cells(m)])
+    keep = ex[_away_from_walls(m, x, y)[ex]]
+    assert len(keep) > 100
+    ref = h * h * (a + b) / dt
+    assert np.allclose(rhs[keep], ref[keep], rtol=1e-11, atol=0)
+    al, be = 2.5, -0.4
+    uo, vo = PR.pressure_correct(m, al * x + be * y, z, z, dt)
+    assert np.allclose(uo[keep], -dt * al, rtol=1e-11, atol=0)
+    assert np.allclose(vo[keep], -dt * be, rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("which", ["graded", "periodic"])
+def test_divergence_theorem(which):
+    """sum of the RHS over the domain = the flux through its boundary: zero
+    for walls (the mirrored normal velocity cancels the face value) and for
+    periodic faces; with chi = 0 and with constant chi.  Coarse and fine
+    fluxes across every coarse-fine face must be equal and opposite for it
+    to hold."""
+    m = CF_MESHES[which]()
+    rng = np.random.default_rng(3)
+    u, v, ud, vd = (rng.normal(size=m.n) for _ in range(4))
+    for c in (0.0, 0.3):
+        r = PR.pressure_rhs(m, u, v, ud, vd, np.full(m.n, c), 0.01)
+        assert abs(r.sum()) <= 1e-15 * np.abs(r).sum() * 10
+
+
+def test_correction_conserves_momentum_periodic():
+    """On a periodic mesh the correction changes sum h^2 u by nothing: its
+    face fluxes telescope, coarse-fine faces included."""
+    m = _periodic_graded()
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=m.n)
+    z = np.zeros(m.n)
+    uo, vo = PR.pressure_correct(m, p, z, z, 0.02)
+    w = m.cell_h() ** 2
+    assert abs(np.dot(w, uo)) <= 1e-14 * np.dot(w, np.abs(uo))
+    assert abs(np.dot(w, vo)) <= 1e-14 * np.dot(w, np.abs(vo))
+
+
+def test_coarse_fine_plain_cells_unchanged():
+    """Cells off the coarse-fine faces are computed exactly as on a mesh
+    without such faces: the central formula of the reference."""
+    m = _tiny_cf()
+    rng = np.random.default_rng(5)
+    u, v, ud, vd, chi = (rng.normal(size=m.n) for _ in range(5))
+    dt, h = 0.1, m.cell_h()
+    rhs = PR.pressure_rhs(m, u, v, ud, vd, chi, dt)
+    # block 0 (level 0, covers [0, .5]^2), cell (3, 3): four in-block neighbours
+    i = 3 * 8 + 3
+    f = 0.5 * h[i] / dt
+    ref = f * ((u[i + 1] - u[i - 1]) + (v[i + 8] - v[i - 8]))
+    ref = ref + (-f) * chi[i] * ((ud[i + 1] - ud[i - 1]) + (vd[i + 8] - vd[i - 8]))
+    assert rhs[i] == ref
