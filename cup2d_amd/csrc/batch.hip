@@ -48,6 +48,9 @@ using namespace cup2d;
 namespace {
 
 constexpr int NQB = 7;        // update totals (NQ5) + sum h^2 x (initial mean row)
+#ifndef CUP2D_XCD_ITEMS
+#define CUP2D_XCD_ITEMS 0
+#endif
 constexpr int BGRID = 2048;   // workgroups of the item / tile kernels (grid-stride beyond)
 constexpr int MAX_INST = 4096;
 constexpr int MAX_ITEM_BLOCKS = 256;  // item_blocks keeps one block header per lane: 64 * WAVES
@@ -175,9 +178,20 @@ __device__ __forceinline__ int4 readlane4(int4 v, int l) {
 template <int UB = 0, class F>
 __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body,
                                               int filter = F_RUNNING) {
-  const int G = (int)gridDim.x, c = lane_id();
-  for (int base = blockIdx.x; base < V.n_items; base += 64 * G) {
-    const int cnt = min(64, (V.n_items - base + G - 1) / G);
+  // With a grid of a multiple of 8 workgroups, workgroup g runs on XCD
+  // g % 8: each XCD takes one contiguous eighth of the items, so that items
+  // next to each other (neighbour blocks) share that XCD's L2.
+  const int NG = (int)gridDim.x, c = lane_id();
+  const bool xm = CUP2D_XCD_ITEMS && (NG & 7) == 0;
+  const int G = xm ? NG >> 3 : NG;
+  int first = blockIdx.x, hi = V.n_items;
+  if (xm) {
+    const int C = (V.n_items + 7) >> 3, lo = min(V.n_items, ((int)blockIdx.x & 7) * C);
+    hi = min(V.n_items, lo + C);
+    first = lo + ((int)blockIdx.x >> 3);
+  }
+  for (int base = first; base < hi; base += 64 * G) {
+    const int cnt = min(64, (hi - base + G - 1) / G);
     const int mine = base + c * G;
     const int4 ml = V.item[c < cnt ? mine : base];
     auto want = [&](int4 mt) { return !st || wanted(st[mt.x], filter); };
@@ -230,18 +244,20 @@ __device__ __forceinline__ void item_blocks(const BView &V, const TV *__restrict
 }
 
 // ----------------------------------------------------------------------------
-// Row kernels of the operator: a thread per block row.  Thread t of the
-// workgroup owns row iy = t & 7 of block t >> 3 of a pass of RB = 32 blocks
-// of the item: its eight values come in with 16-byte loads (contiguous over
-// the workgroup), the horizontal neighbours inside the row stay in
-// registers, the two values across the block's x faces are single loads,
-// and the rows above and below come from an LDS tile holding, per block, its
-// eight rows plus the halo rows across its y faces.  A wall face's halo is
-// the cell itself (its term x_nb - x_c vanishes), so the stencil is the same
-// five terms for every plain cell; explicit (coarse-fine) cells are left to
-// the explicit items.
+// Row kernels of the operator: two threads per block row, four cells each.
+// Thread t of the workgroup owns half h = t & 1 of row iy = (t >> 1) & 7 of
+// block t >> 4 of a pass of RB = 16 blocks of the item: its four values come
+// in with 16-byte loads (contiguous over the workgroup), the horizontal
+// neighbours inside the half row stay in registers, the one across the
+// middle of the row is swapped with the partner thread (DPP, no LDS), the
+// one across the block's x face (the West face for h = 0, East for h = 1) is
+// a single load, and the half rows above and below come from an LDS tile
+// holding, per block, its eight rows plus the halo rows across its y faces.
+// A wall face's halo is the cell itself (its term x_nb - x_c vanishes), so
+// the stencil is the same five terms for every plain cell; explicit
+// (coarse-fine) cells are left to the explicit items.
 // ----------------------------------------------------------------------------
-constexpr int RB = 32;            // blocks per pass: WG threads = RB blocks x 8 rows
+constexpr int RB = WG / 16;       // blocks per pass: 16 threads per block
 constexpr int TROWS = RB * 10;    // LDS rows per chunk column: row below, 8 rows, row above
 
 template <class T>
@@ -262,10 +278,11 @@ __device__ __forceinline__ void split(float4 c, float *v) { v[0] = c.x; v[1] = c
 __device__ __forceinline__ double2 join(const double *v) { return make_double2(v[0], v[1]); }
 __device__ __forceinline__ float4 join(const float *v) { return make_float4(v[0], v[1], v[2], v[3]); }
 
+// the four cells base .. base + 3 (16-byte aligned)
 template <class T>
-__device__ __forceinline__ void ld_row(const T *__restrict__ x, long base, T (&v)[8]) {
+__device__ __forceinline__ void ld4c(const T *__restrict__ x, long base, T (&v)[4]) {
   typedef typename Chunk<T>::type C;
-  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+  constexpr int NC = Chunk<T>::n / 2, W = 4 / NC;
   const C *p = reinterpret_cast<const C *>(x + base);
   C c[NC];
 #pragma unroll
@@ -274,44 +291,56 @@ __device__ __forceinline__ void ld_row(const T *__restrict__ x, long base, T (&v
   for (int k = 0; k < NC; k++) split(c[k], v + W * k);
 }
 template <class T>
-__device__ __forceinline__ void st_row(T *__restrict__ x, long base, const T (&v)[8]) {
+__device__ __forceinline__ void st4c(T *__restrict__ x, long base, const T (&v)[4]) {
   typedef typename Chunk<T>::type C;
-  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+  constexpr int NC = Chunk<T>::n / 2, W = 4 / NC;
   C *p = reinterpret_cast<C *>(x + base);
 #pragma unroll
   for (int k = 0; k < NC; k++) p[k] = join(v + W * k);
 }
-// LDS tile: chunk k of row r at tile[k * TROWS + r] (consecutive threads
-// touch consecutive 16-byte slots: no bank conflicts)
+// LDS tile: chunk k of row r at tile[k * TROWS + r]; half h of a row is
+// chunks h NC / 2 .. (h + 1) NC / 2 - 1 (consecutive threads touch
+// consecutive 16-byte slots of a chunk column: no bank conflicts)
 template <class T>
-__device__ __forceinline__ void st_tile(typename Chunk<T>::type *tile, int r, const T (&v)[8]) {
-  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+__device__ __forceinline__ void st_tile(typename Chunk<T>::type *tile, int r, int h, const T (&v)[4]) {
+  constexpr int NC = Chunk<T>::n / 2, W = 4 / NC;
 #pragma unroll
-  for (int k = 0; k < NC; k++) tile[k * TROWS + r] = join(v + W * k);
+  for (int k = 0; k < NC; k++) tile[(h * NC + k) * TROWS + r] = join(v + W * k);
 }
 template <class T>
-__device__ __forceinline__ void ld_tile(const typename Chunk<T>::type *tile, int r, T (&v)[8]) {
-  constexpr int NC = Chunk<T>::n, W = 8 / NC;
+__device__ __forceinline__ void ld_tile(const typename Chunk<T>::type *tile, int r, int h, T (&v)[4]) {
+  constexpr int NC = Chunk<T>::n / 2, W = 4 / NC;
 #pragma unroll
-  for (int k = 0; k < NC; k++) split(tile[k * TROWS + r], v + W * k);
+  for (int k = 0; k < NC; k++) split(tile[(h * NC + k) * TROWS + r], v + W * k);
+}
+
+// v of the partner thread (lane ^ 1), through DPP
+__device__ __forceinline__ double swap_pair(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xb1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xb1, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float swap_pair(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xb1, 0xf, 0xf, false));
 }
 
 // LDS of the row kernels' tile
 template <class T>
 constexpr int row_tile_bytes() { return Chunk<T>::n * TROWS * 16; }
 
-// Calls f(base, y, x, a, h2, emask) for each row of the item's blocks
-// (b0 .. b0 + nbk - 1): base = first global index of the row, y = the plain
-// stencil of its eight cells (FP64), x / a = the row of x / aux, h2 = h^2
-// of the block, emask = the row's explicit-cell bits.  Every thread of the
-// workgroup must call it (it synchronizes the workgroup).
+// Calls f(base, y, x, a, h2, emask) for each half row of the item's blocks
+// (b0 .. b0 + nbk - 1): base = first global index of the four cells, y =
+// their plain stencil (FP64), x / a = the four values of x / aux, h2 = h^2
+// of the block, emask = the four cells' explicit-cell bits.  Every thread
+// of the workgroup must call it (it synchronizes the workgroup).
 template <class T, bool AUX = true, class F>
 __device__ __forceinline__ void row_blocks(const BView &V, const T *__restrict__ x,
                                            const T *__restrict__ aux, int b0, int nbk,
                                            unsigned char *tile_raw, F &&f) {
   typedef typename Chunk<T>::type C;
   C *tile = reinterpret_cast<C *>(tile_raw);
-  const int t = threadIdx.x, jl = t >> 3, iy = t & 7;
+  const int t = threadIdx.x, jl = t >> 4, iy = (t >> 1) & 7, hf = t & 1;
   const bool edge = iy == 0 || iy == 7;
   for (int s0 = 0; s0 < nbk; s0 += RB) {
     const int j = s0 + jl;
@@ -320,43 +349,49 @@ __device__ __forceinline__ void row_blocks(const BView &V, const T *__restrict__
     const int4 nb = V.A.nbr[b];
     const unsigned long long m = V.A.mask[b];
     const double h2 = V.A.h2[b];
-    const long base = (long)b * 64 + iy * 8;
-    T xr[8], ar[8], hr[8];
-    ld_row(x, base, xr);
-    if constexpr (AUX) ld_row(aux, base, ar);
-    const T xl = x[nb.x >= 0 ? (long)nb.x * 64 + iy * 8 + 7 : base];
-    const T xe = x[nb.y >= 0 ? (long)nb.y * 64 + iy * 8 : base + 7];
+    const long base = (long)b * 64 + iy * 8 + 4 * hf;
+    T xr[4], ar[4], hr[4];
+    ld4c(x, base, xr);
+    if constexpr (AUX) ld4c(aux, base, ar);
+    // across the x face: West (col 7 of nb.x) for the low half, East (col 0
+    // of nb.y) for the high half; the cell itself at a wall
+    const int nx = hf ? nb.y : nb.x;
+    const T xf = x[nx >= 0 ? (long)nx * 64 + iy * 8 + (hf ? 0 : 7) : base + (hf ? 3 : 0)];
     if (edge) {
       const int nv = iy == 0 ? nb.z : nb.w;
-      ld_row(x, nv >= 0 ? (long)nv * 64 + (iy == 0 ? 56 : 0) : base, hr);
+      ld4c(x, nv >= 0 ? (long)nv * 64 + (iy == 0 ? 56 : 0) + 4 * hf : base, hr);
     }
-    st_tile(tile, 10 * jl + iy + 1, xr);
-    if (edge) st_tile(tile, 10 * jl + (iy == 0 ? 0 : 9), hr);
+    st_tile(tile, 10 * jl + iy + 1, hf, xr);
+    if (edge) st_tile(tile, 10 * jl + (iy == 0 ? 0 : 9), hf, hr);
+    // the partner's cell next to the middle of the row
+    const T xm = swap_pair(hf ? xr[0] : xr[3]);
     __syncthreads();
-    T dn[8], up[8];
-    ld_tile(tile, 10 * jl + iy, dn);
-    ld_tile(tile, 10 * jl + iy + 2, up);
+    T dn[4], up[4];
+    ld_tile(tile, 10 * jl + iy, hf, dn);
+    ld_tile(tile, 10 * jl + iy + 2, hf, up);
     __syncthreads();  // the tile is rewritten by the next pass
-    double y[8];
+    const double lo = hf ? (double)xm : (double)xf;   // left of cell 0
+    const double hi = hf ? (double)xf : (double)xm;   // right of cell 3
+    double y[4];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double l = k > 0 ? (double)xr[k - 1] : (double)xl;
-      const double r = k < 7 ? (double)xr[k + 1] : (double)xe;
+    for (int k = 0; k < 4; k++) {
+      const double l = k > 0 ? (double)xr[k - 1] : lo;
+      const double r = k < 3 ? (double)xr[k + 1] : hi;
       y[k] = (((double)dn[k] + l) + (r + (double)up[k])) - 4.0 * (double)xr[k];
     }
-    if (ok) f(base, y, xr, ar, h2, (unsigned)(m >> (8 * iy)) & 0xffu);
+    if (ok) f(base, y, xr, ar, h2, (unsigned)(m >> (8 * iy + 4 * hf)) & 0xfu);
   }
 }
 
-// Stores the row's plain cells (all eight with one set of 16-byte stores
-// when the row has no explicit cell).
+// Stores the plain ones of four cells (all with one set of 16-byte stores
+// when none is explicit).
 template <class T>
-__device__ __forceinline__ void st_plain(T *__restrict__ y, long base, const T (&v)[8], unsigned em) {
+__device__ __forceinline__ void st_plain(T *__restrict__ y, long base, const T (&v)[4], unsigned em) {
   if (em == 0) {
-    st_row(y, base, v);
+    st4c(y, base, v);
   } else {
 #pragma unroll
-    for (int k = 0; k < 8; k++)
+    for (int k = 0; k < 4; k++)
       if (!((em >> k) & 1u)) y[base + k] = v[k];
   }
 }
@@ -419,18 +454,18 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
     double v[2] = {0.0, 0.0};
     if (mt.z > 0) {
       row_blocks<T>(V, z1, rhat, mt.y, mt.z, tile,
-                    [&](long base, const double (&y)[8], const T (&xr)[8], const T (&ar)[8], double h2,
+                    [&](long base, const double (&y)[4], const T (&xr)[4], const T (&ar)[4], double h2,
                         unsigned em) {
                       double hs = 0.0;
-                      T yt[8];
+                      T yt[4];
 #pragma unroll
-                      for (int k = 0; k < 8; k++) {
+                      for (int k = 0; k < 4; k++) {
                         hs += (double)xr[k];
                         yt[k] = (T)y[k];
                       }
                       if (base == mr) yt[0] = T(0);  // the mean row (cell 0 of its block): kb_alpha
 #pragma unroll
-                      for (int k = 0; k < 8; k++)
+                      for (int k = 0; k < 4; k++)
                         v[0] += ((em >> k) & 1u) ? 0.0 : (double)ar[k] * (double)yt[k];
                       v[1] += h2 * hs;
                       st_plain(nu, base, yt, em);
@@ -458,18 +493,18 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
     double v[3] = {0.0, 0.0, 0.0};
     if (mt.z > 0) {
       row_blocks<T>(V, z2, sv, mt.y, mt.z, tile,
-                    [&](long base, const double (&y)[8], const T (&xr)[8], const T (&ar)[8], double h2,
+                    [&](long base, const double (&y)[4], const T (&xr)[4], const T (&ar)[4], double h2,
                         unsigned em) {
                       double hs = 0.0;
-                      T yt[8];
+                      T yt[4];
 #pragma unroll
-                      for (int k = 0; k < 8; k++) {
+                      for (int k = 0; k < 4; k++) {
                         hs += (double)xr[k];
                         yt[k] = (T)y[k];
                       }
                       if (base == mr) yt[0] = T(0);  // the mean row: kb_omega
 #pragma unroll
-                      for (int k = 0; k < 8; k++) {
+                      for (int k = 0; k < 4; k++) {
                         const double yd = ((em >> k) & 1u) ? 0.0 : (double)yt[k];
                         v[0] += yd * (double)ar[k];
                         v[1] += yd * yd;
@@ -500,11 +535,11 @@ __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict_
     double v[1] = {0.0};
     if (mt.z > 0) {
       row_blocks<double, false>(V, x, x, mt.y, mt.z, tile,
-                         [&](long base, const double (&yr)[8], const double (&xr)[8], const double (&)[8],
+                         [&](long base, const double (&yr)[4], const double (&xr)[4], const double (&)[4],
                              double h2, unsigned em) {
                            double hs = 0.0;
 #pragma unroll
-                           for (int k = 0; k < 8; k++) hs += xr[k];
+                           for (int k = 0; k < 4; k++) hs += xr[k];
                            v[0] += h2 * hs;
                            st_plain(y, base, yr, em);
                          });
@@ -526,9 +561,28 @@ __global__ __launch_bounds__(WG) void kb_meanrow(BView V, const double *pm, doub
   if (threadIdx.x == 0) y[mr] = o[0];
 }
 
+// Two consecutive cells of a vector (16-byte loads for FP64, 8-byte for
+// FP32), as doubles.
+__device__ __forceinline__ void ld2(const double *__restrict__ a, long i, double (&v)[2]) {
+  const double2 c = *reinterpret_cast<const double2 *>(a + i);
+  v[0] = c.x;
+  v[1] = c.y;
+}
+__device__ __forceinline__ void ld2(const float *__restrict__ a, long i, double (&v)[2]) {
+  const float2 c = *reinterpret_cast<const float2 *>(a + i);
+  v[0] = (double)c.x;
+  v[1] = (double)c.y;
+}
+__device__ __forceinline__ void st2(double *__restrict__ a, long i, const double (&v)[2]) {
+  *reinterpret_cast<double2 *>(a + i) = make_double2(v[0], v[1]);
+}
+__device__ __forceinline__ void st2(float *__restrict__ a, long i, const float (&v)[2]) {
+  *reinterpret_cast<float2 *>(a + i) = make_float2(v[0], v[1]);
+}
+
 // x_opt = x when the previous iterate was the best so far; x += alpha z1 +
-// omega z2 (FP64 x); r = s - omega t; update partials.  Four cells per
-// thread per pass, all loads first.
+// omega z2 (FP64 x); r = s - omega t; update partials.  Two pairs of
+// consecutive cells per thread per pass (vector loads), all loads first.
 template <class T>
 __global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ x,
                                                   double *__restrict__ xopt,
@@ -537,40 +591,47 @@ __global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ 
                                                   const T *__restrict__ t,
                                                   const T *__restrict__ rhat, const Scal *st,
                                                   double *p5) {
+  constexpr int NP = 2;  // pairs per thread per pass
   running_items(V, st, [&](int item, int4 mt, const Heads &) {
     const double alpha = st[mt.x].alpha, omega = st[mt.x].omega;
     const int improved = st[mt.x].improved;
     double v[NQB] = {0, 0, 0, 0, 0, 0, 0};
     const long i0 = (long)mt.y * 64, ncell = mt.z > 0 ? (long)mt.z * 64 : 0;
-    for (long off = threadIdx.x; off < ncell; off += 4 * WG) {
-      long ii[4];
-      bool ok[4];
-      double xo[4], a1[4], a2[4], rr[4], tt[4], rh[4];
+    for (long off = 2 * (long)threadIdx.x; off < ncell; off += 2 * NP * WG) {
+      long ii[NP];
+      bool ok[NP];
+      double xo[NP][2], a1[NP][2], a2[NP][2], rr[NP][2], tt[NP][2], rh[NP][2];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        ok[q] = off + q * WG < ncell;
-        ii[q] = i0 + (ok[q] ? off + q * WG : off);
-        xo[q] = x[ii[q]];
-        a1[q] = (double)z1[ii[q]];
-        a2[q] = (double)z2[ii[q]];
-        rr[q] = (double)r[ii[q]];
-        tt[q] = (double)t[ii[q]];
-        rh[q] = (double)rhat[ii[q]];
+      for (int q = 0; q < NP; q++) {
+        ok[q] = off + 2 * q * WG < ncell;
+        ii[q] = i0 + (ok[q] ? off + 2 * q * WG : off);
+        ld2(x, ii[q], xo[q]);
+        ld2(z1, ii[q], a1[q]);
+        ld2(z2, ii[q], a2[q]);
+        ld2(r, ii[q], rr[q]);
+        ld2(t, ii[q], tt[q]);
+        ld2(rhat, ii[q], rh[q]);
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < NP; q++) {
         if (!ok[q]) continue;
-        if (improved) xopt[ii[q]] = xo[q];
-        x[ii[q]] = xo[q] + alpha * a1[q] + omega * a2[q];
-        const T rt = (T)(rr[q] - omega * tt[q]);
-        r[ii[q]] = rt;
-        const double rd = (double)rt;
-        v[0] = nmax(v[0], fabs(rd));
-        v[1] += rd * rd;
-        v[2] += rh[q] * rd;
-        v[3] += rh[q] * rh[q];
-        v[4] += rd;
-        v[5] += rh[q];
+        if (improved) st2(xopt, ii[q], xo[q]);
+        double xn[2];
+        T rt[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+          xn[k] = xo[q][k] + alpha * a1[q][k] + omega * a2[q][k];
+          rt[k] = (T)(rr[q][k] - omega * tt[q][k]);
+          const double rd = (double)rt[k];
+          v[0] = nmax(v[0], fabs(rd));
+          v[1] += rd * rd;
+          v[2] += rh[q][k] * rd;
+          v[3] += rh[q][k] * rh[q][k];
+          v[4] += rd;
+          v[5] += rh[q][k];
+        }
+        st2(x, ii[q], xn);
+        st2(r, ii[q], rt);
       }
     }
     store_item_partials<NQB, 1u>(v, p5, V.n_items, item);
@@ -688,26 +749,6 @@ __device__ __forceinline__ void running_tiles(const BView &V, const Scal *st, F 
   }
 }
 
-// Calls body(tm) for the workgroup's tiles (grid-stride over workgroups)
-// whose instance is iterating; the next tile's meta is loaded while the
-// current one is processed.  The skip is uniform over the workgroup, so body
-// may synchronize it.
-template <class F>
-__device__ __forceinline__ void running_tiles_wg(const BView &V, const Scal *st, F &&body) {
-  int tile = blockIdx.x;
-  if (tile >= V.n_tiles) return;
-  int4 tm = V.tile[tile];
-  for (;;) {
-    const int next = tile + gridDim.x;
-    const bool more = next < V.n_tiles;
-    const int4 ntm = V.tile[more ? next : tile];
-    if (!st[tm.x].idle) body(tm);
-    if (!more) break;
-    tile = next;
-    tm = ntm;
-  }
-}
-
 // The thread's four cells of tile tm (see tile_prec64): first index and
 // whether its block exists (a lane past the tile's end addresses the
 // tile's first block).
@@ -717,7 +758,8 @@ __device__ __forceinline__ long btile_cells(int4 tm, int w, int lane, bool &ok) 
 }
 
 // FP64: p = r + beta (p - omega nu), or on restart p = rhat = r; z1 = P p
-// on MFMA (a workgroup per 16-block tile)
+// on MFMA (a workgroup per 16-block tile, tiles grid-stride; the next tile's
+// loads are in flight during the current tile's product)
 __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__restrict__ pf,
                                                    const double *__restrict__ r,
                                                    double *__restrict__ rhat,
@@ -726,57 +768,105 @@ __global__ __launch_bounds__(WG) void kb_pupdate64(BView V, const double *__rest
                                                    double *__restrict__ z1, const Scal *st) {
   __shared__ double tile[TILE][TROW];
   const int lane = lane_id(), w = wave_id();
+  int t = blockIdx.x;
+  if (t >= V.n_tiles) return;
   PFrag64 F;
   load_pfrag64(F, pf, w, lane);
-  running_tiles_wg(V, st, [&](int4 tm) {
-    const int k = tm.x;
-    const double beta = st[k].beta, omega = st[k].omega;
-    const int restart = st[k].restart;
-    bool ok;
-    const long i0 = btile_cells(tm, w, lane, ok);
-    double rv[4], pv[4], nv[4], pi[4];
+  int4 tm = V.tile[t];
+  bool run = !st[tm.x].idle;  // uniform over the workgroup
+  bool ok = false;
+  long i0 = 0;
+  double rv[4], pv[4], nv[4];
+  auto load = [&](int4 m) {
+    i0 = btile_cells(m, w, lane, ok);
     ld4(r, i0, rv);
     ld4(p, i0, pv);
     ld4(nu, i0, nv);
+  };
+  if (run) load(tm);
+  for (;;) {
+    const int nt = t + (int)gridDim.x;
+    const bool more = nt < V.n_tiles;
+    const int4 ntm = V.tile[more ? nt : t];
+    const bool nrun = more && !st[ntm.x].idle;
+    if (run) {
+      const int k = tm.x;
+      const double beta = st[k].beta, omega = st[k].omega;
+      const int restart = st[k].restart;
+      double pi[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) pi[q] = restart ? rv[q] : rv[q] + beta * (pv[q] - omega * nv[q]);
-    if (ok) {
-      if (restart) st4(rhat, i0, rv);
-      st4(p, i0, pi);
+      for (int q = 0; q < 4; q++) pi[q] = restart ? rv[q] : rv[q] + beta * (pv[q] - omega * nv[q]);
+      if (ok) {
+        if (restart) st4(rhat, i0, rv);
+        st4(p, i0, pi);
+      }
+      tile_put(tile, w, lane, ok, pi);
+      const bool oo = ok;
+      const long io = i0;
+      if (nrun) load(ntm);  // in flight during the product
+      __syncthreads();
+      const f64x4 z = tile_prec64(tile, F, lane);
+      __syncthreads();  // the tile is rewritten by the next tile
+      if (oo) st4(z1, io, z);
+    } else if (nrun) {
+      load(ntm);
     }
-    tile_put(tile, w, lane, ok, pi);
-    __syncthreads();
-    const f64x4 z = tile_prec64(tile, F, lane);
-    __syncthreads();  // the tile is rewritten by the next tile
-    if (ok) st4(z1, i0, z);
-  });
+    if (!more) break;
+    t = nt;
+    tm = ntm;
+    run = nrun;
+  }
 }
 
-// FP64: s = r - alpha nu (stored in r); z2 = P s on MFMA
+// FP64: s = r - alpha nu (stored in r); z2 = P s on MFMA (as kb_pupdate64)
 __global__ __launch_bounds__(WG) void kb_supdate64(BView V, const double *__restrict__ pf,
                                                    double *__restrict__ r,
                                                    const double *__restrict__ nu,
                                                    double *__restrict__ z2, const Scal *st) {
   __shared__ double tile[TILE][TROW];
   const int lane = lane_id(), w = wave_id();
+  int t = blockIdx.x;
+  if (t >= V.n_tiles) return;
   PFrag64 F;
   load_pfrag64(F, pf, w, lane);
-  running_tiles_wg(V, st, [&](int4 tm) {
-    const double alpha = st[tm.x].alpha;
-    bool ok;
-    const long i0 = btile_cells(tm, w, lane, ok);
-    double rv[4], nv[4], si[4];
+  int4 tm = V.tile[t];
+  bool run = !st[tm.x].idle;
+  bool ok = false;
+  long i0 = 0;
+  double rv[4], nv[4];
+  auto load = [&](int4 m) {
+    i0 = btile_cells(m, w, lane, ok);
     ld4(r, i0, rv);
     ld4(nu, i0, nv);
+  };
+  if (run) load(tm);
+  for (;;) {
+    const int nt = t + (int)gridDim.x;
+    const bool more = nt < V.n_tiles;
+    const int4 ntm = V.tile[more ? nt : t];
+    const bool nrun = more && !st[ntm.x].idle;
+    if (run) {
+      const double alpha = st[tm.x].alpha;
+      double si[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) si[q] = rv[q] - alpha * nv[q];
-    if (ok) st4(r, i0, si);
-    tile_put(tile, w, lane, ok, si);
-    __syncthreads();
-    const f64x4 z = tile_prec64(tile, F, lane);
-    __syncthreads();
-    if (ok) st4(z2, i0, z);
-  });
+      for (int q = 0; q < 4; q++) si[q] = rv[q] - alpha * nv[q];
+      if (ok) st4(r, i0, si);
+      tile_put(tile, w, lane, ok, si);
+      const bool oo = ok;
+      const long io = i0;
+      if (nrun) load(ntm);
+      __syncthreads();
+      const f64x4 z = tile_prec64(tile, F, lane);
+      __syncthreads();
+      if (oo) st4(z2, io, z);
+    } else if (nrun) {
+      load(ntm);
+    }
+    if (!more) break;
+    t = nt;
+    tm = ntm;
+    run = nrun;
+  }
 }
 
 // FP32 storage, BF16 MFMA: r -= mu (projection); p update or restart;

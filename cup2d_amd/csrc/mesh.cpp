@@ -13,12 +13,23 @@
 // result is then balanced so that face- and corner-adjacent leaves differ by
 // at most one level, which the coarse-fine rows of the operator assume
 // (ExpAMRSolver.cpp:177).
+//
+// Adaptation (cup2d_mesh_adapt) changes an existing mesh from per-block
+// tags, the decision Cubism's MeshAdaptation takes for AdaptTheMesh
+// (AdaptTheMesh.cpp:73-122; the tagging criterion is computed on the GPU):
+// blocks tagged for refinement are split (below the top level); four
+// sibling leaves all tagged for compression are merged when the parent would
+// keep the 2:1 balance with the refined mesh; then the tree is balanced
+// again.  Every new block records where its values come from: an old block
+// unchanged, an old ancestor (refinement, one level or more), or four old
+// children (compression).
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -118,6 +129,24 @@ uint64_t hilbert(int order, uint64_t x, uint64_t y) {
 
 std::string g_err;
 
+// Hilbert order of a leaf set at the finest of `levels` levels.
+std::vector<uint64_t> hilbert_order(const Gen &G) {
+  int order = 1;
+  const int L = G.levels - 1;
+  while ((1LL << order) < (long long)std::max(G.bpdx, G.bpdy) << L) order++;
+  std::vector<std::pair<uint64_t, uint64_t>> ord;
+  ord.reserve(G.leaves.size());
+  for (uint64_t k : G.leaves) {
+    const int d = L - Key::level(k);
+    const uint64_t x = uint64_t(Key::i(k)) << d, y = uint64_t(Key::j(k)) << d;
+    ord.push_back({hilbert(order, x, y), k});
+  }
+  std::sort(ord.begin(), ord.end());
+  std::vector<uint64_t> out(ord.size());
+  for (size_t b = 0; b < ord.size(); b++) out[b] = ord[b].second;
+  return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -166,22 +195,139 @@ long long cup2d_mesh_graded(int bpdx, int bpdy, int levels, double extent, int p
   const long long nb = (long long)G.leaves.size();
   if (nb > cap || !level || !bx || !by) return nb;
 
-  int order = 1;
-  const int L = levels - 1;
-  while ((1LL << order) < (long long)std::max(bpdx, bpdy) << L) order++;
-  std::vector<std::pair<uint64_t, uint64_t>> ord;
-  ord.reserve(nb);
-  for (uint64_t k : G.leaves) {
-    const int d = L - Key::level(k);
-    const uint64_t x = uint64_t(Key::i(k)) << d, y = uint64_t(Key::j(k)) << d;
-    ord.push_back({hilbert(order, x, y), k});
-  }
-  std::sort(ord.begin(), ord.end());
+  const std::vector<uint64_t> ord = hilbert_order(G);
   for (long long b = 0; b < nb; b++) {
-    const uint64_t k = ord[b].second;
+    const uint64_t k = ord[b];
     level[b] = Key::level(k);
     bx[b] = (int)Key::i(k);
     by[b] = (int)Key::j(k);
+  }
+  return nb;
+}
+
+long long cup2d_mesh_adapt(int bpdx, int bpdy, int levels, int periodic_x, int periodic_y,
+                           long long nb_old, const int *level_old, const int *bx_old,
+                           const int *by_old, const signed char *tag, int *level, int *bx, int *by,
+                           int *src_kind, long long *src, long long cap) {
+  if (bpdx <= 0 || bpdy <= 0 || levels <= 0 || levels > 16 || nb_old <= 0 || !level_old || !bx_old ||
+      !by_old || !tag) {
+    g_err = "cup2d_mesh_adapt: invalid arguments";
+    return -1;
+  }
+  Gen G;
+  G.bpdx = bpdx;
+  G.bpdy = bpdy;
+  G.levels = levels;
+  G.perx = periodic_x != 0;
+  G.pery = periodic_y != 0;
+  G.s0 = 1.0;
+  std::unordered_map<uint64_t, long long> old_id;
+  old_id.reserve(size_t(nb_old) * 2);
+  for (long long b = 0; b < nb_old; b++) {
+    const int l = level_old[b];
+    if (l < 0 || l >= levels || bx_old[b] < 0 || bx_old[b] >= (bpdx << l) || by_old[b] < 0 ||
+        by_old[b] >= (bpdy << l)) {
+      g_err = "cup2d_mesh_adapt: block " + std::to_string(b) + " out of range";
+      return -1;
+    }
+    const uint64_t k = Key::make(l, bx_old[b], by_old[b]);
+    if (!old_id.emplace(k, b).second) {
+      g_err = "cup2d_mesh_adapt: block " + std::to_string(b) + " duplicate";
+      return -1;
+    }
+    G.leaves.insert(k);
+  }
+  // 1. refinement of the tagged blocks below the top level
+  for (long long b = 0; b < nb_old; b++)
+    if (tag[b] > 0 && level_old[b] + 1 < levels) G.split(Key::make(level_old[b], bx_old[b], by_old[b]));
+  G.balance();
+  // 2. compression: four sibling leaves, all old blocks tagged -1, whose
+  // parent keeps every face and corner neighbour within one level
+  std::vector<uint64_t> cand;
+  for (long long b = 0; b < nb_old; b++) {
+    const int l = level_old[b];
+    if (tag[b] < 0 && l > 0 && (bx_old[b] & 1) == 0 && (by_old[b] & 1) == 0)
+      cand.push_back(Key::make(l - 1, bx_old[b] >> 1, by_old[b] >> 1));
+  }
+  std::sort(cand.begin(), cand.end());
+  for (uint64_t pk : cand) {
+    const int l = Key::level(pk) + 1;
+    const long long pi = Key::i(pk), pj = Key::j(pk);
+    bool all = true;
+    uint64_t kid[4];
+    for (int q = 0; q < 4 && all; q++) {
+      kid[q] = Key::make(l, 2 * pi + (q & 1), 2 * pj + (q >> 1));
+      auto it = old_id.find(kid[q]);
+      all = it != old_id.end() && tag[it->second] < 0 && G.leaves.count(kid[q]);
+    }
+    if (!all) continue;
+    // the parent at level l - 1: every leaf touching it must be at level <= l
+    bool ok = true;
+    const int pl = l - 1;
+    for (int dj = -1; dj <= 1 && ok; dj++)
+      for (int di = -1; di <= 1 && ok; di++) {
+        if (!di && !dj) continue;
+        long long ni = pi + di, nj = pj + dj;
+        const long long nx = (long long)bpdx << pl, ny = (long long)bpdy << pl;
+        if (ni < 0 || ni >= nx) {
+          if (!G.perx) continue;
+          ni = (ni + nx) % nx;
+        }
+        if (nj < 0 || nj >= ny) {
+          if (!G.pery) continue;
+          nj = (nj + ny) % ny;
+        }
+        // the neighbour region at level pl is covered by a leaf at level <= pl
+        // or subdivided: every leaf inside it must be at level <= pl + 1
+        if (G.coveringLevel(pl, ni, nj) >= 0) continue;
+        for (int q = 0; q < 4 && ok; q++) {
+          const long long ci = 2 * ni + (q & 1), cj = 2 * nj + (q >> 1);
+          if (!G.leaves.count(Key::make(pl + 1, ci, cj))) ok = false;  // deeper than pl + 1
+        }
+      }
+    if (!ok) continue;
+    for (int q = 0; q < 4; q++) G.leaves.erase(kid[q]);
+    G.leaves.insert(pk);
+  }
+  G.balance();
+
+  const long long nb = (long long)G.leaves.size();
+  if (nb > cap || !level || !bx || !by || !src_kind || !src) return nb;
+  const std::vector<uint64_t> ord = hilbert_order(G);
+  for (long long b = 0; b < nb; b++) {
+    const uint64_t k = ord[b];
+    const int l = Key::level(k);
+    const long long i = Key::i(k), j = Key::j(k);
+    level[b] = l;
+    bx[b] = (int)i;
+    by[b] = (int)j;
+    src[4 * b] = src[4 * b + 1] = src[4 * b + 2] = src[4 * b + 3] = -1;
+    auto it = old_id.find(k);
+    if (it != old_id.end()) {  // unchanged
+      src_kind[b] = 0;
+      src[4 * b] = it->second;
+      continue;
+    }
+    bool found = false;
+    for (int m = l - 1; m >= 0 && !found; m--) {  // an old ancestor: refinement
+      const int d = l - m;
+      auto a = old_id.find(Key::make(m, i >> d, j >> d));
+      if (a != old_id.end()) {
+        src_kind[b] = 1;
+        src[4 * b] = a->second;
+        found = true;
+      }
+    }
+    if (found) continue;
+    src_kind[b] = 2;  // four old children: compression
+    for (int q = 0; q < 4; q++) {
+      auto c = old_id.find(Key::make(l + 1, 2 * i + (q & 1), 2 * j + (q >> 1)));
+      if (c == old_id.end()) {
+        g_err = "cup2d_mesh_adapt: internal error (a new block has no source)";
+        return -1;
+      }
+      src[4 * b + q] = c->second;
+    }
   }
   return nb;
 }

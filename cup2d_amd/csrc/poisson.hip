@@ -36,6 +36,7 @@
 #include "assemble_device.h"
 #include "assembly.h"
 #include "kernels.h"
+#include "flow.h"
 
 using namespace cup2d;
 
@@ -884,6 +885,8 @@ struct cup2d_ps {
   float *f_r = nullptr, *f_rhat = nullptr, *f_p = nullptr, *f_nu = nullptr, *f_z1 = nullptr,
         *f_z2 = nullptr, *f_t = nullptr;
   Scal *d_s = nullptr;                   // two state slots (iteration parity)
+  int2 *d_bxy = nullptr;                 // [nb] block index (bx, by) at its level
+  double h_min = 0.0;                    // smallest cell size of the mesh
   double *d_u = nullptr;                 // scalars of the unfused comparator (UScal + constants)
   double *d_p2 = nullptr, *d_p4 = nullptr, *d_p5 = nullptr, *d_pm = nullptr, *d_pfin = nullptr;
   Scal *h_s = nullptr;                   // pinned copy of the state slots
@@ -1599,8 +1602,14 @@ static int finish_mesh(cup2d_ps *h);
 static int upload_h(cup2d_ps *h, const cup2d_mesh_desc *m) {
   const double h0 = m->extent / (double(std::max(m->bpdx, m->bpdy)) * BS);
   std::vector<double> hb((size_t)m->n_blocks);
-  for (long long b = 0; b < m->n_blocks; b++) hb[b] = h0 / double(1LL << m->level[b]);
-  return upload(h, &h->d_h, hb);
+  std::vector<int2> bxy((size_t)m->n_blocks);
+  for (long long b = 0; b < m->n_blocks; b++) {
+    hb[b] = h0 / double(1LL << m->level[b]);
+    bxy[b] = make_int2(m->bx[b], m->by[b]);
+  }
+  h->h_min = *std::min_element(hb.begin(), hb.end());
+  int rc = upload(h, &h->d_h, hb);
+  return rc ? rc : upload(h, &h->d_bxy, bxy);
 }
 
 static int set_mesh_body(cup2d_ps *h, const cup2d_mesh_desc *m) {
@@ -2091,6 +2100,176 @@ int cup2d_ps_kernel_time(cup2d_ps *h, int which, int reps, double *ms_per_launch
 
 int cup2d_ps_residual_device(cup2d_ps *h, const double *d_b, const double *d_x, double out[4]) {
   return guarded(h, [&] { return residual_body(h, d_b, d_x, out); });
+}
+
+// ---- flow operators of the projection step (flow.h) --------------------------
+
+static FlowMesh flow_mesh(const cup2d_ps *h) {
+  return FlowMesh{(int)h->op.nb, h->d_nbr, h->d_fid, h->d_h, h->d_bxy};
+}
+static int flow_grid(const cup2d_ps *h) {
+  return (int)std::min<long long>((h->op.nb + FWB - 1) / FWB, 8192);
+}
+
+static int advect_diffuse_body(cup2d_ps *h, double *d_u, double *d_v, double uinf_x, double uinf_y,
+                               double nu, double dt) {
+  if (!h || !d_u || !d_v) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  if (d_u == d_v) return fail(h, CUP2D_PS_EINVAL, "u and v must not alias");
+  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+  if (!(dt > 0) || !(nu >= 0)) return fail(h, CUP2D_PS_EINVAL, "dt must be positive and nu non-negative");
+  HIPCHK(h, hipSetDevice(h->device));
+  const FlowMesh M = flow_mesh(h);
+  // the solver's work vectors serve as the stage's work space: u1, v1 in
+  // p, nu; the right-hand sides in z1, z2
+  double *u1 = h->d_p, *v1 = h->d_nu, *tu = h->d_z1, *tv = h->d_z2;
+  const int Gf = flow_grid(h), Gc = cell_grid(h);
+  hipStream_t s = h->stream;
+  kf_advdiff_rhs<<<Gf, WG, 0, s>>>(M, d_u, d_v, uinf_x, uinf_y, nu, dt, tu, tv);
+  kf_stage<<<Gc, WG, 0, s>>>(M, d_u, d_v, tu, tv, 1, u1, v1);
+  kf_advdiff_rhs<<<Gf, WG, 0, s>>>(M, u1, v1, uinf_x, uinf_y, nu, dt, tu, tv);
+  kf_stage<<<Gc, WG, 0, s>>>(M, d_u, d_v, tu, tv, 0, d_u, d_v);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(s));
+  return CUP2D_PS_OK;
+}
+
+int cup2d_ps_advect_diffuse_device(cup2d_ps *h, double *d_u, double *d_v, double uinf_x,
+                                   double uinf_y, double nu, double dt) {
+  return guarded(h, [&] { return advect_diffuse_body(h, d_u, d_v, uinf_x, uinf_y, nu, dt); });
+}
+
+int cup2d_ps_penalize_device(cup2d_ps *h, const double *d_chi, double us, double vs, double lambda,
+                             double dt, double *d_u, double *d_v) {
+  return guarded(h, [&]() -> int {
+    if (!h || !d_chi || !d_u || !d_v) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+    if (d_u == d_v || d_chi == d_u || d_chi == d_v) return fail(h, CUP2D_PS_EINVAL, "chi, u and v must not alias");
+    if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+    if (!(dt > 0) || !(lambda >= 0)) return fail(h, CUP2D_PS_EINVAL, "dt must be positive and lambda non-negative");
+    HIPCHK(h, hipSetDevice(h->device));
+    kf_penalize<<<cell_grid(h), WG, 0, h->stream>>>((long)h->op.n, d_chi, us, vs, lambda, dt, d_u, d_v);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return CUP2D_PS_OK;
+  });
+}
+
+int cup2d_ps_chi_disk_device(cup2d_ps *h, double cx, double cy, double r, double *d_chi) {
+  return guarded(h, [&]() -> int {
+    if (!h || !d_chi) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+    if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+    if (!(r > 0)) return fail(h, CUP2D_PS_EINVAL, "radius must be positive");
+    HIPCHK(h, hipSetDevice(h->device));
+    kf_chi_disk<<<cell_grid(h), WG, 0, h->stream>>>(flow_mesh(h), cx, cy, r, d_chi);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return CUP2D_PS_OK;
+  });
+}
+
+int cup2d_ps_max_speed_device(cup2d_ps *h, const double *d_u, const double *d_v, double uinf_x,
+                              double uinf_y, double *umax) {
+  return guarded(h, [&]() -> int {
+    if (!h || !d_u || !d_v || !umax) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+    if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int G = std::min<int>(MAXG, cell_grid(h));
+    kf_max_speed<<<G, WG, 0, h->stream>>>((long)h->op.n, d_u, d_v, uinf_x, uinf_y, h->d_pfin);
+    HIPCHK(h, hipGetLastError());
+    double part[4 * MAXG];
+    HIPCHK(h, hipMemcpyAsync(part, h->d_pfin, 4 * (size_t)G * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double m = 0.0;
+    for (int i = 0; i < 4 * G; i++) m = std::max(m, part[i]);
+    *umax = m;
+    return CUP2D_PS_OK;
+  });
+}
+
+int cup2d_ps_block_score_device(cup2d_ps *h, const double *d_u, const double *d_v,
+                                const double *d_chi, double rtol, double *score) {
+  return guarded(h, [&]() -> int {
+    if (!h || !d_u || !d_v || !d_chi || !score) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+    if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "no mesh");
+    if (!(rtol > 0)) return fail(h, CUP2D_PS_EINVAL, "rtol must be positive");
+    HIPCHK(h, hipSetDevice(h->device));
+    double *d_sc = h->d_t;  // nb doubles of the solver's work vector (n = 64 nb)
+    kf_block_score<<<flow_grid(h), WG, 0, h->stream>>>(flow_mesh(h), d_u, d_v, d_chi, rtol, h->h_min, d_sc);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(score, d_sc, (size_t)h->op.nb * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return CUP2D_PS_OK;
+  });
+}
+
+static int transfer_body(cup2d_ps *h, const cup2d_mesh_desc *om, const cup2d_mesh_desc *nm,
+                         const int *src_kind, const long long *src, int n_fields,
+                         const double *const *d_old, double *const *d_new) {
+  if (!h || !om || !nm || !src_kind || !src || n_fields < 0 || (n_fields > 0 && (!d_old || !d_new)))
+    return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  const long long onb = om->n_blocks, nnb = nm->n_blocks;
+  if (onb <= 0 || nnb <= 0) return fail(h, CUP2D_PS_EINVAL, "empty mesh");
+  for (long long b = 0; b < nnb; b++) {  // the plan's sources are old blocks of the right kind
+    const int k = src_kind[b];
+    if (k < 0 || k > 2) return fail(h, CUP2D_PS_EINVAL, "src_kind out of range");
+    for (int q = 0; q < (k == 2 ? 4 : 1); q++)
+      if (src[4 * b + q] < 0 || src[4 * b + q] >= onb) return fail(h, CUP2D_PS_EINVAL, "src out of range");
+    bool ok = true;
+    if (k == 0) {
+      const long long a = src[4 * b];
+      ok = nm->level[b] == om->level[a] && nm->bx[b] == om->bx[a] && nm->by[b] == om->by[a];
+    } else if (k == 1) {
+      const long long a = src[4 * b];
+      const int d = nm->level[b] - om->level[a];
+      ok = d >= 1 && d <= 15 && (nm->bx[b] >> d) == om->bx[a] && (nm->by[b] >> d) == om->by[a];
+    } else {
+      for (int q = 0; q < 4; q++) {
+        const long long c = src[4 * b + q];
+        ok = ok && om->level[c] == nm->level[b] + 1 && om->bx[c] == 2 * nm->bx[b] + (q & 1) &&
+             om->by[c] == 2 * nm->by[b] + (q >> 1);
+      }
+    }
+    if (!ok) return fail(h, CUP2D_PS_EINVAL, "block " + std::to_string(b) + " does not match its source");
+  }
+  for (int f = 0; f < n_fields; f++) {
+    if (!d_old[f] || !d_new[f]) return fail(h, CUP2D_PS_EINVAL, "NULL field");
+    for (int g = 0; g < n_fields; g++)
+      if ((const double *)d_new[g] == d_old[f] || (g != f && d_new[g] == d_new[f]))
+        return fail(h, CUP2D_PS_EINVAL, "fields must not alias");
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  std::vector<int2> nij(nnb), oij(onb);
+  for (long long b = 0; b < nnb; b++) nij[b] = make_int2(nm->bx[b], nm->by[b]);
+  for (long long b = 0; b < onb; b++) oij[b] = make_int2(om->bx[b], om->by[b]);
+  void *buf[6] = {};
+  const size_t sz[6] = {nnb * sizeof(int), 4 * nnb * sizeof(long long), nnb * sizeof(int),
+                        nnb * sizeof(int2), onb * sizeof(int), onb * sizeof(int2)};
+  const void *host[6] = {src_kind, src, nm->level, nij.data(), om->level, oij.data()};
+  int rc = CUP2D_PS_OK;
+  for (int k = 0; k < 6 && !rc; k++) {
+    if (hipMalloc(&buf[k], sz[k]) != hipSuccess) rc = fail(h, CUP2D_PS_ENOMEM, "device allocation failed");
+    else if (hipMemcpyAsync(buf[k], host[k], sz[k], hipMemcpyHostToDevice, h->stream) != hipSuccess)
+      rc = fail(h, CUP2D_PS_EHIP, "upload failed");
+  }
+  if (!rc) {
+    const Transfer T{(int)nnb, (const int *)buf[0], (const long long *)buf[1], (const int *)buf[2],
+                     (const int2 *)buf[3], (const int *)buf[4], (const int2 *)buf[5]};
+    const int G = (int)std::min<long long>((nnb * 64 + WG - 1) / WG, 16384);
+    for (int f = 0; f < n_fields; f++) kf_transfer<<<G, WG, 0, h->stream>>>(T, d_old[f], d_new[f]);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = fail(h, CUP2D_PS_EHIP, "transfer kernel failed");
+  }
+  (void)hipStreamSynchronize(h->stream);
+  for (void *p : buf)
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int cup2d_ps_transfer_device(cup2d_ps *h, const cup2d_mesh_desc *old_mesh,
+                             const cup2d_mesh_desc *new_mesh, const int *src_kind, const long long *src,
+                             int n_fields, const double *const *d_old, double *const *d_new) {
+  return guarded(h, [&] { return transfer_body(h, old_mesh, new_mesh, src_kind, src, n_fields, d_old, d_new); });
 }
 
 long long cup2d_op_csr(const cup2d_mesh_desc *m, int mean_constraint, long long *row_ptr, int *col,

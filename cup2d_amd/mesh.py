@@ -33,6 +33,13 @@ def _load():
             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
         ]
+        lib.cup2d_mesh_adapt.restype = ctypes.c_longlong
+        lib.cup2d_mesh_adapt.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_longlong,
+        ]
         lib.cup2d_mesh_last_error.restype = ctypes.c_char_p
         _lib = lib
     return _lib
@@ -124,3 +131,30 @@ def from_blocks(bpdx, bpdy, levels, extent, blocks, periodic_x=False, periodic_y
     a = np.asarray(blocks, dtype=np.int32).reshape(-1, 3)
     return Mesh(bpdx, bpdy, levels, float(extent), a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy(),
                 bool(periodic_x), bool(periodic_y))
+
+
+def adapt(mesh: Mesh, tags):
+    """The mesh adapted to per-block tags (+1 refine, -1 compress, 0 leave;
+    cup2d_mesh_adapt in csrc/mesh.cpp).  Returns (new mesh, src_kind, src):
+    per new block, 0 = the old block src[b, 0] unchanged, 1 = refined from
+    the old ancestor src[b, 0], 2 = compressed from the four old children
+    src[b, 0..3] (quadrant (x & 1) + 2 (y & 1))."""
+    lib = _load()
+    t = np.ascontiguousarray(np.asarray(tags, dtype=np.int8))
+    if t.shape[0] != mesh.n_blocks:
+        raise ValueError("one tag per block")
+    lev, bx, by = (np.ascontiguousarray(a, dtype=np.int32) for a in (mesh.level, mesh.bx, mesh.by))
+    args = (mesh.bpdx, mesh.bpdy, mesh.levels, int(mesh.periodic_x), int(mesh.periodic_y), mesh.n_blocks,
+            lev.ctypes.data, bx.ctypes.data, by.ctypes.data, t.ctypes.data)
+    nb = lib.cup2d_mesh_adapt(*args, None, None, None, None, None, 0)
+    if nb < 0:
+        raise ValueError(lib.cup2d_mesh_last_error().decode())
+    nl, nx, ny = (np.empty(nb, np.int32) for _ in range(3))
+    kind = np.empty(nb, np.int32)
+    src = np.empty((nb, 4), np.int64)
+    nb2 = lib.cup2d_mesh_adapt(*args, nl.ctypes.data, nx.ctypes.data, ny.ctypes.data, kind.ctypes.data,
+                               src.ctypes.data, nb)
+    if nb2 != nb:
+        raise ValueError(lib.cup2d_mesh_last_error().decode())
+    m = Mesh(mesh.bpdx, mesh.bpdy, mesh.levels, mesh.extent, nl, nx, ny, mesh.periodic_x, mesh.periodic_y)
+    return m, kind, src

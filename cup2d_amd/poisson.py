@@ -224,6 +224,63 @@ class Solver:
                                                                     d_v.ptr))
         return d_u.numpy(), d_v.numpy()
 
+    # ---- the rest of the projection time step (device arrays) ----------------
+
+    def advect_diffuse_device(self, d_u, d_v, uinf, nu, dt):
+        """advDiff::operator() (advDiff.cpp:238-282) in place on device arrays."""
+        _check(self.h, _lib.load().cup2d_ps_advect_diffuse_device(self.h, d_u.ptr, d_v.ptr, float(uinf[0]),
+                                                                  float(uinf[1]), float(nu), float(dt)))
+
+    def penalize_device(self, d_chi, us, vs, lam, dt, d_u, d_v):
+        """PressureSingle::penalize (PressureSingle.cpp:268-322), one rigid obstacle."""
+        _check(self.h, _lib.load().cup2d_ps_penalize_device(self.h, d_chi.ptr, float(us), float(vs), float(lam),
+                                                            float(dt), d_u.ptr, d_v.ptr))
+
+    def chi_disk_device(self, cx, cy, r, d_chi):
+        """The disk's characteristic function (PutObjectsOnGrid.cpp:70-96)."""
+        _check(self.h, _lib.load().cup2d_ps_chi_disk_device(self.h, float(cx), float(cy), float(r), d_chi.ptr))
+
+    def max_speed_device(self, d_u, d_v, uinf):
+        """findMaxU (Helpers.cpp:155-207)."""
+        out = ctypes.c_double()
+        _check(self.h, _lib.load().cup2d_ps_max_speed_device(self.h, d_u.ptr, d_v.ptr, float(uinf[0]),
+                                                             float(uinf[1]), ctypes.byref(out)))
+        return out.value
+
+    def block_score_device(self, d_u, d_v, d_chi, rtol):
+        """Per-block refinement score (AdaptTheMesh.cpp:11-122): max |omega|,
+        or 2 rtol at the obstacle's interface; a host array."""
+        n_blocks = self.sizes()["n"] // 64
+        out = np.empty(n_blocks, np.float64)
+        _check(self.h, _lib.load().cup2d_ps_block_score_device(self.h, d_u.ptr, d_v.ptr, d_chi.ptr,
+                                                               float(rtol), out.ctypes.data))
+        return out
+
+    def transfer_device(self, old_mesh, new_mesh, kind, src, d_old, d_new):
+        """Fields d_old (on old_mesh) onto d_new (on new_mesh) by the plan of
+        mesh.adapt."""
+        (do, ko), (dn, kn) = _mesh_desc(old_mesh), _mesh_desc(new_mesh)
+        descs = (do, dn)
+        k = np.ascontiguousarray(kind, dtype=np.int32)
+        sr = np.ascontiguousarray(src, dtype=np.int64)
+        n = len(d_old)
+        if (len(d_new) != n or k.shape != (new_mesh.n_blocks,) or sr.shape != (new_mesh.n_blocks, 4)
+                or any(a.n != old_mesh.n for a in d_old) or any(a.n != new_mesh.n for a in d_new)):
+            raise ValueError("transfer: fields or plan do not match the meshes")
+        po = (ctypes.c_void_p * n)(*[a.ptr for a in d_old])
+        pn = (ctypes.c_void_p * n)(*[a.ptr for a in d_new])
+        _check(self.h, _lib.load().cup2d_ps_transfer_device(self.h, ctypes.byref(descs[0]),
+                                                            ctypes.byref(descs[1]), k.ctypes.data,
+                                                            sr.ctypes.data, n, po, pn))
+        del ko, kn
+
+    def advect_diffuse(self, u, v, uinf, nu, dt):
+        """Host arrays in, (u, v) after the two-stage update out."""
+        d_u = DeviceArray.from_host(self, np.ascontiguousarray(u, dtype=np.float64))
+        d_v = DeviceArray.from_host(self, np.ascontiguousarray(v, dtype=np.float64))
+        self.advect_diffuse_device(d_u, d_v, uinf, nu, dt)
+        return d_u.numpy(), d_v.numpy()
+
     def pressure_step(self, rhs, p_old, tol=0.0, rel_tol=1e-10, max_restarts=100):
         """Host arrays in, (p, stats) out."""
         d_rhs = DeviceArray.from_host(self, np.ascontiguousarray(rhs, dtype=np.float64))

@@ -141,3 +141,26 @@ def workload_rhs(m, seed, with_sources=False):
     if with_sources:
         src = disk_sources(seed, 0.5, 0.5, 0.1)
     return vortex_rhs(m, seed, sources=src)
+
+
+# Flow past a cylinder (the projection time-step workload): disk r = 0.05 at
+# (0.3, 0.5) in the [0, 2] x [0, 1] domain (16 x 8 level-0 blocks), U = 0.2,
+# Re = U 2r / nu = 1000, CFL 0.4.
+CYL_CENTER = (0.3, 0.5)
+CYL_RADIUS = 0.05
+CYL_U = 0.2
+CYL_RE = 1000.0
+CYL_NU = CYL_U * 2 * CYL_RADIUS / CYL_RE
+CYL_CFL = 0.4
+
+
+def cylinder_mesh(levels=5, target=8192, tol=0.02):
+    """A static mesh refined in bands around the cylinder (levels up to
+    `levels`), ~`target` blocks."""
+    cx, cy = CYL_CENTER
+
+    def make(s):
+        bands = [s * 0.6 * 0.5 ** lv for lv in range(levels - 1)]
+        return meshmod.graded(16, 8, levels, 2.0, [(cx, cy, CYL_RADIUS)], bands)
+
+    return _bisect_mesh(make, target, tol)

@@ -192,6 +192,64 @@ int cup2d_ps_pressure_rhs_device(cup2d_ps *h, const double *d_u, const double *d
 int cup2d_ps_pressure_correct_device(cup2d_ps *h, const double *d_p, double dt, double *d_u,
                                      double *d_v);
 
+/* ---- the rest of the projection time step on device arrays ---------------
+ * Block-major fields of the handle's mesh (n doubles each).  Ghost cells at
+ * coarse-fine faces: the coarse cell containing the ghost plus its gradient
+ * times the offset (across a coarser face), the average of the four fine
+ * cells under it (across a finer face); the reference takes them from
+ * Cubism's AMR lab (absent here: parity unpinned there). */
+
+/* Advection-diffusion, in place (advDiff::operator(), advDiff.cpp:238-282):
+ * u_old = u; u1 = u_old + 0.5 RHS(u) / h^2; u = u_old + RHS(u1) / h^2, RHS =
+ * KernelAdvectDiffuse (advDiff.cpp:140-236): -dt h ((u + uinf_x) du/dx + (v +
+ * uinf_y) du/dy) with the upwinded WENO5 differences of advDiff.cpp:18-82,
+ * plus nu dt times the 5-point Laplacian, flux-corrected at coarse-fine
+ * faces; the same for v.  Domain faces: the "freespace" velocity lab
+ * (Definitions.h:84-117).  Uses the handle's solver work vectors as
+ * scratch (not between the calls of a solve). */
+int cup2d_ps_advect_diffuse_device(cup2d_ps *h, double *d_u, double *d_v, double uinf_x,
+                                   double uinf_y, double nu, double dt);
+
+/* Brinkman penalization toward the rigid velocity (us, vs), in place
+ * (PressureSingle::penalize, PressureSingle.cpp:268-322, one obstacle, no
+ * deformation velocity): where chi > 0, alpha = 1 / (1 + lambda dt) if
+ * chi > 0.5 else 1, u = alpha u + (1 - alpha) us. */
+int cup2d_ps_penalize_device(cup2d_ps *h, const double *d_chi, double us, double vs, double lambda,
+                             double dt, double *d_u, double *d_v);
+
+/* chi of the disk of radius r centred at (cx, cy): the signed distance
+ * r - |x - c| (FillBlocks_Cylinder, ShapeLibrary.cpp:19-34) through Towers'
+ * mollified Heaviside within one cell of the circle (PutObjectsOnGrid.cpp:
+ * 70-96). */
+int cup2d_ps_chi_disk_device(cup2d_ps *h, double cx, double cy, double r, double *d_chi);
+
+/* *umax = max(|u + uinf_x|, |v + uinf_y|, |u|, |v|) over the mesh
+ * (findMaxU, Helpers.cpp:155-207), for the time-step size. */
+int cup2d_ps_max_speed_device(cup2d_ps *h, const double *d_u, const double *d_v, double uinf_x,
+                              double uinf_y, double *umax);
+
+/* Mesh adaptation (AdaptTheMesh.cpp:11-122).  Refinement score of every
+ * block, written to the host array score[n_blocks]: max |omega| over the
+ * block, omega = (0.5 / h) ((u(x, y-1) - u(x, y+1)) + (v(x+1, y) - v(x-1, y)))
+ * (KernelVorticity, Helpers.h:103-116), or 2 rtol when the obstacle's
+ * interface (0 < chi < 0.9, GradChiOnTmp, AdaptTheMesh.cpp:11-71) is in the
+ * block or within 2 cells (3 on the finest level) across its faces.  The
+ * caller tags (refine above Rtol, compress below Ctol) and calls
+ * cup2d_mesh_adapt (include/cup2d_mesh.h). */
+int cup2d_ps_block_score_device(cup2d_ps *h, const double *d_u, const double *d_v,
+                                const double *d_chi, double rtol, double *score);
+
+/* Fields of old_mesh onto new_mesh by the plan of cup2d_mesh_adapt: copies;
+ * refinement from an ancestor (the containing ancestor cell plus its
+ * gradient times the offset: exact for linear fields, keeps each parent
+ * cell's mean one level down); compression (the average of four fine cells).
+ * d_old[f] (64 old_mesh->n_blocks doubles) -> d_new[f] (64 new_mesh->n_blocks),
+ * f < n_fields.  Independent of the handle's current mesh.  The reference
+ * refines and compresses in Cubism (absent here): parity unpinned. */
+int cup2d_ps_transfer_device(cup2d_ps *h, const cup2d_mesh_desc *old_mesh,
+                             const cup2d_mesh_desc *new_mesh, const int *src_kind, const long long *src,
+                             int n_fields, const double *const *d_old, double *const *d_new);
+
 /* y = A x on the device (the operator of the last cup2d_ps_set_mesh, with the
  * mean-constraint row if enabled).  kind 0: stencil kernel used by the solver,
  * kind 1: plain CSR kernel.  Reference: BiCGSTABSolver::hd_cusparseSpMV

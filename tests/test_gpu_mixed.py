@@ -137,11 +137,14 @@ def test_four_disk_amr6_full_size_mixed_matches_fp64(oracle):
 @pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
 def test_four_disk_amr6_full_size_batched(oracle, precision):
     """The same mesh as a one-instance batch: the batched solver's device-side
-    residual replacement converges on it too (FP64 residual to 1e-10)."""
+    residual replacement converges on it too (FP64 residual to 1e-10).  BiCGSTAB
+    is erratic on this mesh (the FP64 solve restarts ~60 times): the mixed
+    batch takes 7,000-9,600 iterations depending on the rounding order of its
+    reductions, hence the cap of 12,000."""
     m = W.four_disk_amr6_mesh()
     A = oracle.assemble(m)
     b = W.workload_rhs(m, 3)
-    c = PS.default_config(precision=precision, norm=PS.NORM_L2, max_iter=8000)
+    c = PS.default_config(precision=precision, norm=PS.NORM_L2, max_iter=12000)
     bs = PS.BatchSolver(c).set_meshes([m])
     x, st = bs.solve(b, rel_tol=1e-10)
     assert st[0]["converged"], st[0]
