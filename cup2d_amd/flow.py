@@ -33,7 +33,10 @@ class CylinderFlow:
     """Flow past the disk of radius `radius` at `center`, free stream `uinf`,
     viscosity `nu`, penalization `lam` (the Python driver's default 1e6,
     cubismup2d/simulation.py:73), CFL number `cfl`; the pressure solved to
-    `rel_tol` relative residual by the handle's solver (FP64 or mixed)."""
+    `rel_tol` relative residual by the handle's solver (FP64 or mixed).
+    Adaptation every `adapt_every` steps (0: a static mesh) with the
+    refinement / compression tolerances `rtol`, `ctol` on the vorticity of
+    the disk launcher (launch/launchDiskAMR.sh:51-52)."""
 
     def __init__(self, mesh, radius=W.CYL_RADIUS, center=W.CYL_CENTER, uinf=(W.CYL_U, 0.0),
                  nu=W.CYL_NU, lam=1e6, cfl=W.CYL_CFL, precision=PS.MIXED, rel_tol=1e-10,

@@ -52,8 +52,12 @@ def test_transfer_matches_oracle():
         assert np.array_equal(dn.numpy(), FL.transfer(m, m2, kind, src, a))
     assert (kind == 1).any() and (kind == 2).any()
     # plans that do not match the meshes, aliased fields
-    with pytest.raises(_lib.SolverError):
+    with pytest.raises(ValueError):  # sizes checked by the Python layer
         s.transfer_device(m, m2, kind, src, d_old, d_old)
+    m3, k3, s3 = M.adapt(m, np.zeros(m.n_blocks, np.int8))  # the same mesh
+    assert m3.n == m.n and (k3 == 0).all()
+    with pytest.raises(_lib.SolverError):  # the library's alias check
+        s.transfer_device(m, m3, k3, s3, d_old, d_old)
     bad = src.copy()
     b2 = int(np.flatnonzero(kind == 2)[0])
     bad[b2, [0, 1]] = bad[b2, [1, 0]]
