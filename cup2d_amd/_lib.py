@@ -55,6 +55,7 @@ SIGNATURES = {
     "cup2d_ps_set_mesh_on_device": (_I, [_P, _P]),
     "cup2d_ps_solve": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_ps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
+    "cup2d_ps_solve_device_noalloc": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_ps_advect_diffuse_device": (_I, [_P, _P, _P, _D, _D, _D, _D]),
     "cup2d_ps_penalize_device": (_I, [_P, _P, _D, _D, _D, _D, _P, _P]),
     "cup2d_ps_chi_disk_device": (_I, [_P, _D, _D, _D, _P]),

@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -885,6 +886,7 @@ struct cup2d_ps {
   float *f_r = nullptr, *f_rhat = nullptr, *f_p = nullptr, *f_nu = nullptr, *f_z1 = nullptr,
         *f_z2 = nullptr, *f_t = nullptr;
   Scal *d_s = nullptr;                   // two state slots (iteration parity)
+  double *scratch = nullptr;             // this device's allocation-free work space (see below)
   int2 *d_bxy = nullptr;                 // [nb] block index (bx, by) at its level
   double h_min = 0.0;                    // smallest cell size of the mesh
   double *d_u = nullptr;                 // scalars of the unfused comparator (UScal + constants)
@@ -1525,6 +1527,132 @@ int solve_impl(cup2d_ps *h, const double *d_b, const double *d_x0, double *d_out
   });
 }
 
+// ----------------------------------------------------------------------------
+// Allocation-free solve (cup2d_ps_solve_device_noalloc).  Its work space --
+// the nine Krylov / solution vectors, the per-workgroup partial sums and the
+// two solver-state slots -- is this file-scope device array, not the
+// handle's buffers, so the call allocates nothing.  Each device holds its own
+// copy until the process ends; host code takes its address from
+// hipGetSymbolAddress (once per device, at handle creation) and never uses
+// the array's name.  Every call on a device shares the device's array:
+// calls that overlap on one device race.  The solve initializes every part
+// of it it reads (r, rhat, p, nu, x, x_opt by the first kernel, the state
+// slots from the host, the partials before each use).
+// ----------------------------------------------------------------------------
+constexpr long long NOALLOC_N = CUP2D_PS_NOALLOC_MAX_N;   // unknowns
+constexpr size_t NOALLOC_GT = MAXG + MAXGE;                 // partial slots per quantity
+constexpr size_t NOALLOC_SCAL = 2 * ((sizeof(Scal) + 255) / 256) * 32;  // doubles: 2 slots, 256-byte aligned
+constexpr size_t NOALLOC_DOUBLES = NOALLOC_SCAL + 9 * (size_t)NOALLOC_N + 17 * NOALLOC_GT + 8;
+static_assert(NOALLOC_N % 64 == 0, "the work vectors are whole blocks");
+
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64] = {};  // per device ordinal, set once
+
+// The device address of `scratch` on the current device (`dev`), looked up
+// once per device.
+int scratch_address(int dev, double *&out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= 64) return CUP2D_PS_EINVAL;
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return CUP2D_PS_EHIP;
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  out = g_scratch_ptr[dev];
+  return CUP2D_PS_OK;
+}
+
+// The handle's work-buffer pointers (what the allocation-free solve swaps).
+struct WorkBufs {
+  double *x, *xopt, *r, *rhat, *p, *nu, *z1, *z2, *t, *p2, *p4, *p5, *pm, *pfin;
+  Scal *s;
+};
+WorkBufs get_bufs(const cup2d_ps *h) {
+  return WorkBufs{h->d_x, h->d_xopt, h->d_r, h->d_rhat, h->d_p, h->d_nu, h->d_z1, h->d_z2, h->d_t,
+                  h->d_p2, h->d_p4, h->d_p5, h->d_pm, h->d_pfin, h->d_s};
+}
+void set_bufs(cup2d_ps *h, const WorkBufs &b) {
+  h->d_x = b.x; h->d_xopt = b.xopt; h->d_r = b.r; h->d_rhat = b.rhat; h->d_p = b.p;
+  h->d_nu = b.nu; h->d_z1 = b.z1; h->d_z2 = b.z2; h->d_t = b.t; h->d_p2 = b.p2; h->d_p4 = b.p4;
+  h->d_p5 = b.p5; h->d_pm = b.pm; h->d_pfin = b.pfin; h->d_s = b.s;
+}
+// The same buffers carved out of the scratch array for n unknowns.
+WorkBufs scratch_bufs(double *base, size_t n) {
+  WorkBufs b;
+  b.s = reinterpret_cast<Scal *>(base);
+  double *q = base + NOALLOC_SCAL;
+  double **vec[] = {&b.x, &b.xopt, &b.r, &b.rhat, &b.p, &b.nu, &b.z1, &b.z2, &b.t};
+  for (double **v : vec) {
+    *v = q;
+    q += n;  // n is a multiple of 64: every vector starts 512-byte aligned
+  }
+  b.p2 = q;
+  b.p4 = b.p2 + 2 * NOALLOC_GT;
+  b.p5 = b.p4 + 4 * NOALLOC_GT;
+  b.pm = b.p5 + NQ5 * NOALLOC_GT;
+  b.pfin = b.pm + NOALLOC_GT;
+  return b;
+}
+
+// The FP64 solve of cup2d_ps_solve_device with the scratch array as work
+// space: the same kernels in the same order (plain stream launches: a graph
+// would be instantiated, which allocates), then the true residual, reduced
+// on the device into the scratch and read back through the handle's pinned
+// state buffer.
+int solve_noalloc_body(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs, double tol_rel,
+                       int max_restarts, cup2d_ps_stats *st) {
+  if (!h || !d_b || !d_x) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
+  if (!(tol_abs >= 0) || !(tol_rel >= 0) || max_restarts < 0)
+    return fail(h, CUP2D_PS_EINVAL, "negative tolerance or restart count");
+  if (!h->have_mesh) return fail(h, CUP2D_PS_ENOMESH, "solve before cup2d_ps_set_mesh");
+  if (h->mixed || h->cfg.schedule != 0)
+    return fail(h, CUP2D_PS_EINVAL,
+                "the allocation-free solve takes an FP64 handle with the fused schedule");
+  if (h->op.n > NOALLOC_N)
+    return fail(h, CUP2D_PS_EINVAL,
+                "mesh has " + std::to_string(h->op.n) + " unknowns; the allocation-free solve takes at most " +
+                    std::to_string(NOALLOC_N) + " (its work space is a fixed device array)");
+  if (!h->scratch) return fail(h, CUP2D_PS_EHIP, "no work-space address for this device");
+  if (d_b == d_x) return fail(h, CUP2D_PS_EINVAL, "b and x must not alias");
+  HIPCHK(h, hipSetDevice(h->device));
+  const WorkBufs own = get_bufs(h);
+  const WorkBufs sb = scratch_bufs(h->scratch, (size_t)h->op.n);
+  const int graph = h->cfg.use_graph, prof = h->cfg.profile;
+  set_bufs(h, sb);
+  h->cfg.use_graph = 0;
+  h->cfg.profile = 0;
+  int rc = guarded(h, [&]() -> int {
+    int r = solve_fp64(h, d_b, d_x, d_x, tol_abs, tol_rel, max_restarts, st);
+    if (r) return r;
+    // true residual of the returned x: partials of |r|^2, |r|_inf, |b|^2,
+    // |b|_inf, each reduced by one workgroup into scratch
+    const int G = h->G;
+    k_spmv<double><<<G, WG, 0, h->stream>>>(view(h), h->Gb, d_x, h->d_t, h->d_pm);
+    if (h->op.mean_row >= 0)
+      k_meanrow_fix<double><<<1, WG, 0, h->stream>>>(h->d_pm, G, h->op.mean_row, h->d_t);
+    k_residual<<<G, WG, 0, h->stream>>>((long)h->op.n, d_b, h->d_t, h->op.mean_row, h->d_pfin);
+    double *tot = h->d_p2;  // free again: four totals
+    for (int q = 0; q < 4; q++)
+      ku_total<<<1, WG, 0, h->stream>>>(h->d_pfin + (size_t)q * G, G, (q & 1) ? 2 : 0, tot + q);
+    HIPCHK(h, hipGetLastError());
+    double *pin = reinterpret_cast<double *>(h->h_s);
+    HIPCHK(h, hipMemcpyAsync(pin, tot, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (st) {
+      st->true_rel2 = pin[2] > 0 ? std::sqrt(pin[0]) / std::sqrt(pin[2]) : 0.0;
+      st->true_relinf = pin[3] > 0 ? pin[1] / pin[3] : 0.0;
+    }
+    return CUP2D_PS_OK;
+  });
+  set_bufs(h, own);
+  h->cfg.use_graph = graph;
+  h->cfg.profile = prof;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1569,6 +1697,9 @@ static int create_body(const cup2d_ps_config *cfg, cup2d_ps **out) {
   h->device = c.device;
   h->mixed = c.precision == CUP2D_PS_MIXED;
   if (hipSetDevice(c.device) != hipSuccess) return fail(nullptr, CUP2D_PS_EHIP, "hipSetDevice failed");
+  // the allocation-free solve's work space on this device (looked up once)
+  if (scratch_address(c.device, h->scratch) != CUP2D_PS_OK)
+    return fail(nullptr, CUP2D_PS_EHIP, "hipGetSymbolAddress of the work space failed");
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipHostMalloc((void **)&h->h_s, 2 * sizeof(Scal), hipHostMallocDefault) != hipSuccess) {
@@ -1749,6 +1880,11 @@ int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double to
   if (!h || !d_b || !d_x) return fail(h, CUP2D_PS_EINVAL, "NULL argument");
   if (tol_abs < 0 || tol_rel < 0 || max_restarts < 0) return fail(h, CUP2D_PS_EINVAL, "negative tolerance");
   return solve_impl(h, d_b, d_x, d_x, tol_abs, tol_rel, max_restarts, st);
+}
+
+int cup2d_ps_solve_device_noalloc(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs,
+                                  double tol_rel, int max_restarts, cup2d_ps_stats *st) {
+  return guarded(h, [&] { return solve_noalloc_body(h, d_b, d_x, tol_abs, tol_rel, max_restarts, st); });
 }
 
 int cup2d_ps_solve(cup2d_ps *h, const double *b, double *x, double tol_abs, double tol_rel,

@@ -147,6 +147,23 @@ int cup2d_ps_solve(cup2d_ps *h, const double *b, double *x, double tol_abs, doub
 int cup2d_ps_solve_device(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs,
                           double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
+/* Largest mesh (unknowns) of cup2d_ps_solve_device_noalloc: 2048 blocks. */
+#define CUP2D_PS_NOALLOC_MAX_N 131072
+
+/* cup2d_ps_solve_device without any allocation: no hipMalloc / hipFree /
+ * graph instantiation in the call.  No counterpart in the reference; it is
+ * for callers that cannot allocate (serving loops, harnesses that forbid
+ * allocation).  The work vectors, partial sums and solver state live in a
+ * fixed per-device array of the library (about 9.5 MB, held until the
+ * process ends) instead of the handle's buffers, so: FP64 handles with the
+ * fused schedule only, meshes of at most CUP2D_PS_NOALLOC_MAX_N unknowns
+ * (CUP2D_PS_EINVAL beyond), d_b must not alias d_x, and calls that overlap
+ * on one device race (every call on a device shares its array).  Same
+ * results as cup2d_ps_solve_device (the same kernels in the same order);
+ * stats as there, without per-phase times. */
+int cup2d_ps_solve_device_noalloc(cup2d_ps *h, const double *d_b, double *d_x, double tol_abs,
+                                  double tol_rel, int max_restarts, cup2d_ps_stats *stats);
+
 /* The pressure step around the solve, device-resident (reference
  * PressureSingle::operator(), PressureSingle.cpp:842-879):
  *   b  = rhs - L p_old   (updatePressureRHS1, :401-456, applied at :848-849;

@@ -14,7 +14,8 @@ static const fn_t entry_points[] = {
     (fn_t)&cup2d_ps_config_default, (fn_t)&cup2d_ps_create,
     (fn_t)&cup2d_ps_destroy, (fn_t)&cup2d_ps_set_mesh,
     (fn_t)&cup2d_ps_set_mesh_on_device, (fn_t)&cup2d_ps_solve,
-    (fn_t)&cup2d_ps_solve_device, (fn_t)&cup2d_ps_pressure_step_device, (fn_t)&cup2d_ps_advect_diffuse_device,
+    (fn_t)&cup2d_ps_solve_device, (fn_t)&cup2d_ps_solve_device_noalloc,
+    (fn_t)&cup2d_ps_pressure_step_device, (fn_t)&cup2d_ps_advect_diffuse_device,
     (fn_t)&cup2d_ps_penalize_device, (fn_t)&cup2d_ps_chi_disk_device,
     (fn_t)&cup2d_ps_max_speed_device, (fn_t)&cup2d_ps_block_score_device,
     (fn_t)&cup2d_ps_transfer_device,
