@@ -377,10 +377,12 @@ def batch_kernels(PS, bs, n, n_blocks, fp64, reps=20):
     return out
 
 
-def cylinder_flow(PS, W, steps, precision, cpu_steps=0):
+def cylinder_flow(PS, W, steps, precision, cpu_steps=0, adapt_every=0):
     """The projection time step past the cylinder (r = 0.05 at (0.3, 0.5),
-    U = 0.2, Re = 1000, CFL 0.4) on the static 5-level mesh (~8.3k blocks),
-    every operator on the GPU: steps/s over `steps` steps after one untimed
+    U = 0.2, Re = 1000, CFL 0.4) from the 5-level mesh (~8.3k blocks),
+    static or, with adapt_every > 0, adapted as AdaptTheMesh schedules it
+    (every step of the first ten, then every adapt_every steps), every
+    operator on the GPU: steps/s over `steps` steps after one untimed
     step from rest, and, with cpu_steps > 0, the relative
     L2 difference of velocity and pressure from the same first steps run on
     the CPU (oracle operators, FP64 BiCGSTAB to 1e-12)."""
@@ -414,16 +416,21 @@ def cylinder_flow(PS, W, steps, precision, cpu_steps=0):
             "rel_l2_u": float(np.linalg.norm(gu - u) / np.linalg.norm(u)),
             "rel_l2_v": float(np.linalg.norm(gv - v) / np.linalg.norm(v)),
             "rel_l2_p": float(np.linalg.norm(gp - p) / np.linalg.norm(p))}
-    flow = FLOW.CylinderFlow(m, precision=precision, rel_tol=1e-10)
+    flow = FLOW.CylinderFlow(m, precision=precision, rel_tol=1e-10, adapt_every=adapt_every)
     flow.step()  # untimed first step (graph capture of the solver's iteration chunks)
-    its = []
+    its, nbs = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         st = flow.step()
         its.append(st["iterations"])
+        nbs.append(flow.mesh.n_blocks)
         if not st["converged"]:
             out["unconverged_step"] = flow.steps
     sec = time.perf_counter() - t0
+    if adapt_every:
+        out["adaptation"] = {"every": adapt_every, "adaptations": flow.adaptations,
+                             "n_blocks_min": int(min(nbs)), "n_blocks_max": int(max(nbs)),
+                             "n_blocks_final": int(nbs[-1]), "levels_final": int(flow.mesh.level.max()) + 1}
     gu, gv, _ = flow.fields()
     out.update({"steps": steps, "steps_per_s": steps / sec, "ms_per_step": sec * 1e3 / steps,
                 "simulated_time": flow.time, "pressure_iterations_mean": float(np.mean(its)),
@@ -616,6 +623,8 @@ def main():
         progress("cylinder flow (mixed) done")
         cfg["cylinder_flow_fp64"] = cylinder_flow(PS, W, 100, PS.FP64)
         progress("cylinder flow (FP64) done")
+        cfg["cylinder_flow_adaptive_mixed"] = cylinder_flow(PS, W, 100, PS.MIXED, adapt_every=20)
+        progress("cylinder flow (adaptive, mixed) done")
     if not args.no_cpu:
         cfg["error_vs_oracle"] = oracle_errors(meshes, rhs, {"fp64": x64, "mixed": x32}, offsets)
         progress("errors against the oracle done")

@@ -48,9 +48,6 @@ using namespace cup2d;
 namespace {
 
 constexpr int NQB = 7;        // update totals (NQ5) + sum h^2 x (initial mean row)
-#ifndef CUP2D_XCD_ITEMS
-#define CUP2D_XCD_ITEMS 0
-#endif
 constexpr int BGRID = 2048;   // workgroups of the item / tile kernels (grid-stride beyond)
 constexpr int MAX_INST = 4096;
 constexpr int MAX_ITEM_BLOCKS = 256;  // item_blocks keeps one block header per lane: 64 * WAVES
@@ -175,25 +172,30 @@ __device__ __forceinline__ int4 readlane4(int4 v, int l) {
 // with UB > 0 the next item's block headers (load_heads<UB>) are loaded
 // while the current item is processed, so an item's data loads wait on no
 // other load.  body may synchronize the workgroup (the skip is uniform).
-template <int UB = 0, class F>
+template <int UB = 0, bool XCD = false, class F>
 __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body,
                                               int filter = F_RUNNING) {
-  // With a grid of a multiple of 8 workgroups, workgroup g runs on XCD
-  // g % 8: each XCD takes one contiguous eighth of the items, so that items
-  // next to each other (neighbour blocks) share that XCD's L2.
+  // XCD: with a grid of a multiple of 8 workgroups (workgroup g runs on XCD
+  // g % 8), the items go to the XCDs in runs of XG consecutive items, run r
+  // to XCD r % 8, so that neighbouring items (whose blocks read each other's
+  // faces) mostly share one XCD's L2 while every instance -- the last ones
+  // running included -- still spreads over all eight XCDs.  Item t of XCD x
+  // is (t / XG) 8 XG + x XG + t % XG.
+  constexpr int XG = 8;
   const int NG = (int)gridDim.x, c = lane_id();
-  const bool xm = CUP2D_XCD_ITEMS && (NG & 7) == 0;
-  const int G = xm ? NG >> 3 : NG;
-  int first = blockIdx.x, hi = V.n_items;
+  const bool xm = XCD && (NG & 7) == 0;
+  const int G = xm ? NG >> 3 : NG, x = (int)blockIdx.x & 7;
+  int n_mine = V.n_items, first = blockIdx.x;
   if (xm) {
-    const int C = (V.n_items + 7) >> 3, lo = min(V.n_items, ((int)blockIdx.x & 7) * C);
-    hi = min(V.n_items, lo + C);
-    first = lo + ((int)blockIdx.x >> 3);
+    const int rem = V.n_items % (8 * XG);
+    n_mine = V.n_items / (8 * XG) * XG + min(XG, max(0, rem - x * XG));
+    first = (int)blockIdx.x >> 3;
   }
-  for (int base = first; base < hi; base += 64 * G) {
-    const int cnt = min(64, (hi - base + G - 1) / G);
-    const int mine = base + c * G;
-    const int4 ml = V.item[c < cnt ? mine : base];
+  auto item_of = [&](int t) { return xm ? t / XG * (8 * XG) + x * XG + t % XG : t; };
+  for (int base = first; base < n_mine; base += 64 * G) {
+    const int cnt = min(64, (n_mine - base + G - 1) / G);
+    const int mine = item_of(base + c * G);
+    const int4 ml = V.item[c < cnt ? mine : item_of(base)];
     auto want = [&](int4 mt) { return !st || wanted(st[mt.x], filter); };
     int4 mt = readlane4(ml, 0);
     Heads hd{make_int4(-1, -1, -1, -1), 0ull};
@@ -204,7 +206,7 @@ __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F 
       Heads nhd{make_int4(-1, -1, -1, -1), 0ull};
       if constexpr (UB > 0)
         if (k + 1 < cnt && want(nmt)) nhd = load_heads<UB>(V, nmt);
-      if (want(mt)) body(base + k * G, mt, hd);
+      if (want(mt)) body(item_of(base + k * G), mt, hd);
       mt = nmt;
       hd = nhd;
     }
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_nu(BView V, const T *__restrict__ 
                                                  T *__restrict__ nu, const T *__restrict__ rhat,
                                                  const Scal *st, double *p2) {
   __shared__ __align__(16) unsigned char tile[row_tile_bytes<T>()];
-  running_items(V, st, [&](int item, int4 mt, const Heads &) {
+  running_items<0, true>(V, st, [&](int item, int4 mt, const Heads &) {
     const long long mr = V.mean_row[mt.x];
     double v[2] = {0.0, 0.0};
     if (mt.z > 0) {
@@ -488,7 +490,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
                                                 T *__restrict__ t, const T *__restrict__ sv,
                                                 const Scal *st, double *p4) {
   __shared__ __align__(16) unsigned char tile[row_tile_bytes<T>()];
-  running_items(V, st, [&](int item, int4 mt, const Heads &) {
+  running_items<0, true>(V, st, [&](int item, int4 mt, const Heads &) {
     const long long mr = V.mean_row[mt.x];
     double v[3] = {0.0, 0.0, 0.0};
     if (mt.z > 0) {
@@ -531,7 +533,7 @@ __global__ __launch_bounds__(WG) void kb_spmv_t(BView V, const T *__restrict__ z
 __global__ __launch_bounds__(WG) void kb_spmv(BView V, const double *__restrict__ x,
                                               double *__restrict__ y, double *pm) {
   __shared__ __align__(16) unsigned char tile[row_tile_bytes<double>()];
-  running_items(V, nullptr, [&](int item, int4 mt, const Heads &) {
+  running_items<0, true>(V, nullptr, [&](int item, int4 mt, const Heads &) {
     double v[1] = {0.0};
     if (mt.z > 0) {
       row_blocks<double, false>(V, x, x, mt.y, mt.z, tile,
