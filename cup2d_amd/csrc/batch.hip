@@ -398,14 +398,15 @@ __device__ __forceinline__ void st_plain(T *__restrict__ y, long base, const T (
   }
 }
 
-// r = b - A x (FP64), stored as T to r and rhat; p = nu = 0; x_opt = x;
+// r = b - A x (FP64), stored as T to r and rhat; p = nu = 0; x_opt = x
+// (FP64) or the correction and its best = 0 (mixed: xcor, xopt FP32);
 // partials of the update totals and of sum h^2 x, for the instances that
 // pass `filter` (the start of a solve, or a residual replacement).  The
 // mean row's residual is finished by kb_eval.
 template <class T>
 __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict__ b,
                                               const double *__restrict__ x,
-                                              double *__restrict__ xopt, T *__restrict__ r,
+                                              T *__restrict__ xopt, T *__restrict__ xcor, T *__restrict__ r,
                                               T *__restrict__ rhat, T *__restrict__ p,
                                               T *__restrict__ nu, const Scal *st, double *p5,
                                               int filter) {
@@ -430,7 +431,12 @@ __global__ __launch_bounds__(WG) void kb_init(BView V, const double *__restrict_
         const long i = R.i(u, c);
         p[i] = T(0);
         nu[i] = T(0);
-        xopt[i] = R.xc(u);
+        if constexpr (sizeof(T) == 8) {
+          xopt[i] = R.xc(u);
+        } else {  // the pass's correction and its best start at 0 (x itself)
+          xopt[i] = T(0);
+          xcor[i] = T(0);
+        }
         v[6] += R.hh[u] * R.xc(u);
         if (plain) store_r(i, y, R.a(u));
       });
@@ -581,13 +587,20 @@ __device__ __forceinline__ void st2(double *__restrict__ a, long i, const double
 __device__ __forceinline__ void st2(float *__restrict__ a, long i, const float (&v)[2]) {
   *reinterpret_cast<float2 *>(a + i) = make_float2(v[0], v[1]);
 }
+__device__ __forceinline__ void st2(float *__restrict__ a, long i, const double (&v)[2]) {
+  *reinterpret_cast<float2 *>(a + i) = make_float2((float)v[0], (float)v[1]);
+}
 
 // x_opt = x when the previous iterate was the best so far; x += alpha z1 +
-// omega z2 (FP64 x); r = s - omega t; update partials.  Two pairs of
-// consecutive cells per thread per pass (vector loads), all loads first.
+// omega z2; r = s - omega t; update partials.  FP64: x is the iterate.
+// Mixed: x is the pass's FP32 correction e (the iterate is the FP64 x of the
+// last residual replacement plus e; kb_fold adds it in), x_opt the best
+// correction -- 12 B per cell of the 24 an FP64 x and x_opt would move.  Two
+// pairs of consecutive cells per thread per pass (vector loads), all loads
+// first.
 template <class T>
-__global__ __launch_bounds__(WG) void kb_xrupdate(BView V, double *__restrict__ x,
-                                                  double *__restrict__ xopt,
+__global__ __launch_bounds__(WG) void kb_xrupdate(BView V, T *__restrict__ x,
+                                                  T *__restrict__ xopt,
                                                   const T *__restrict__ z1,
                                                   const T *__restrict__ z2, T *__restrict__ r,
                                                   const T *__restrict__ t,
@@ -657,6 +670,22 @@ __global__ __launch_bounds__(WG) void kb_select(BView V, double *__restrict__ x,
     long i0, i1;
     item_cells(mt, i0, i1);
     for (long i = i0 + threadIdx.x; i < i1; i += WG) x[i] = xopt[i];
+  }
+}
+
+// Mixed: x += the pass's FP32 correction on the instances (passing
+// `filter`): the last one when the last iterate is the best, else the best.
+__global__ __launch_bounds__(WG) void kb_fold(BView V, double *__restrict__ x,
+                                              const float *__restrict__ e,
+                                              const float *__restrict__ eopt, const Scal *st,
+                                              int filter) {
+  for (int item = blockIdx.x; item < V.n_items; item += gridDim.x) {
+    const int4 mt = V.item[item];
+    if (!wanted(st[mt.x], filter)) continue;
+    const float *__restrict__ c = st[mt.x].improved ? e : eopt;
+    long i0, i1;
+    item_cells(mt, i0, i1);
+    for (long i = i0 + threadIdx.x; i < i1; i += WG) x[i] += (double)c[i];
   }
 }
 
@@ -1307,6 +1336,21 @@ int grid_tiles(const cup2d_psb *h) {
 // Item and tile lists for `ib` blocks per item (tiles are TILE blocks; an
 // instance's last item and tile may be shorter); each instance's explicit
 // rows follow its block items as items of up to WG rows.
+// The iterate's buffers of the iteration kernels: FP64 x and x_opt, or, in
+// mixed precision, the pass's FP32 correction and its best, carved out of
+// the FP64 x_opt buffer (2n floats; x_opt is not needed there: the best
+// iterate is the last replacement's x plus the best correction).
+template <class T>
+T *iter_x(const cup2d_psb *h) {
+  if constexpr (sizeof(T) == 8) return h->d_x;
+  else return reinterpret_cast<float *>(h->d_xopt);
+}
+template <class T>
+T *iter_xopt(const cup2d_psb *h) {
+  if constexpr (sizeof(T) == 8) return h->d_xopt;
+  else return reinterpret_cast<float *>(h->d_xopt) + h->n;
+}
+
 int build_units(cup2d_psb *h, int ib) {
   drop_graph(h);
   (void)hipStreamSynchronize(h->stream);
@@ -1389,7 +1433,7 @@ void launch_iteration_b(cup2d_psb *h) {
   bptick(h, CUP2D_PS_PHASE_SPMV);
   kb_omega<T><<<K, WG, 0, s>>>(V, st, h->d_p4, t, r);
   bptick(h, CUP2D_PS_PHASE_SCALAR);
-  kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, z1, z2, r, t, rhat, st, h->d_p5);
+  kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, iter_x<T>(h), iter_xopt<T>(h), z1, z2, r, t, rhat, st, h->d_p5);
   bptick(h, CUP2D_PS_PHASE_UPDATE);
   kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0, 0);
   bptick(h, CUP2D_PS_PHASE_SCALAR);
@@ -1398,16 +1442,17 @@ void launch_iteration_b(cup2d_psb *h) {
 int chunk_b(const cup2d_psb *h) { return h->cfg.chunk > 0 ? h->cfg.chunk : 8; }
 
 // Residual replacement step (mixed precision): the instances with one
-// pending take their best iterate, recompute r = b - A x in FP64 from it and
-// restart.
+// pending add their best correction to x, recompute r = b - A x in FP64 from
+// it and restart with a zero correction.
 void launch_replacement_b(cup2d_psb *h) {
   const BView V = bview(h);
   const int Gi = grid_items(h);
   hipStream_t s = h->stream;
   float *r = (float *)h->v_r, *rhat = (float *)h->v_rhat, *p = (float *)h->v_p,
         *nu = (float *)h->v_nu;
-  kb_select<<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, h->d_st, F_REPLACE);
-  kb_init<float><<<Gi, WG, 0, s>>>(V, h->d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5,
+  float *e = iter_x<float>(h), *eopt = iter_xopt<float>(h);
+  kb_fold<<<Gi, WG, 0, s>>>(V, h->d_x, e, eopt, h->d_st, F_REPLACE);
+  kb_init<float><<<Gi, WG, 0, s>>>(V, h->d_b, h->d_x, eopt, e, r, rhat, p, nu, h->d_st, h->d_p5,
                                    F_REPLACE);
   kb_eval<float><<<h->K, WG, 0, s>>>(V, h->d_st, h->d_p5, r, rhat, 2, 0);
   bptick(h, CUP2D_PS_PHASE_SETUP);
@@ -1511,8 +1556,9 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
   const int Gi = grid_items(h), K = h->K;
   T *r = (T *)h->v_r, *rhat = (T *)h->v_rhat, *p = (T *)h->v_p, *nu = (T *)h->v_nu;
   BCHK(h, hipMemcpyAsync(h->d_st, h->h_st.data(), K * sizeof(Scal), hipMemcpyHostToDevice, h->stream));
-  kb_init<T><<<Gi, WG, 0, h->stream>>>(V, d_b, h->d_x, h->d_xopt, r, rhat, p, nu, h->d_st, h->d_p5,
-                                       F_LIVE);
+  kb_init<T><<<Gi, WG, 0, h->stream>>>(V, d_b, h->d_x, iter_xopt<T>(h),
+                                       sizeof(T) == 8 ? nullptr : iter_x<T>(h), r, rhat, p, nu,
+                                       h->d_st, h->d_p5, F_LIVE);
   kb_eval<T><<<K, WG, 0, h->stream>>>(V, h->d_st, h->d_p5, r, rhat, 1, sizeof(T) == 4 ? 1 : 0);
   kb_alldone<<<1, WG, 0, h->stream>>>(K, h->d_st, h->d_all_done);
   bptick(h, CUP2D_PS_PHASE_SETUP);
@@ -1523,7 +1569,10 @@ int run_pass(cup2d_psb *h, const double *d_b, int max_iter, int max_chunks) {
   if (!done && max_iter > 0) {
     if ((rc = run_chunks_b(h, max_iter, max_chunks))) return rc;
   }
-  kb_select<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_xopt, h->d_st, F_ALL);
+  if constexpr (sizeof(T) == 8)
+    kb_select<<<Gi, WG, 0, h->stream>>>(V, h->d_x, h->d_xopt, h->d_st, F_ALL);
+  else
+    kb_fold<<<Gi, WG, 0, h->stream>>>(V, h->d_x, iter_x<float>(h), iter_xopt<float>(h), h->d_st, F_ALL);
   bptick(h, CUP2D_PS_PHASE_SETUP);
   BCHK(h, hipGetLastError());
   BCHK(h, hipMemcpyAsync(h->h_st.data(), h->d_st, K * sizeof(Scal), hipMemcpyDeviceToHost, h->stream));
@@ -2011,7 +2060,7 @@ static int kernel_time_b(cup2d_psb *h, int which, int reps, double *ms_per_launc
         else kb_supdate32<<<Gt, WG, 0, s>>>(V, h->d_pfrag, r, nu, z2, st);
         break;
       case 3: kb_spmv_t<T><<<Gi, WG, 0, s>>>(V, z2, t, r, st, h->d_p4); break;
-      case 4: kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, h->d_x, h->d_xopt, z1, z2, r, t, rhat, st, h->d_p5); break;
+      case 4: kb_xrupdate<T><<<Gi, WG, 0, s>>>(V, iter_x<T>(h), iter_xopt<T>(h), z1, z2, r, t, rhat, st, h->d_p5); break;
       case 5: kb_alpha<T><<<K, WG, 0, s>>>(V, st, h->d_p2, nu, rhat); break;
       case 6: kb_omega<T><<<K, WG, 0, s>>>(V, st, h->d_p4, t, r); break;
       default: kb_eval<T><<<K, WG, 0, s>>>(V, st, h->d_p5, r, rhat, 0, 0);

@@ -198,30 +198,48 @@ __device__ __forceinline__ double stencil_eval(int4 nb, int c, TV tc, TV th, TV 
 // `width` entries (wave-uniform, at least the row's length; entries past the
 // row's length are (own row, 0.0)), in ascending column order; loads issued
 // eight entries at a time.
+// ELL entries w0 .. w0 + 7 of explicit row e (entries past `width` repeat
+// entry w0: the loads stay unconditional)
+struct Ent8 {
+  int col[8];
+  unsigned code[8];
+};
+__device__ __forceinline__ void load_ent8(const OpView &A, int e, int w0, int width, Ent8 &E) {
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    const long k = (long)(w0 + w < width ? w0 + w : w0) * A.n_exc + e;
+    E.col[w] = A.ecol[k];
+    E.code[w] = A.ecode[k];
+  }
+}
+// acc + the entries' products, in entry order
+template <class TV>
+__device__ __forceinline__ double sum_ent8(const OpView &A, const TV *__restrict__ x, const Ent8 &E,
+                                           int w0, int width, double acc) {
+  double xv[8], val[8];
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    xv[w] = (double)x[E.col[w]];
+    val[w] = A.etab[E.code[w]];
+  }
+#pragma unroll
+  for (int w = 0; w < 8; w++)
+    if (w0 + w < width) acc += val[w] * xv[w];
+  return acc;
+}
+
+// Explicit row e (width <= ELL_MAX = 16 entries): the index loads of both
+// halves of a wide row are issued before the first half's gathers, so a row
+// costs two dependent round trips whatever its width.
 template <class TV>
 __device__ __forceinline__ double explicit_row(const OpView &A, const TV *__restrict__ x, int e,
                                                int width) {
-  double acc = 0.0;
-  for (int w0 = 0; w0 < width; w0 += 8) {
-    int col[8];
-    unsigned code[8];
-#pragma unroll
-    for (int w = 0; w < 8; w++) {
-      const long k = (long)(w0 + w < width ? w0 + w : w0) * A.n_exc + e;  // loads stay unconditional
-      col[w] = A.ecol[k];
-      code[w] = A.ecode[k];
-    }
-    double xv[8], val[8];
-#pragma unroll
-    for (int w = 0; w < 8; w++) {
-      xv[w] = (double)x[col[w]];
-      val[w] = A.etab[code[w]];
-    }
-#pragma unroll
-    for (int w = 0; w < 8; w++)
-      if (w0 + w < width) acc += val[w] * xv[w];
-  }
-  return acc;
+  Ent8 lo;
+  load_ent8(A, e, 0, width, lo);
+  if (width <= 8) return sum_ent8(A, x, lo, 0, width, 0.0);
+  Ent8 hi;
+  load_ent8(A, e, 8, width, hi);
+  return sum_ent8(A, x, hi, 8, width, sum_ent8(A, x, lo, 0, width, 0.0));
 }
 
 // Width of explicit row e's run of 64 rows (e..e+63 aligned at 64 are one
