@@ -53,15 +53,19 @@ KERNEL_NAMES64 = ["kb_pupdate64", "kb_spmv_nu<double>", "kb_supdate64", "kb_spmv
                   "kb_xrupdate<double>"]
 
 
-def kernel_bytes(which, n, n_blocks, fp64=True):
+def kernel_bytes(which, n, n_blocks, fp64=True, batch=True):
     """Algorithmic HBM bytes of one launch of iteration kernel `which`: the
     vectors it must read and write once (s = 8 B FP64 / 4 B FP32 Krylov
-    vectors, x and x_opt always 8 B), plus 32 B of operator metadata per
-    block for the operator kernels (explicit rows add < 1 %)."""
+    vectors; the x update moves the FP64 x and x_opt, 8 B each, except in
+    the batched mixed solve, which updates the pass's FP32 correction and its
+    best, 4 B each), plus 32 B of operator metadata per block for the
+    operator kernels (explicit rows add < 1 %)."""
     s = 8 if fp64 else 4
+    xs = 4 if (batch and not fp64) else 8
     per_cell = {0: 5 * s + (0 if fp64 else 4),   # r, p, nu in; p, z1 out (+ projected r)
                 1: 3 * s, 2: 4 * s, 3: 3 * s,    # z, aux in; y out / r, nu in; r, z2 out
-                4: 24 + 6 * s}[which]            # x in/out, x_opt; z1, z2, r, t, rhat in; r out
+                4: 3 * xs + 6 * s}[which]        # x in/out, x_opt; z1, z2, r, t, rhat in;
+                                                 # r out
     meta = 32 * n_blocks if which in (1, 3) else 0
     return per_cell * n + meta
 
@@ -251,7 +255,7 @@ def timed_batch(PS, meshes, b, precision, steps, warmup, use_graph, profile=0):
     w_its = float(np.dot(its, nbk) / nbk.sum())
     res = {"solves_per_s": len(meshes) * steps / tot, "ms_per_batch": tot * 1e3 / steps,
            "ms_per_batch_median": float(np.median(t)) * 1e3,
-           "set_meshes_ms_device_assembly": setup_ms, "item_blocks": ib,
+           "set_meshes_ms_device_assembly": setup_ms, "item_blocks": ib, "xcd_run": bs.xcd_run(),
            "ms_per_iteration_tuned": ms_it,
            "iterations_max": int(its.max()), "iterations_mean": float(its.mean()),
            "iterations_mean_weighted_by_blocks": w_its,
@@ -511,14 +515,17 @@ def kernel_only(PS, W, which, reps):
         print(json.dumps({"kernel": KERNELS[k], "ms_per_launch": ms, "launches": reps + 1,
                           "algorithmic_bytes": byt, "GB_per_s": byt / (ms * 1e-3) / 1e9}))
         return
-    if which.startswith("iter"):
+    if which.startswith("iter") or which.startswith("xiter"):
+        # iteration kernel K of the 3-level mesh's single solve (iterK FP64,
+        # xiterK mixed)
         mesh = W.graded_disk_mesh()
-        s = PS.Solver(norm=PS.NORM_L2)
+        mixed = which[0] == "x"
+        s = PS.Solver(norm=PS.NORM_L2, precision=PS.MIXED if mixed else PS.FP64)
         s.set_mesh(mesh)
-        k = int(which[4:])
+        k = int(which[-1])
         ms = s.kernel_time(k, reps=reps)
         print(json.dumps({"kernel": KERNELS[k], "ms_per_launch": ms, "launches": reps + 2,
-                          "algorithmic_bytes": kernel_bytes(k, mesh.n, mesh.n_blocks)}))
+                          "algorithmic_bytes": kernel_bytes(k, mesh.n, mesh.n_blocks, not mixed, False)}))
         return
     if which == "spmv":
         mesh = W.graded_disk_mesh()
@@ -553,13 +560,13 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the sequential solves and the single-mesh workloads")
     modes = ["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection"]
-    modes += [f"iter{k}" for k in range(5)] + [f"biter{k}" for k in range(5)] + [f"miter{k}" for k in range(5)]
+    modes += [f"{p}iter{k}" for p in ("", "x", "b", "m") for k in range(5)]
     ap.add_argument("--kernel-only", choices=modes, default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
                          "(spmv: operator on the 3-level mesh; batch_spmv: batched operator on the "
                          "64-instance batch; batch_fp64 / batch_mixed: 64 iterations of the "
                          "batched solve, every kernel launched from the stream; iterK: iteration "
-                         "kernel K of the 3-level mesh's solve; biterK / miterK: batched FP64 / "
+                         "kernel K of the 3-level mesh's solve (xiterK: mixed); biterK / miterK: batched FP64 / "
                          "mixed iteration kernel K with every instance iterating; copy: the "
                          "library's streaming copy over 13,881,344 doubles, the bandwidth ceiling "
                          "and the traffic counters' calibration; assembly: host against device-side "

@@ -26,7 +26,7 @@ class MeshDesc(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [("device", _I), ("precision", _I), ("mean_constraint", _I), ("norm", _I),
                 ("max_iter", _I), ("chunk", _I), ("use_graph", _I), ("profile", _I),
-                ("item_blocks", _I), ("schedule", _I), ("reserved_i", _I * 2),
+                ("item_blocks", _I), ("schedule", _I), ("xcd_run", _I), ("reserved_i", _I * 1),
                 ("eps", _D), ("breakdown_tol", _D), ("inner_tol", _D), ("reserved_d", _D * 5)]
 
 
@@ -91,6 +91,7 @@ SIGNATURES = {
     "cup2d_psb_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _P]),
     "cup2d_psb_autotune": (_I, [_P, _P, _P, _P]),
     "cup2d_psb_item_blocks": (_I, [_P]),
+    "cup2d_psb_xcd_run": (_I, [_P]),
     "cup2d_psb_spmv_device": (_I, [_P, _P, _P]),
     "cup2d_psb_spmv_time": (_I, [_P, _P, _P, _I, _P]),
     "cup2d_psb_kernel_time": (_I, [_P, _I, _I, _P]),

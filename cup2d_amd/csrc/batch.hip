@@ -60,6 +60,7 @@ struct BView {
   const int4 *tile;                  // {instance, first block, blocks, 0}
   const long long *mean_row;         // [K] global row of the mean constraint or -1
   int K, n_items, n_tiles;
+  int xg_log2;                       // operator kernels' XCD run length: 2^xg_log2 items (0: plain order)
 };
 
 // Workgroup reduction of NQ quantities; thread 0 stores them at
@@ -176,22 +177,23 @@ template <int UB = 0, bool XCD = false, class F>
 __device__ __forceinline__ void running_items(const BView &V, const Scal *st, F &&body,
                                               int filter = F_RUNNING) {
   // XCD: with a grid of a multiple of 8 workgroups (workgroup g runs on XCD
-  // g % 8), the items go to the XCDs in runs of XG consecutive items, run r
-  // to XCD r % 8, so that neighbouring items (whose blocks read each other's
-  // faces) mostly share one XCD's L2 while every instance -- the last ones
-  // running included -- still spreads over all eight XCDs.  Item t of XCD x
-  // is (t / XG) 8 XG + x XG + t % XG.
-  constexpr int XG = 8;
+  // g % 8), the items go to the XCDs in runs of XG = 2^xg_log2 consecutive
+  // items, run r to XCD r % 8, so that neighbouring items (whose blocks read
+  // each other's faces) mostly share one XCD's L2 while every instance --
+  // the last ones running included -- still spreads over all eight XCDs.
+  // Item t of XCD x is (t / XG) 8 XG + x XG + t % XG; XG = 1 is the plain
+  // grid-stride order.
   const int NG = (int)gridDim.x, c = lane_id();
   const bool xm = XCD && (NG & 7) == 0;
+  const int sh = xm ? V.xg_log2 : 0, XG = 1 << sh;
   const int G = xm ? NG >> 3 : NG, x = (int)blockIdx.x & 7;
   int n_mine = V.n_items, first = blockIdx.x;
   if (xm) {
-    const int rem = V.n_items % (8 * XG);
-    n_mine = V.n_items / (8 * XG) * XG + min(XG, max(0, rem - x * XG));
+    const int rem = V.n_items & (8 * XG - 1);
+    n_mine = (V.n_items >> (sh + 3) << sh) + min(XG, max(0, rem - x * XG));
     first = (int)blockIdx.x >> 3;
   }
-  auto item_of = [&](int t) { return xm ? t / XG * (8 * XG) + x * XG + t % XG : t; };
+  auto item_of = [&](int t) { return xm ? ((t >> sh) << (sh + 3)) + (x << sh) + (t & (XG - 1)) : t; };
   for (int base = first; base < n_mine; base += 64 * G) {
     const int cnt = min(64, (n_mine - base + G - 1) / G);
     const int mine = item_of(base + c * G);
@@ -1128,6 +1130,7 @@ struct cup2d_psb {
   int K = 0;
   long long nb = 0, n = 0, nnz = 0;
   int ib = 0;                       // blocks per item
+  int xg_log2 = 3;                  // XCD run length of the operator kernels' items: 2^xg_log2 (0: plain)
   std::vector<long long> boff;      // [K+1] block offset of each instance
   std::vector<long long> eoff;      // [K+1] explicit-row offset of each instance
   std::vector<long long> h_mean_row;
@@ -1319,11 +1322,15 @@ BView bview(const cup2d_psb *h) {
   V.K = h->K;
   V.n_items = (int)h->item_inst.size();
   V.n_tiles = (int)h->tile_inst.size();
+  V.xg_log2 = h->xg_log2;
   return V;
 }
 
+// item kernels: a multiple of 8 workgroups from 8 items on (the operator
+// kernels' XCD order needs it; the grid-stride loops cover every item)
 int grid_items(const cup2d_psb *h) {
-  return (int)std::min<size_t>(BGRID, std::max<size_t>(1, h->item_inst.size()));
+  const size_t g = std::min<size_t>(BGRID, std::max<size_t>(1, h->item_inst.size()));
+  return (int)(g >= 8 ? g & ~(size_t)7 : g);
 }
 // FP64 preconditioner kernels: a workgroup per tile
 int grid_tiles64(const cup2d_psb *h) {
@@ -1679,7 +1686,7 @@ int solve_b(cup2d_psb *h, const double *d_b, const double *d_x0, double *d_out, 
 
 // Tuned item sizes of this process, by (precision, instances, blocks).
 // Handles on different threads may tune and read it at once: g_tuned_mu.
-std::map<std::tuple<int, int, long long>, int> g_tuned;
+std::map<std::tuple<int, int, long long>, std::pair<int, int>> g_tuned;  // (ib, xg_log2)
 std::mutex g_tuned_mu;
 
 bool config_ok(const cup2d_ps_config &c) {
@@ -1687,7 +1694,8 @@ bool config_ok(const cup2d_ps_config &c) {
          c.inner_tol >= 0 && c.inner_tol < 1 &&
          (c.precision == CUP2D_PS_FP64 || c.precision == CUP2D_PS_MIXED) &&
          (c.norm == CUP2D_PS_NORM_INF || c.norm == CUP2D_PS_NORM_L2) && c.item_blocks >= 0 &&
-         c.item_blocks <= MAX_ITEM_BLOCKS && c.schedule == 0 && (c.profile == 0 || c.profile == 1);
+         c.item_blocks <= MAX_ITEM_BLOCKS && c.schedule == 0 && (c.profile == 0 || c.profile == 1) &&
+         (c.xcd_run == 0 || (c.xcd_run > 0 && c.xcd_run <= 32 && (c.xcd_run & (c.xcd_run - 1)) == 0));
 }
 
 }  // namespace
@@ -1946,10 +1954,12 @@ static int finish_meshes(cup2d_psb *h, int n_inst) {
     return rc;
   h->h_st.assign(n_inst, Scal{});
   int ib = h->cfg.item_blocks;
-  if (ib <= 0) {
+  {
     std::lock_guard<std::mutex> lock(g_tuned_mu);
     auto it = g_tuned.find({(int)h->mixed, n_inst, h->nb});
-    ib = it != g_tuned.end() ? it->second : 8;
+    if (ib <= 0) ib = it != g_tuned.end() ? it->second.first : 8;
+    const int xr = h->cfg.xcd_run;
+    h->xg_log2 = xr > 0 ? __builtin_ctz((unsigned)xr) : (it != g_tuned.end() ? it->second.second : 3);
   }
   if ((rc = build_units(h, ib))) return rc;
   btrim(h);
@@ -1998,25 +2008,33 @@ static int autotune_body(cup2d_psb *h, const double *d_b, int *item_blocks, doub
   BCHK(h, hipSetDevice(h->cfg.device));
   if (d_b != h->d_b)  // the captured graph reads the handle's copy of b
     BCHK(h, hipMemcpyAsync(h->d_b, d_b, h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  // the schedule: blocks per item, and the XCD run length of the operator
+  // kernels' items (1: plain grid-stride order, or runs of 4 .. 32 items)
   static const int cand[] = {4, 8, 16, 32, 64};
-  int best = h->ib;
+  static const int xgs[] = {0, 2, 3, 4, 5};
+  int best = h->ib, best_xg = h->xg_log2;
   double best_ms = 1e300;
   for (int ib : cand) {
     int rc = build_units(h, ib);
     if (rc) return rc;
-    double ms = 0;
-    if ((rc = time_units(h, h->d_b, ms))) return rc;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = ib;
+    for (int xg : xgs) {
+      h->xg_log2 = xg;
+      double ms = 0;
+      if ((rc = time_units(h, h->d_b, ms))) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = ib;
+        best_xg = xg;
+      }
     }
   }
+  h->xg_log2 = best_xg;
   int rc = build_units(h, best);
   if (rc) return rc;
   btrim(h);
   {
     std::lock_guard<std::mutex> lock(g_tuned_mu);
-    g_tuned[{(int)h->mixed, h->K, h->nb}] = best;
+    g_tuned[{(int)h->mixed, h->K, h->nb}] = {best, best_xg};
   }
   if (item_blocks) *item_blocks = best;
   if (ms_per_iter) *ms_per_iter = best_ms;
@@ -2237,6 +2255,7 @@ int cup2d_psb_sizes(const cup2d_psb *h, long long *n, long long *nnz, long long 
 }
 
 int cup2d_psb_item_blocks(const cup2d_psb *h) { return h ? h->ib : 0; }
+int cup2d_psb_xcd_run(const cup2d_psb *h) { return h ? 1 << h->xg_log2 : 0; }
 
 const char *cup2d_psb_last_error(const cup2d_psb *h) {
   return h ? h->err.c_str() : g_berr.c_str();

@@ -448,6 +448,10 @@ class BatchSolver:
     def item_blocks(self):
         return int(_lib.load().cup2d_psb_item_blocks(self.h))
 
+    def xcd_run(self):
+        """Items per XCD run of the operator kernels (1: plain order)."""
+        return int(_lib.load().cup2d_psb_xcd_run(self.h))
+
     def sizes(self):
         n, nnz, ne = (ctypes.c_longlong() for _ in range(3))
         _check_b(self.h, _lib.load().cup2d_psb_sizes(self.h, ctypes.byref(n), ctypes.byref(nnz),

@@ -74,7 +74,10 @@ typedef struct {
                              host check of the stopping norm every iteration, the structure of the
                              reference GPU loop (BiCGSTAB.cu:412-620), FP64 single solves only: the
                              unfused comparator */
-  int reserved_i[2];
+  int xcd_run;            /* batch handles: the operator kernels take their work items in runs of
+                             xcd_run consecutive items per XCD (1 = plain grid-stride order; a power
+                             of two up to 32); 0 = the value tuned in this process, else 8 */
+  int reserved_i[1];
   double eps;             /* denominator guard; reference GPU 1e-21 (BiCGSTAB.cu:427),
                              reference CPU 1e-100 (AMRSolver.cpp:167) */
   double breakdown_tol;   /* restart when rho^2 < breakdown_tol |r|^2 |rhat|^2 (BiCGSTAB.cu:483) */
@@ -379,13 +382,17 @@ int cup2d_psb_solve(cup2d_psb *h, const double *b, double *x, double tol_abs, do
 int cup2d_psb_solve_device(cup2d_psb *h, const double *d_b, double *d_x, double tol_abs,
                            double tol_rel, int max_restarts, cup2d_ps_stats *stats);
 
-/* Times short runs of the batched iteration for several work-item sizes on
+/* Times short runs of the batched iteration for each schedule -- blocks per
+ * work item (4 .. 64) and the XCD run length of the operator kernels' items
+ * (plain order, or runs of 4 .. 32 consecutive items per XCD) -- on
  * right-hand side d_b, keeps the fastest (also for later handles of this
- * process with the same precision and batch shape) and reports it and its
- * ms per iteration.  No counterpart in the reference. */
+ * process with the same precision and batch shape) and reports its item
+ * size and ms per iteration.  No counterpart in the reference. */
 int cup2d_psb_autotune(cup2d_psb *h, const double *d_b, int *item_blocks, double *ms_per_iter);
 /* Blocks per work item in use. */
 int cup2d_psb_item_blocks(const cup2d_psb *h);
+/* XCD run length of the operator kernels' items in use (1: plain order). */
+int cup2d_psb_xcd_run(const cup2d_psb *h);
 
 /* y = A x over every instance (device pointers, concatenated vectors; the
  * mean rows with mean_constraint), and a timing of `reps` back-to-back

@@ -31,7 +31,7 @@ static const fn_t entry_points[] = {
     (fn_t)&cup2d_psb_pressure_step_device, (fn_t)&cup2d_psb_pressure_rhs_device,
     (fn_t)&cup2d_psb_pressure_correct_device, (fn_t)&cup2d_psb_offsets,
     (fn_t)&cup2d_psb_solve, (fn_t)&cup2d_psb_solve_device,
-    (fn_t)&cup2d_psb_autotune, (fn_t)&cup2d_psb_item_blocks,
+    (fn_t)&cup2d_psb_autotune, (fn_t)&cup2d_psb_item_blocks, (fn_t)&cup2d_psb_xcd_run,
     (fn_t)&cup2d_psb_spmv_device, (fn_t)&cup2d_psb_spmv_time,
     (fn_t)&cup2d_psb_kernel_time, (fn_t)&cup2d_psb_copy_time, (fn_t)&cup2d_psb_sizes,
     (fn_t)&cup2d_psb_last_error,

@@ -167,13 +167,34 @@ def test_batch_item_sizes_and_autotune(batch_meshes, batch_rhs, oracle_solutions
     bs = make_batch(batch_meshes, precision=PS.MIXED, norm=PS.NORM_L2)
     ib, ms = bs.autotune(bs.device_array(b))
     assert ib in (4, 8, 16, 32, 64) and ms > 0
-    assert bs.item_blocks() == ib
+    assert bs.item_blocks() == ib and bs.xcd_run() in (1, 4, 8, 16, 32)
     x, st = bs.solve(b, rel_tol=1e-10)
     assert all(s["converged"] for s in st)
     for k, xr in enumerate(oracle_solutions):
         assert np.linalg.norm(bs.split(x)[k] - xr) <= 1e-6 * np.linalg.norm(xr)
-    # a new handle of the same shape starts from the tuned size
-    assert make_batch(batch_meshes, precision=PS.MIXED, norm=PS.NORM_L2).item_blocks() == ib
+    # a new handle of the same shape starts from the tuned schedule
+    again = make_batch(batch_meshes, precision=PS.MIXED, norm=PS.NORM_L2)
+    assert again.item_blocks() == ib and again.xcd_run() == bs.xcd_run()
+
+
+@pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
+def test_batch_xcd_run_order_is_bitwise_neutral(batch_meshes, batch_rhs, precision):
+    """The operator kernels' item order (runs of 1 = plain, 4, 8, 32 items per
+    XCD) moves work between workgroups only: every partial sum is stored per
+    item and reduced in item order, so the solutions are bit-identical."""
+    b = np.concatenate(batch_rhs)
+    ref = None
+    for run in (1, 4, 8, 32):
+        bs = make_batch(batch_meshes, precision=precision, item_blocks=4, norm=PS.NORM_L2, xcd_run=run)
+        assert bs.xcd_run() == run
+        x, st = bs.solve(b, rel_tol=1e-10)
+        assert all(s["converged"] for s in st)
+        if ref is None:
+            ref = x
+        else:
+            assert np.array_equal(x, ref), run
+    with pytest.raises(PS._lib.SolverError):
+        make_batch(batch_meshes, xcd_run=3)
 
 
 def test_batch_rejects_bad_input(batch_meshes):

@@ -119,14 +119,18 @@ def test_pressure_step_errors():
 @pytest.mark.parametrize("mc", [0, 1])
 def test_batch_pressure_step_matches_oracle(oracle, mc):
     """The batched pressure step, each instance against the oracle's pieces
-    (as the single-mesh test), instances of different sizes in one call."""
+    (as the single-mesh test), instances of different sizes in one call.  The
+    step solves to 1e-13 here, as the oracle does: two solutions at 1e-12 of
+    b differ by up to the condition number times that, which on the 6-level
+    instance is the whole 1e-8 bar (the oracle's own OpenMP reductions move
+    its solution by ~1e-9 from run to run)."""
     meshes = [MESHES[n]() for n in ("graded", "tiny_cf", "amr6_small", "periodic")]
     bs = PS.BatchSolver(mean_constraint=mc, norm=PS.NORM_L2, max_iter=4000).set_meshes(meshes, on_device=True)
     rhs = np.concatenate([W.vortex_rhs(m, 20 + k) for k, m in enumerate(meshes)])
     p_old = np.concatenate([_p_old(m, 30 + k) for k, m in enumerate(meshes)])
     d_rhs = bs.device_array(rhs)
     d_p = bs.device_array(p_old)
-    st = bs.pressure_step_device(d_rhs, d_p, rel_tol=1e-12)
+    st = bs.pressure_step_device(d_rhs, d_p, rel_tol=1e-13)
     p = d_p.numpy()
     assert np.array_equal(d_rhs.numpy(), rhs)
     off = bs.offsets
