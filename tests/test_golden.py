@@ -54,3 +54,24 @@ def test_mean_row_fixture(oracle):
     # both mean treatments give the same mean-free pressure
     x0, x1 = d["x"], d["x_mean_row"]
     assert np.linalg.norm(x0 - x1) <= 1e-8 * np.linalg.norm(x0)
+
+
+def test_cylinder_record_fixture():
+    """The CPU run's record of the 100-step cylinder flow (make_cylinder100.py):
+    its shape, and its first time step, which from rest is calcMaxTimestep's
+    closed form with umax = |uinf| (Simulation.cpp:426-460)."""
+    from cup2d_amd import workloads as W
+
+    rec = np.load(G.GOLDEN + "/cylinder100.npz", allow_pickle=False)
+    m = W.cylinder_mesh()
+    assert int(rec["n_blocks"]) == m.n_blocks
+    assert rec["dt"].shape == (100,) and rec["iterations"].shape == (100,)
+    assert (rec["dt"] > 0).all() and (rec["iterations"] > 0).all()
+    for k in (10, 100):
+        for f in "uvp":
+            assert rec[f"{f}_mean_{k}"].shape == (m.n_blocks,)
+            assert rec[f"{f}_absmax_{k}"].shape == (m.n_blocks,)
+    h = float(m.cell_h().min())
+    umax = W.CYL_U
+    dt0 = min(0.25 * h * h / (W.CYL_NU + 0.25 * h * umax), W.CYL_CFL * (h / (umax + 1e-8)))
+    assert rec["dt"][0] == dt0
