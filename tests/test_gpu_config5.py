@@ -55,3 +55,47 @@ def test_cylinder_100_steps_match_cpu_run(precision):
 # 1e-12, and the velocity correction carries only its gradient.  The bars are
 # the velocity at 1e-10 and the pressure at the 1e-8 pressure bar of the solve.
 BARS = {"u": 1e-10, "v": 1e-10, "p": 1e-8}
+
+
+REC_ADAPT = os.path.join(os.path.dirname(__file__), "golden", "cylinder_adapt.npz")
+
+
+@pytest.mark.parametrize("precision", [PS.FP64, PS.MIXED])
+def test_adapting_cylinder_100_steps_match_cpu_run(precision):
+    """The same flow with the mesh adapted before each of the first ten steps
+    and every 20 steps after (8348 blocks down to ~540), against the CPU
+    run's record (tests/golden/make_cylinder_adapt.py: the oracle's score,
+    the tags from it, cup2d_mesh_adapt, the oracle's transfer and time step).
+    The GPU adapts with the record's tags, so both runs have the same meshes;
+    the GPU's own score gives the same tags at every adaptation."""
+    rec = np.load(REC_ADAPT)
+    m = W.cylinder_mesh()
+    assert m.n_blocks == int(rec["n_blocks_start"])
+    flow = FLOW.CylinderFlow(m, precision=precision, rel_tol=float(rec["rel_tol"]), max_iter=8000,
+                             rtol=float(rec["rtol"]), ctol=float(rec["ctol"]))
+    off, when = rec["tag_offsets"], rec["tag_steps"]
+    j = 0
+    worst = {}
+    for k, dt in enumerate(rec["dt"], start=1):
+        if j < len(when) and when[j] == k - 1:
+            tags = rec["tags"][off[j]:off[j + 1]]
+            assert tags.size == flow.mesh.n_blocks, (k, j)
+            sc = flow.solver.block_score_device(flow.u, flow.v, flow.chi, flow.rtol)
+            mine = np.where(sc > flow.rtol, 1, np.where(sc < flow.ctol, -1, 0))
+            assert np.array_equal(mine, tags), (k, int((mine != tags).sum()))
+            flow.adapt(tags=tags)
+            j += 1
+        assert flow.mesh.n_blocks == int(rec["n_blocks"][k - 1]), k
+        st = flow.step(float(dt))
+        assert st["converged"], (k, st)
+        if k in (10, 100):
+            assert np.array_equal(flow.mesh.level, rec[f"level_{k}"])
+            assert np.array_equal(flow.mesh.bx, rec[f"bx_{k}"]) and np.array_equal(flow.mesh.by, rec[f"by_{k}"])
+            for name, f in zip("uvp", flow.fields()):
+                mean, amax = _stats(f)
+                worst[(name, k, "mean")] = _rel(mean, rec[f"{name}_mean_{k}"])
+                worst[(name, k, "absmax")] = _rel(amax, rec[f"{name}_absmax_{k}"])
+    assert j == len(when)
+    print({f"{a}{k}{c}": f"{v:.2e}" for (a, k, c), v in worst.items()})
+    for (name, k, what), err in worst.items():
+        assert err <= BARS[name], (name, k, what, err)
