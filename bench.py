@@ -473,6 +473,47 @@ def kernel_only(PS, W, which, reps):
             out[name] = {"ms_per_call": ms, "GB_per_s": byt * m.n / (ms * 1e-3) / 1e9}
         print(json.dumps({"kernel": "projection", "cells": m.n, **out}))
         return
+    if which == "flow":
+        # the flow operators of the time step on a uniform 128x128-block mesh
+        # (1,048,576 cells, walls) and on the 5-level cylinder mesh (8348
+        # blocks, coarse-fine faces).  Algorithmic bytes per cell: the
+        # advection-diffusion step 160 (two stages, each: u, v in and the two
+        # right-hand sides out (32), then u0, v0 and the right-hand sides in
+        # and u, v out (48)); penalization 40 (chi, u, v in, u, v out); the
+        # largest speed 16 (u, v in); the adaptation score 24 (u, v, chi in);
+        # the transfer of one field to a refined copy of the mesh 16 per new
+        # cell (8 in, 8 out)
+        from cup2d_amd import mesh as M
+        out = {}
+        uni = M.uniform(128, 128, 1.0)
+        uni.levels = 2  # room for the refined copy of the transfer
+        for label, m in (("uniform_128x128", uni), ("cylinder_5_levels", W.cylinder_mesh())):
+            s = PS.Solver().set_mesh(m, on_device=True)
+            rng = np.random.default_rng(0)
+            x, y = m.cell_centers()
+            d_u = PS.DeviceArray.from_host(s, 0.1 * np.sin(6 * x) * np.cos(5 * y))
+            d_v = PS.DeviceArray.from_host(s, -0.1 * np.cos(6 * x) * np.sin(5 * y))
+            d_chi = PS.DeviceArray(s, m.n)
+            s.chi_disk_device(0.3, 0.5, 0.05, d_chi)
+            tags = np.ones(m.n_blocks, np.int8)
+            m2, kind, src = M.adapt(m, tags)
+            d_new = PS.DeviceArray(s, m2.n)
+            res = {"cells": m.n}
+            for name, fn, byt in (
+                    ("advect_diffuse", lambda: s.advect_diffuse_device(d_u, d_v, (0.2, 0.0), 1e-4, 1e-4), 160 * m.n),
+                    ("penalize", lambda: s.penalize_device(d_chi, -0.2, 0.0, 1e6, 1e-4, d_u, d_v), 40 * m.n),
+                    ("max_speed", lambda: s.max_speed_device(d_u, d_v, (0.2, 0.0)), 16 * m.n),
+                    ("block_score", lambda: s.block_score_device(d_u, d_v, d_chi, 5.0), 24 * m.n),
+                    ("transfer_refine_all", lambda: s.transfer_device(m, m2, kind, src, [d_u], [d_new]), 16 * m2.n)):
+                fn()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn()
+                ms = (time.perf_counter() - t0) * 1e3 / reps
+                res[name] = {"ms_per_call": ms, "algorithmic_bytes": byt, "GB_per_s": byt / (ms * 1e-3) / 1e9}
+            out[label] = res
+        print(json.dumps({"kernel": "flow", **out}))
+        return
     if which == "assembly":
         # mesh changes on the ~16k-block 6-level mesh: host assembly + upload
         # against the device-side assembly
@@ -565,7 +606,7 @@ def main():
                     help="1: replay iteration chunks from a hipGraph; 0: plain stream launches")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the sequential solves and the single-mesh workloads")
-    modes = ["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection"]
+    modes = ["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection", "flow"]
     modes += [f"{p}iter{k}" for p in ("", "x", "b", "m") for k in range(5)]
     ap.add_argument("--kernel-only", choices=modes, default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
@@ -576,7 +617,8 @@ def main():
                          "mixed iteration kernel K with every instance iterating; copy: the "
                          "library's streaming copy over 13,881,344 doubles, the bandwidth ceiling "
                          "and the traffic counters' calibration; assembly: host against device-side "
-                         "assembly of the 6-level 16k-block mesh)")
+                         "assembly of the 6-level 16k-block mesh; projection / flow: the operators "
+                         "either side of the pressure solve / the rest of the time step)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the solver has no "
