@@ -144,3 +144,13 @@ def test_block_score_rotation_and_interface():
     hit = sc2 == 10.0
     assert hit.sum() > itf.sum()  # the ghost layers reach further
     assert np.array_equal(sc2[~hit], sc[~hit])
+
+
+def test_adapt_rejects_bad_input():
+    m = M.uniform(4, 2, 1.0)
+    with pytest.raises(ValueError):  # one tag per block
+        M.adapt(m, np.zeros(m.n_blocks + 1, np.int8))
+    # the top level cannot be refined: tags there leave the block as it is
+    m2, kind, src = M.adapt(m, np.ones(m.n_blocks, np.int8))
+    assert m2.n_blocks == m.n_blocks and (kind == 0).all()
+    assert np.array_equal(src[:, 0], np.arange(m.n_blocks)) and (src[:, 1:] == -1).all()
