@@ -318,6 +318,21 @@ def bench_single(PS, W, mesh, b, steps, use_graph):
                      "true_rel2_residual": st["true_rel2"]}
         if name == "fp64":
             out["fp64"]["kernel_ms"] = [s.kernel_time(k, reps=100) for k in range(5)]
+            # the device-resident pressure step (b = rhs - L p_old, the solve
+            # of the increment from 0, the h^2-mean-free update), p_old a
+            # perturbed solution of this RHS
+            x = d_x.numpy()
+            p_old = 0.9 * x + 0.01 * np.abs(x).max() * np.sin(np.arange(mesh.n))
+            d_p = PS.DeviceArray(s, mesh.n)
+            t = []
+            for _ in range(max(1, steps // 2) + 1):
+                d_p.upload(p_old)
+                t0 = time.perf_counter()
+                sp = s.pressure_step_device(d_b, d_p, rel_tol=1e-10)
+                t.append(time.perf_counter() - t0)
+            out["fp64"]["pressure_step"] = {"ms": float(np.median(t[1:])) * 1e3,
+                                            "iterations": sp["iterations"], "converged": sp["converged"]}
+            del d_p
         del s, d_b, d_x
     out["mixed_speedup_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     out["fused_speedup_vs_unfused"] = out["fp64"]["solves_per_s"] / out["fp64_unfused"]["solves_per_s"]
