@@ -45,7 +45,12 @@ namespace {
 
 constexpr int MAXG = 512;  // workgroups of the block part of the vector kernels (grid-stride beyond)
 constexpr int MAXGE = 64;  // workgroups of the explicit-row part
-constexpr int MAXGP = 2048;  // workgroups of the FP64 preconditioner kernels (a tile each)
+// workgroups of the FP64 preconditioner kernels (grid-stride over tiles of
+// TILE blocks): a workgroup loads its P fragments and reduces the previous
+// kernel's partials once for all of its tiles, which on meshes of 1k+ tiles
+// pays more than one tile per workgroup (6-level 16k-block mesh: the p
+// update 18 -> 13 us with 512 instead of 2048; 256 alike)
+constexpr int MAXGP = 512;
 constexpr int SPMV_U = 2;  // blocks per wavefront in flight in the operator kernels
 
 // ----------------------------------------------------------------------------
