@@ -303,8 +303,9 @@ __device__ __forceinline__ void st4c(T *__restrict__ x, long base, const T (&v)[
   for (int k = 0; k < NC; k++) p[k] = join(v + W * k);
 }
 // LDS tile: chunk k of row r at tile[k * TROWS + r]; half h of a row is
-// chunks h NC / 2 .. (h + 1) NC / 2 - 1 (consecutive threads touch
-// consecutive 16-byte slots of a chunk column: no bank conflicts)
+// chunks h NC / 2 .. (h + 1) NC / 2 - 1.  The threads of a row pair write
+// chunk columns a multiple of the bank period apart (2-way conflicts); a
+// padded column stride that removes them measured no faster.
 template <class T>
 __device__ __forceinline__ void st_tile(typename Chunk<T>::type *tile, int r, int h, const T (&v)[4]) {
   constexpr int NC = Chunk<T>::n / 2, W = 4 / NC;
