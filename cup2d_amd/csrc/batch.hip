@@ -1344,9 +1344,6 @@ int grid_tiles(const cup2d_psb *h) {
   return (int)std::min<size_t>(BGRID, std::max<size_t>(1, (h->tile_inst.size() + WAVES - 1) / WAVES));
 }
 
-// Item and tile lists for `ib` blocks per item (tiles are TILE blocks; an
-// instance's last item and tile may be shorter); each instance's explicit
-// rows follow its block items as items of up to WG rows.
 // The iterate's buffers of the iteration kernels: FP64 x and x_opt, or, in
 // mixed precision, the pass's FP32 correction and its best, carved out of
 // the FP64 x_opt buffer (2n floats; x_opt is not needed there: the best
@@ -1362,6 +1359,9 @@ T *iter_xopt(const cup2d_psb *h) {
   else return reinterpret_cast<float *>(h->d_xopt) + h->n;
 }
 
+// Item and tile lists for `ib` blocks per item (tiles are TILE blocks; an
+// instance's last item and tile may be shorter); each instance's explicit
+// rows follow its block items as items of up to WG rows.
 int build_units(cup2d_psb *h, int ib) {
   drop_graph(h);
   (void)hipStreamSynchronize(h->stream);
