@@ -577,10 +577,10 @@ def kernel_only(PS, W, which, reps):
         print(json.dumps({"kernel": KERNELS[k], "ms_per_launch": ms, "launches": reps + 1,
                           "algorithmic_bytes": byt, "GB_per_s": byt / (ms * 1e-3) / 1e9}))
         return
-    if which.startswith("iter") or which.startswith("xiter"):
+    if which.startswith("iter") or which.startswith("xiter") or which.startswith("aiter"):
         # iteration kernel K of the 3-level mesh's single solve (iterK FP64,
-        # xiterK mixed)
-        mesh = W.graded_disk_mesh()
+        # xiterK mixed) or of the 6-level 16k-block mesh's (aiterK, FP64)
+        mesh = W.four_disk_amr6_mesh() if which[0] == "a" else W.graded_disk_mesh()
         mixed = which[0] == "x"
         s = PS.Solver(norm=PS.NORM_L2, precision=PS.MIXED if mixed else PS.FP64)
         s.set_mesh(mesh)
@@ -622,13 +622,13 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the sequential solves and the single-mesh workloads")
     modes = ["spmv", "batch_spmv", "batch_fp64", "batch_mixed", "copy", "assembly", "projection", "flow"]
-    modes += [f"{p}iter{k}" for p in ("", "x", "b", "m") for k in range(5)]
+    modes += [f"{p}iter{k}" for p in ("", "x", "a", "b", "m") for k in range(5)]
     ap.add_argument("--kernel-only", choices=modes, default=None,
                     help="profiling aid: launch only this kernel --steps times and exit "
                          "(spmv: operator on the 3-level mesh; batch_spmv: batched operator on the "
                          "64-instance batch; batch_fp64 / batch_mixed: 64 iterations of the "
                          "batched solve, every kernel launched from the stream; iterK: iteration "
-                         "kernel K of the 3-level mesh's solve (xiterK: mixed); biterK / miterK: batched FP64 / "
+                         "kernel K of the 3-level mesh's solve (xiterK: mixed; aiterK: FP64 on the 6-level mesh); biterK / miterK: batched FP64 / "
                          "mixed iteration kernel K with every instance iterating; copy: the "
                          "library's streaming copy over 13,881,344 doubles, the bandwidth ceiling "
                          "and the traffic counters' calibration; assembly: host against device-side "
