@@ -127,12 +127,22 @@ __device__ void load_totals(const double *__restrict__ part, int G, double (&out
   double a[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) a[q] = 0.0;
-  for (int i = threadIdx.x; i < G; i += WG) {
-    double p[NQ];  // the NQ loads of a partial are in flight together
+  // up to LR partials per thread per pass, all their loads issued before the
+  // first is added (one round trip for G <= LR * WG); added in the order of
+  // a one-at-a-time loop, so the totals do not depend on LR
+  constexpr int LR = 3;
+  for (int i0 = threadIdx.x; i0 < G; i0 += LR * WG) {
+    double p[LR][NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) p[q] = part[q * G + i];
+    for (int r = 0; r < LR; r++)
+      if (i0 + r * WG < G)
 #pragma unroll
-    for (int q = 0; q < NQ; q++) a[q] = ((MAXQ >> q) & 1) ? nmax(a[q], p[q]) : a[q] + p[q];
+        for (int q = 0; q < NQ; q++) p[r][q] = part[q * G + i0 + r * WG];
+#pragma unroll
+    for (int r = 0; r < LR; r++)
+      if (i0 + r * WG < G)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) a[q] = ((MAXQ >> q) & 1) ? nmax(a[q], p[r][q]) : a[q] + p[r][q];
   }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
